@@ -1,0 +1,113 @@
+/*
+ * mestrado — MI355X (gfx950) implementation of the all-pairs Needleman-Wunsch,
+ * pairwise/profile alignment and neighbour-joining path of ptersnow/mestrado.
+ *
+ * Thin C ABI: plain pointers and sizes, every call returns 0 on success or a
+ * negative MST_E* code; mst_last_error() gives the message of the last failure
+ * on that context. Host buffers are owned by the caller; device memory is
+ * owned by the context and lives until mst_destroy (or mst_seqset_free).
+ * A context is single-threaded: calls on one context must not overlap. Many
+ * contexts may coexist, on one or several devices.
+ */
+#ifndef MESTRADO_H
+#define MESTRADO_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MST_OK 0
+#define MST_E_ARG -1      /* invalid argument */
+#define MST_E_HIP -2      /* HIP runtime error */
+#define MST_E_NOMEM -3    /* device or host allocation failed */
+#define MST_E_STATE -4    /* object in the wrong state (unknown handle, ...) */
+
+/* Forms of the all-pairs score computation (same integer results). */
+#define MST_FORM_PLAIN 0    /* one launch per length bucket, synchronous */
+#define MST_FORM_BATCHED 1  /* persistent work-queue launches, graph-captured */
+
+typedef struct mst_ctx mst_ctx;
+
+/* Replaces the constructors AlignmentScore(gapOpen, gapExt, wSize, maxSeqLen)
+ * (AlignmentScore.cu:110-136) and AlignmentMatch(...) (AlignmentMatch.cu:82-109):
+ * gap penalties and the 24x24 substitution matrix (row-major, letter order
+ * ARNDCQEGHILKMFPSTWYVBZX*, S[row residue][column residue]; NULL selects the
+ * built-in BLOSUM62 of definitions.cuh:96-121) become context state instead of
+ * __constant__ symbols. device is the HIP device ordinal. */
+int mst_create(int gap_open, int gap_ext, const int8_t* subst576, int device,
+               mst_ctx** out);
+int mst_destroy(mst_ctx* ctx);
+const char* mst_last_error(const mst_ctx* ctx);
+/* Number of SIMD-resident waves the batched form launches (informational). */
+int mst_device_info(mst_ctx* ctx, int* num_cus, int64_t* hbm_bytes);
+
+/* Replaces AlignmentScore::loadDB/parse (AlignmentScore.cu:143-185) and
+ * AlignmentMatch::prepare (AlignmentMatch.cu:116-131): n sequences as ASCII in
+ * forward order, concatenated, with offsets[n+1] (int64). The device encodes
+ * every residue (lower case folded, unknown letters -> X) and reverses each
+ * sequence, and keeps the set resident in HBM. */
+int mst_seqset_upload(mst_ctx* ctx, const char* ascii, const int64_t* offsets,
+                      int n, int* set_id);
+int mst_seqset_free(mst_ctx* ctx, int set_id);
+/* Reversed codes back to the host (test hook for the encode step). */
+int mst_seqset_codes(mst_ctx* ctx, int set_id, int8_t* codes_out, int64_t cap);
+
+/* Replaces AlignmentScore::computeDistanceMtx (AlignmentScore.cu:187-277) and
+ * the score kernel NeedlemanWunschGlobalScoreKernel (NeedlemanWunsch.cu:22-161):
+ * every pair i <= j of the set (X = sequence i, Y = sequence j), int32 global
+ * alignment scores written to scores_nxn[i*n+j] and [j*n+i] (may be NULL) and,
+ * as float, to dist_nxn (may be NULL). */
+int mst_all_pairs(mst_ctx* ctx, int set_id, int form, int32_t* scores_nxn,
+                  float* dist_nxn);
+
+/* Explicit pair list: out[k] = score(X = seq pi[k], Y = seq pj[k]). */
+int mst_score_pairs(mst_ctx* ctx, int set_id, const int32_t* pi,
+                    const int32_t* pj, int64_t npairs, int form, int32_t* out);
+
+/* Benchmark hook: all pairs of a resident set with the result left on the
+ * device; *kernel_ms receives the device time of the score launches (HIP
+ * events on the context's stream). */
+int mst_all_pairs_resident(mst_ctx* ctx, int set_id, int form, float* kernel_ms);
+
+
+/* Replaces AlignmentMatch::computePairwise (AlignmentMatch.cu:140-317) with
+ * the kernels NeedlemanWunschGlobalMatchKernel, NeedlemanWunschGlobalBackKernel
+ * and reorderMatchesMemory (NeedlemanWunsch.cu:184-563): global alignment of
+ * X = sequence pi[k] (columns) against Y = sequence pj[k] (rows) with the
+ * reference's traceback and tie rules. scores (may be NULL) gets the int32
+ * scores; gapped_x / gapped_y (may be NULL) get NUL-terminated gapped strings,
+ * pair k at offset k*stride (stride >= len(X)+len(Y)+1), residues printed as
+ * the letter of their code (revConvConst, definitions.cuh:134-137). */
+int mst_align_pairs(mst_ctx* ctx, int set_id, const int32_t* pi, const int32_t* pj,
+                    int64_t npairs, int32_t* scores, char* gapped_x, char* gapped_y,
+                    int64_t stride);
+
+/* One join of the guide tree (TreeNode::join, TreeNode.cpp:30-58). */
+typedef struct {
+  int node;             /* id of the new node (n0, n0+1, ...) */
+  int left, right;      /* joined node ids, left < right */
+  float d;              /* D(left, right) */
+  double Ti, Tj;        /* row sums T of left and right at the join */
+  int n;                /* active taxa before the join */
+  double q;             /* the minimal q */
+  double len_left;      /* branch lengths as TreeNode::join computes them */
+  double len_right;
+} mst_join;
+
+/* Replaces SpeciesBlock::buildTree (SpeciesBlock.cpp:844-1343) on one device:
+ * neighbour joining of the n x n matrix D (row-major fp32, symmetric, the
+ * diagonal holds the self scores the reference keeps there). joins receives
+ * the n-1 joins in order; rebuild_ratio is CmdLineOptions -r (default 0.5). */
+int mst_nj(mst_ctx* ctx, const float* D_nxn, int n, float rebuild_ratio, mst_join* joins);
+
+/* NJ on the score matrix of the set's last mst_all_pairs call, left resident
+ * on the device (d = (float)score, SpeciesBlock.cpp:264 via AlignmentScore). */
+int mst_nj_scores(mst_ctx* ctx, int set_id, float rebuild_ratio, mst_join* joins);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

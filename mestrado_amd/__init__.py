@@ -1,0 +1,214 @@
+"""mestrado_amd — MI355X (gfx950) all-pairs Needleman-Wunsch, alignment with
+traceback and neighbour joining, behind the C ABI of include/mestrado.h.
+
+The native library (libmestrado.so, built in-tree by mestrado_amd.build) is the
+only compute path: importing this package on a machine where the library is
+missing raises, and there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libmestrado.so")
+
+FORM_PLAIN = 0
+FORM_BATCHED = 1
+
+
+class MestradoError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MestradoError("libmestrado.so is not built (run python -m mestrado_amd.build)")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.mst_create.argtypes = [C.c_int, C.c_int, P, C.c_int, C.POINTER(P)]
+        L.mst_destroy.argtypes = [P]
+        L.mst_last_error.argtypes = [P]
+        L.mst_last_error.restype = C.c_char_p
+        L.mst_device_info.argtypes = [P, C.POINTER(C.c_int), C.POINTER(C.c_int64)]
+        L.mst_seqset_upload.argtypes = [P, P, P, C.c_int, C.POINTER(C.c_int)]
+        L.mst_seqset_free.argtypes = [P, C.c_int]
+        L.mst_seqset_codes.argtypes = [P, C.c_int, P, C.c_int64]
+        L.mst_all_pairs.argtypes = [P, C.c_int, C.c_int, P, P]
+        L.mst_score_pairs.argtypes = [P, C.c_int, P, P, C.c_int64, C.c_int, P]
+        L.mst_all_pairs_resident.argtypes = [P, C.c_int, C.c_int, C.POINTER(C.c_float)]
+        L.mst_align_pairs.argtypes = [P, C.c_int, P, P, C.c_int64, P, P, P, C.c_int64]
+        L.mst_nj.argtypes = [P, P, C.c_int, C.c_float, P]
+        L.mst_nj_scores.argtypes = [P, C.c_int, C.c_float, P]
+        L.mst_synth.argtypes = [C.c_int, C.c_uint64, C.c_int, P, C.c_int64, P, C.c_int64,
+                                C.POINTER(C.c_int), C.POINTER(C.c_int64)]
+        L.mst_synth_job_sizes.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_double, P]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def synth(config, seed=0, n=0):
+    """Synthetic set of benchmark configuration `config` -> list of str."""
+    L = lib()
+    cnt = C.c_int(0)
+    tot = C.c_int64(0)
+    if L.mst_synth(config, seed, n, None, 0, None, 0, C.byref(cnt), C.byref(tot)) != 0:
+        raise MestradoError("bad synthetic configuration %r" % config)
+    buf = np.zeros(max(1, tot.value), dtype=np.uint8)
+    off = np.zeros(cnt.value + 1, dtype=np.int64)
+    rc = L.mst_synth(config, seed, n, _p(buf), len(buf), _p(off), len(off), C.byref(cnt), C.byref(tot))
+    if rc != 0:
+        raise MestradoError("mst_synth failed (%d)" % rc)
+    raw = buf.tobytes()
+    return [raw[off[i]:off[i + 1]].decode() for i in range(cnt.value)]
+
+
+def job_sizes(seed, njobs, lo=8, hi=512, exponent=1.5):
+    out = np.zeros(njobs, dtype=np.int32)
+    lib().mst_synth_job_sizes(seed, njobs, lo, hi, exponent, _p(out))
+    return out
+
+
+def pack(seqs):
+    """list of str -> (uint8 ASCII buffer, int64 offsets[n+1])."""
+    off = np.zeros(len(seqs) + 1, dtype=np.int64)
+    for i, s in enumerate(seqs):
+        off[i + 1] = off[i] + len(s)
+    buf = np.frombuffer("".join(seqs).encode(), dtype=np.uint8).copy()
+    if buf.size == 0:
+        buf = np.zeros(1, dtype=np.uint8)
+    return buf, off
+
+
+class Join(C.Structure):
+    """mst_join (include/mestrado.h)."""
+    _fields_ = [("node", C.c_int), ("left", C.c_int), ("right", C.c_int), ("d", C.c_float),
+                ("Ti", C.c_double), ("Tj", C.c_double), ("n", C.c_int), ("q", C.c_double),
+                ("len_left", C.c_double), ("len_right", C.c_double)]
+
+
+class Context:
+    """One device context (mst_create/mst_destroy)."""
+
+    def __init__(self, gap_open=10, gap_ext=2, subst=None, device=0):
+        L = lib()
+        self._L = L
+        h = C.c_void_p()
+        s = None
+        if subst is not None:
+            s = np.ascontiguousarray(subst, dtype=np.int8).reshape(576)
+            self._subst = s
+        rc = L.mst_create(gap_open, gap_ext, _p(s), device, C.byref(h))
+        if rc != 0:
+            raise MestradoError("mst_create failed (%d)" % rc)
+        self.h = h
+
+    def _ck(self, rc, what):
+        if rc != 0:
+            msg = self._L.mst_last_error(self.h)
+            raise MestradoError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    def close(self):
+        if self.h:
+            self._L.mst_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def device_info(self):
+        cus = C.c_int(0)
+        hbm = C.c_int64(0)
+        self._ck(self._L.mst_device_info(self.h, C.byref(cus), C.byref(hbm)), "mst_device_info")
+        return cus.value, hbm.value
+
+    def nj(self, D, ratio=0.5):
+        """Neighbour joining of an n x n float matrix -> list of Join."""
+        D = np.ascontiguousarray(D, dtype=np.float32)
+        n = D.shape[0]
+        joins = (Join * max(1, n - 1))()
+        self._ck(self._L.mst_nj(self.h, _p(D), n, ratio, joins), "mst_nj")
+        return list(joins)[: n - 1]
+
+    def upload(self, seqs):
+        buf, off = pack(seqs)
+        sid = C.c_int(0)
+        self._ck(self._L.mst_seqset_upload(self.h, _p(buf), _p(off), len(seqs), C.byref(sid)),
+                 "mst_seqset_upload")
+        return SeqSet(self, sid.value, len(seqs), int(off[-1]))
+
+
+class SeqSet:
+    def __init__(self, ctx, sid, n, total):
+        self.ctx, self.id, self.n, self.total = ctx, sid, n, total
+
+    def free(self):
+        self.ctx._ck(self.ctx._L.mst_seqset_free(self.ctx.h, self.id), "mst_seqset_free")
+
+    def codes(self):
+        out = np.zeros(max(1, self.total), dtype=np.int8)
+        self.ctx._ck(self.ctx._L.mst_seqset_codes(self.ctx.h, self.id, _p(out), len(out)),
+                     "mst_seqset_codes")
+        return out[: self.total]
+
+    def all_pairs(self, form=FORM_PLAIN, want_dist=False):
+        sc = np.zeros((self.n, self.n), dtype=np.int32)
+        dist = np.zeros((self.n, self.n), dtype=np.float32) if want_dist else None
+        self.ctx._ck(self.ctx._L.mst_all_pairs(self.ctx.h, self.id, form, _p(sc), _p(dist)),
+                     "mst_all_pairs")
+        return (sc, dist) if want_dist else sc
+
+    def score_pairs(self, pi, pj, form=FORM_PLAIN):
+        pi = np.ascontiguousarray(pi, dtype=np.int32)
+        pj = np.ascontiguousarray(pj, dtype=np.int32)
+        out = np.zeros(len(pi), dtype=np.int32)
+        self.ctx._ck(self.ctx._L.mst_score_pairs(self.ctx.h, self.id, _p(pi), _p(pj), len(pi), form,
+                                                 _p(out)), "mst_score_pairs")
+        return out
+
+    def align_pairs(self, pi, pj, lengths):
+        """Global alignments with traceback -> (scores, [gapped_x], [gapped_y]).
+        lengths: the set's sequence lengths (for the output stride)."""
+        pi = np.ascontiguousarray(pi, dtype=np.int32)
+        pj = np.ascontiguousarray(pj, dtype=np.int32)
+        np_ = len(pi)
+        stride = int(max([lengths[a] + lengths[b] for a, b in zip(pi, pj)] + [0])) + 1
+        sc = np.zeros(np_, dtype=np.int32)
+        gx = C.create_string_buffer(max(1, np_ * stride))
+        gy = C.create_string_buffer(max(1, np_ * stride))
+        self.ctx._ck(self.ctx._L.mst_align_pairs(self.ctx.h, self.id, _p(pi), _p(pj), np_, _p(sc),
+                                                 gx, gy, stride), "mst_align_pairs")
+        rx, ry = gx.raw, gy.raw
+        outx = [rx[k * stride: rx.index(b"\0", k * stride)].decode() for k in range(np_)]
+        outy = [ry[k * stride: ry.index(b"\0", k * stride)].decode() for k in range(np_)]
+        return sc, outx, outy
+
+    def nj(self, ratio=0.5):
+        """NJ on this set's resident score matrix (after all_pairs)."""
+        joins = (Join * max(1, self.n - 1))()
+        self.ctx._ck(self.ctx._L.mst_nj_scores(self.ctx.h, self.id, ratio, joins), "mst_nj_scores")
+        return list(joins)[: self.n - 1]
+
+    def all_pairs_resident(self, form=FORM_BATCHED):
+        ms = C.c_float(0)
+        self.ctx._ck(self.ctx._L.mst_all_pairs_resident(self.ctx.h, self.id, form, C.byref(ms)),
+                     "mst_all_pairs_resident")
+        return ms.value

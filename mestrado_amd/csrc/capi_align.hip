@@ -1,0 +1,184 @@
+// C ABI of pairwise alignment with traceback (include/mestrado.h:
+// mst_align_pairs), the path of AlignmentMatch::computePairwise
+// (AlignmentMatch.cu:140-317) made whole.
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mestrado.h"
+#include "ctx.h"
+#include "nw_align.h"
+
+using namespace mst;
+
+#define CK(call)                                                                      \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess) return ctx->fail(MST_E_HIP, #call, hipGetErrorString(e_)); \
+  } while (0)
+
+static const char kLetters[] = "ARNDCQEGHILKMFPSTWYVBZX*";
+
+// Runs the forward pass and traceback of `jobs` (already sized) and returns
+// the paths on the host. Jobs are processed in batches whose flag blocks fit
+// the context's traceback budget.
+int run_align_jobs(mst_ctx* ctx, int src, const int8_t* d_seqs, const uint8_t* d_tab,
+                   std::vector<AlignJob>& jobs, std::vector<int32_t>& scores,
+                   std::vector<std::vector<uint8_t>>& paths) {
+  const int64_t nj = (int64_t)jobs.size();
+  scores.assign(nj, 0);
+  paths.assign(nj, {});
+  int64_t k0 = 0;
+  const int64_t budget_words = ctx->flag_budget_bytes / 4;
+  while (k0 < nj) {
+    // batch [k0, k1)
+    int64_t words = 0, opsb = 0;
+    int maxn = 1;
+    int64_t k1 = k0;
+    while (k1 < nj) {
+      int64_t w = flag_words(jobs[k1].n, jobs[k1].m);
+      if (k1 > k0 && words + w > budget_words) break;
+      jobs[k1].flag_off = words;
+      jobs[k1].ops_off = opsb;
+      words += w;
+      opsb += jobs[k1].n + jobs[k1].m;
+      maxn = std::max(maxn, jobs[k1].n);
+      ++k1;
+    }
+    const int64_t nb = k1 - k0;
+    int rc = ctx->ensure((void**)&ctx->d_flags, &ctx->flags_cap, sizeof(uint32_t) * std::max<int64_t>(words, 1));
+    if (rc) return rc;
+    rc = ctx->ensure((void**)&ctx->d_ops, &ctx->ops_cap, std::max<int64_t>(opsb, 1));
+    if (rc) return rc;
+    rc = ctx->ensure((void**)&ctx->d_jobs, &ctx->jobs_cap, sizeof(AlignJob) * nb);
+    if (rc) return rc;
+    rc = ctx->ensure((void**)&ctx->d_jscore, &ctx->jscore_cap, sizeof(int32_t) * 2 * nb);
+    if (rc) return rc;
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ctx->num_cus * 16, nb));
+    const int64_t bufcap = maxn;
+    rc = ctx->ensure((void**)&ctx->d_bufs, &ctx->bufs_cap, sizeof(int2) * 2 * bufcap * (size_t)grid);
+    if (rc) return rc;
+    CK(hipMemcpyAsync(ctx->d_jobs, jobs.data() + k0, sizeof(AlignJob) * nb, hipMemcpyHostToDevice,
+                      ctx->stream));
+    AlignArgs a;
+    a.jobs = (const AlignJob*)ctx->d_jobs;
+    a.first = 0;
+    a.last = nb;
+    a.seqs = d_seqs;
+    a.tab = d_tab;
+    a.subst = ctx->d_S;
+    a.flags = (uint32_t*)ctx->d_flags;
+    a.scores = (int32_t*)ctx->d_jscore;
+    a.bufs = ctx->d_bufs;
+    a.bufcap = bufcap;
+    a.go = ctx->go;
+    a.ge = ctx->ge;
+    launch_align(src, a, grid, ctx->stream);
+    CK(hipGetLastError());
+    TraceArgs t;
+    t.jobs = (const AlignJob*)ctx->d_jobs;
+    t.first = 0;
+    t.last = nb;
+    t.flags = (const uint32_t*)ctx->d_flags;
+    t.ops = (uint8_t*)ctx->d_ops;
+    t.lens = (int32_t*)ctx->d_jscore + nb;
+    launch_traceback(t, ctx->stream);
+    CK(hipGetLastError());
+    std::vector<int32_t> sl(2 * nb);
+    std::vector<uint8_t> ops(opsb);
+    CK(hipMemcpyAsync(sl.data(), ctx->d_jscore, sizeof(int32_t) * 2 * nb, hipMemcpyDeviceToHost,
+                      ctx->stream));
+    if (opsb) CK(hipMemcpyAsync(ops.data(), ctx->d_ops, opsb, hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    for (int64_t k = 0; k < nb; ++k) {
+      const AlignJob& jb = jobs[k0 + k];
+      scores[k0 + k] = sl[k];
+      int len = sl[nb + k];
+      paths[k0 + k].assign(ops.begin() + jb.ops_off, ops.begin() + jb.ops_off + len);
+    }
+    k0 = k1;
+  }
+  return MST_OK;
+}
+
+extern "C" int mst_align_pairs(mst_ctx* ctx, int id, const int32_t* pi, const int32_t* pj,
+                               int64_t np, int32_t* scores, char* gx, char* gy, int64_t stride) {
+  if (!ctx || np < 0 || (np > 0 && (!pi || !pj))) return MST_E_ARG;
+  auto it = ctx->sets.find(id);
+  if (it == ctx->sets.end()) return ctx->fail(MST_E_STATE, "unknown sequence set");
+  const SeqSet& s = it->second;
+  if (np == 0) return MST_OK;
+  CK(hipSetDevice(ctx->device));
+  std::vector<AlignJob> jobs;
+  std::vector<int64_t> where;  // caller index of each device job
+  std::vector<int32_t> out_scores(np, 0);
+  std::vector<std::vector<uint8_t>> out_paths(np);
+  for (int64_t k = 0; k < np; ++k) {
+    const int a = pi[k], b = pj[k];
+    if (a < 0 || b < 0 || a >= s.n || b >= s.n) return ctx->fail(MST_E_ARG, "pair index out of range");
+    const int n = (int)(s.h_off[a + 1] - s.h_off[a]), m = (int)(s.h_off[b + 1] - s.h_off[b]);
+    if (gx && stride < (int64_t)n + m + 1) return ctx->fail(MST_E_ARG, "stride too small");
+    if (n == 0 || m == 0) {  // all gaps; no DP
+      out_scores[k] = -ctx->ge * (n + m);
+      out_paths[k].assign(n + m, n == 0 ? (uint8_t)OP_UP : (uint8_t)OP_LEFT);
+      continue;
+    }
+    AlignJob j;
+    j.n = n;
+    j.m = m;
+    j.x_off = s.h_off[a];
+    j.y_off = s.h_off[b];
+    j.tab_off = 0;
+    j.flag_off = 0;
+    j.ops_off = 0;
+    jobs.push_back(j);
+    where.push_back(k);
+  }
+  std::vector<int32_t> sc;
+  std::vector<std::vector<uint8_t>> paths;
+  int rc = run_align_jobs(ctx, SRC_SEQ, s.d_codes, nullptr, jobs, sc, paths);
+  if (rc) return rc;
+  for (size_t t = 0; t < jobs.size(); ++t) {
+    out_scores[where[t]] = sc[t];
+    out_paths[where[t]].swap(paths[t]);
+  }
+  for (int64_t k = 0; k < np; ++k) {
+    if (scores) scores[k] = out_scores[k];
+    if (!gx && !gy) continue;
+    const int a = pi[k], b = pj[k];
+    const char* X = s.h_canon.data() + s.h_off[a];
+    const char* Y = s.h_canon.data() + s.h_off[b];
+    int x = 0, y = 0, t = 0;
+    char* ox = gx ? gx + k * stride : nullptr;
+    char* oy = gy ? gy + k * stride : nullptr;
+    for (uint8_t op : out_paths[k]) {
+      char cx = '-', cy = '-';
+      if (op != OP_UP) cx = X[x++];
+      if (op != OP_LEFT) cy = Y[y++];
+      if (ox) ox[t] = cx;
+      if (oy) oy[t] = cy;
+      ++t;
+    }
+    if (ox) ox[t] = 0;
+    if (oy) oy[t] = 0;
+  }
+  return MST_OK;
+}
+
+std::string canonical_letters(const char* ascii, int64_t n) {
+  static int8_t table[256];
+  static bool init = false;
+  if (!init) {
+    for (int i = 0; i < 256; ++i) table[i] = 22;
+    for (int k = 0; k < 24; ++k) {
+      unsigned char c = (unsigned char)kLetters[k];
+      table[c] = (int8_t)k;
+      if (c >= 'A' && c <= 'Z') table[c - 'A' + 'a'] = (int8_t)k;
+    }
+    init = true;
+  }
+  std::string out(n, 'X');
+  for (int64_t i = 0; i < n; ++i) out[i] = kLetters[table[(unsigned char)ascii[i]]];
+  return out;
+}

@@ -1,0 +1,31 @@
+// Shared definitions of the HIP kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define MST_NEG_INF (-(1 << 29))
+#define MST_LETTERS 24
+#define MST_WAVE 64
+
+// DPP controls of the GFX9 family (wave-wide shift by one lane).
+#define DPP_WAVE_SHR1 0x138
+
+namespace mst {
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// v_mov_b32_dpp wave_shr:1 with bound_ctrl off: lane 0 keeps `old`.
+__device__ __forceinline__ int shr1(int old, int v) {
+  return __builtin_amdgcn_update_dpp(old, v, DPP_WAVE_SHR1, 0xF, 0xF, false);
+}
+
+__device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
+__device__ __forceinline__ int imax3(int a, int b, int c) { return imax(imax(a, b), c); }
+
+// Agent-scope fence: orders this wave's global stores before later loads of
+// the same addresses by any wave (used between strips and between steps).
+__device__ __forceinline__ void agent_fence() {
+  __atomic_thread_fence(__ATOMIC_SEQ_CST);
+}
+
+}  // namespace mst

@@ -1,0 +1,85 @@
+// Context state behind the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+struct SeqSet {
+  int n = 0;
+  int64_t total = 0;
+  int maxlen = 0;
+  int8_t* d_codes = nullptr;  // reversed codes, concatenated
+  int64_t* d_off = nullptr;   // n+1 offsets
+  std::vector<int64_t> h_off;
+  std::string h_canon;  // forward letters as the device codes print them
+};
+
+std::string canonical_letters(const char* ascii, int64_t n);
+
+struct Bucket {
+  int K;
+  int64_t begin, end;
+};
+
+// A bucketed pair list (pairs sorted by bucket, longest first in a bucket).
+struct PairPlan {
+  bool built = false, uploaded = false;
+  std::vector<int2> pairs;
+  std::vector<int64_t> index;  // plan position -> caller's pair index
+  std::vector<Bucket> buckets;
+  int64_t cells = 0;
+};
+
+struct mst_ctx {
+  int device = 0;
+  int num_cus = 0;
+  int64_t hbm_bytes = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int go = 10, ge = 2;
+  int8_t S[576];
+  bool symmetric = true;
+  int8_t* d_S = nullptr;
+  std::string err;
+  std::map<int, SeqSet> sets;
+  std::map<int, PairPlan> plans;  // all-pairs plan per set
+  int next_set = 1;
+  int resident_plan = -1;
+
+  int2* d_pairs = nullptr;
+  size_t pairs_cap = 0;
+  int2* d_bufs = nullptr;
+  size_t bufs_cap = 0;
+  int32_t* d_mat = nullptr;
+  size_t mat_cap = 0;
+  int32_t* d_pscore = nullptr;
+  size_t pscore_cap = 0;
+  int mat_set = -1;  // set whose all-pairs matrix d_mat holds
+
+  float* d_nj = nullptr;
+  size_t nj_cap = 0;
+  double* d_nj_T = nullptr;
+  size_t nj_T_cap = 0;
+  int* d_nj_act = nullptr;
+  size_t nj_act_cap = 0;
+  void* d_nj_joins = nullptr;
+  size_t nj_joins_cap = 0;
+
+  // traceback working set
+  int64_t flag_budget_bytes = (int64_t)4 << 30;
+  void* d_flags = nullptr;
+  size_t flags_cap = 0;
+  void* d_ops = nullptr;
+  size_t ops_cap = 0;
+  void* d_jobs = nullptr;
+  size_t jobs_cap = 0;
+  void* d_jscore = nullptr;
+  size_t jscore_cap = 0;
+
+  ~mst_ctx();
+  int fail(int code, const char* what, const char* detail = nullptr);
+  int ensure(void** p, size_t* cap, size_t bytes);
+};

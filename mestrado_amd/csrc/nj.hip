@@ -1,0 +1,217 @@
+// Neighbour joining on the device (R5): SpeciesBlock::buildTree
+// (SpeciesBlock.cpp:844-1343) with the arithmetic of buildGblRowSums (:426-453)
+// and modifyMatrix (:710-842), one process, no MPI.
+//
+// Selection is exact: every step scans the whole upper triangle of the active
+// pairs for the minimum q, ties broken towards the larger max(i,j), then the
+// larger min(i,j) (:1169-1178). That is the pair the reference's bin/heap
+// search returns, since its pruning bounds (:1083-1127) only skip pairs whose
+// q is provably larger. q is evaluated as ((double)d*(n-2) - T_a) - T_b with
+// (a,b) in the order the reference's candidate list holds the pair: node-id
+// order for pairs created by the last bin rebuild, storage-index order for
+// pairs created by the insertion of a newer node (:1280-1287).
+//
+// Arithmetic that reaches the output follows the reference bit for bit:
+//   T_i   = sum over active j in id order of D(i,j) in fp64, diagonal included
+//   tmp_k = ((D_ik + D_jk) - d) / 2 in fp32                          (:795,811)
+//   T_k  += (double)(float)(tmp_k - (D_ik + D_jk))                    (:801,814)
+//   T_new = sum over active k in id order of tmp_k in fp64            (:799,818)
+// and T is rebuilt from D at the reference's schedule (:864,1225-1247). The new
+// node takes the storage row of the smaller id and keeps its stale diagonal.
+//
+// Layout: D is an n0 x n0 fp32 matrix in HBM indexed by storage row (n0 <=
+// 16384 here keeps it in the MALL); the active nodes are a compact array in id
+// order. One workgroup of 1024 threads runs every step of the loop inside one
+// launch, so the n-1 steps cost no launches or host round trips.
+#include <float.h>
+#include <limits.h>
+
+#include "common.h"
+#include "nj.h"
+
+namespace mst {
+
+constexpr int NJ_THREADS = 1024;
+
+struct Best {
+  double q;
+  int J, I;  // node ids, I < J
+  int pa, pb;  // positions in the active array
+};
+
+__device__ __forceinline__ bool better(const Best& a, const Best& b) {
+  if (a.q < b.q) return true;
+  if (a.q > b.q) return false;
+  if (a.J != b.J) return a.J > b.J;
+  return a.I > b.I;
+}
+
+__global__ void __launch_bounds__(NJ_THREADS) nj_kernel(NjArgs a) {
+  __shared__ Best red[NJ_THREADS / 64];
+  __shared__ Best win;
+  __shared__ int s_m, s_R, s_steps, s_next;
+  const int tid = threadIdx.x;
+  const int n0 = a.n0;
+  float* D = a.D;
+  double* T = a.T;
+  int* act = a.act;   // storage rows of the active nodes, id order
+  int* ids = a.ids;   // node ids of the active nodes
+  int* act2 = a.act + n0;  // ping-pong halves
+  int* ids2 = a.ids + n0;
+  const int ld = a.ld;
+
+  // initial row sums (buildGblRowSums): one thread per row, id order; D is
+  // symmetric so thread r reads column r, coalesced across threads
+  for (int r = tid; r < n0; r += NJ_THREADS) {
+    double s = 0.0;
+    for (int j = 0; j < n0; ++j) s += (double)D[(int64_t)j * ld + r];
+    T[r] = s;
+  }
+  if (tid == 0) {
+    s_m = n0;
+    s_R = n0;
+    s_next = n0;
+    int st = (int)((float)n0 * a.ratio);
+    s_steps = st > 200 ? st : 200;
+  }
+  __syncthreads();
+
+  for (int step = 0; step < n0 - 1; ++step) {
+    const int m = s_m, R = s_R;
+    const double nm2 = (double)(m - 2);
+    // ---- scan of the upper triangle (positions a < b) ----
+    Best best;
+    best.q = DBL_MAX;
+    best.J = -1;
+    best.I = -1;
+    best.pa = best.pb = -1;
+    // one wave per row pa (rows dealt round robin), lanes stride the columns
+    const int wave = tid >> 6, lane = tid & 63;
+    for (int pa = wave; pa < m - 1; pa += NJ_THREADS / 64) {
+      const int ra = act[pa], ia = ids[pa];
+      const double Ta = T[ra];
+      const float* Drow = D + (int64_t)ra * ld;
+      for (int pb = pa + 1 + lane; pb < m; pb += 64) {
+        const int rb = act[pb], ib = ids[pb];
+        const double dq = (double)Drow[rb] * nm2;
+        // pair created by an insertion after the last rebuild: storage order
+        const bool swap = (ib >= R) && (rb < ra);
+        const double Tb = T[rb];
+        const double q = swap ? (dq - Tb) - Ta : (dq - Ta) - Tb;
+        Best c;
+        c.q = q;
+        c.I = ia;
+        c.J = ib;
+        c.pa = pa;
+        c.pb = pb;
+        if (better(c, best)) best = c;
+      }
+    }
+    // wave reduction then workgroup reduction
+    for (int off = 32; off > 0; off >>= 1) {
+      Best o;
+      o.q = __shfl_down(best.q, off);
+      o.J = __shfl_down(best.J, off);
+      o.I = __shfl_down(best.I, off);
+      o.pa = __shfl_down(best.pa, off);
+      o.pb = __shfl_down(best.pb, off);
+      if (better(o, best)) best = o;
+    }
+    if ((tid & 63) == 0) red[tid >> 6] = best;
+    __syncthreads();
+    if (tid == 0) {
+      Best b = red[0];
+      for (int w = 1; w < NJ_THREADS / 64; ++w)
+        if (better(red[w], b)) b = red[w];
+      win = b;
+    }
+    __syncthreads();
+    const Best w = win;
+    const int ra = act[w.pa], rb = act[w.pb];
+    const float d = D[(int64_t)ra * ld + rb];
+    if (tid == 0) {
+      NjJoin& J = a.joins[step];
+      J.node = s_next;
+      J.left = w.I;
+      J.right = w.J;
+      J.d = d;
+      J.Ti = T[ra];
+      J.Tj = T[rb];
+      J.n = m;
+      J.q = w.q;
+    }
+    __syncthreads();
+    // ---- new row (modifyMatrix) into storage row ra ----
+    for (int p = tid; p < m; p += NJ_THREADS) {
+      if (p == w.pa || p == w.pb) continue;
+      const int rk = act[p];
+      const float Dik = D[(int64_t)ra * ld + rk];
+      const float Djk = D[(int64_t)rb * ld + rk];
+      const float tmp = ((Dik + Djk) - d) / 2;
+      const float dlt = tmp - (Dik + Djk);
+      T[rk] += (double)dlt;
+      D[(int64_t)ra * ld + rk] = tmp;
+      D[(int64_t)rk * ld + ra] = tmp;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double s = 0.0;
+      for (int p = 0; p < m; ++p) {
+        if (p == w.pa || p == w.pb) continue;
+        s += (double)D[(int64_t)ra * ld + act[p]];
+      }
+      T[ra] = s;
+    }
+    __syncthreads();
+    // ---- compact the active array: drop pa, pb, append the new node ----
+    for (int p = tid; p < m; p += NJ_THREADS) {
+      if (p == w.pa || p == w.pb) continue;
+      const int np = p - (p > w.pa) - (p > w.pb);
+      act2[np] = act[p];
+      ids2[np] = ids[p];
+    }
+    if (tid == 0) {
+      act2[m - 2] = ra;
+      ids2[m - 2] = s_next;
+    }
+    __syncthreads();
+    {
+      int* t = act;
+      act = act2;
+      act2 = t;
+      t = ids;
+      ids = ids2;
+      ids2 = t;
+    }
+    // ---- step bookkeeping and the rebuild schedule ----
+    const int m1 = m - 1;
+    const bool rebuild = (s_steps == 0);
+    if (rebuild) {
+      for (int p = tid; p < m1; p += NJ_THREADS) {
+        const int r = act[p];
+        double s = 0.0;
+        for (int j = 0; j < m1; ++j) s += (double)D[(int64_t)act[j] * ld + r];
+        T[r] = s;
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      if (rebuild) {
+        s_steps = (m1 < 200) ? INT_MAX : (int)((float)m1 * a.ratio);
+        s_R = s_next + 1;
+      } else {
+        s_steps -= 1;
+      }
+      s_m = m1;
+      s_next += 1;
+    }
+    __syncthreads();
+  }
+}
+
+hipError_t launch_nj(const NjArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(nj_kernel, dim3(1), dim3(NJ_THREADS), 0, st, a);
+  return hipGetLastError();
+}
+
+}  // namespace mst

@@ -1,0 +1,337 @@
+// Alignment with traceback (R2 + R3): NeedlemanWunschGlobalMatchKernel
+// (NeedlemanWunsch.cu:184-365) and NeedlemanWunschGlobalBackKernel (:393-494).
+//
+// The forward pass is the score kernel's wavefront (one pair per wave, lane l
+// owns K = 8 consecutive rows of a 512-row strip, columns skewed by one step
+// per lane) with the reference's per-cell nibble added:
+//   bit3 LEFT  = (H==E && H!=F) || H==D     bit1 contUp   = F came from F-ge
+//   bit2 UP    = (H==F) || H==D             bit0 contLeft = E came from E-ge
+// (diagonal before up before left; extension preferred on a tie). In the
+// biased max-plus form of the score kernel these become two compares per
+// direction and one per continuation flag:
+//   eqD = D >= max(E,F), fe = F >= E, UP = eqD|fe, LEFT = eqD|!fe,
+//   contUp = Fin >= Hm_up, contLeft = E_left >= Hm_left.
+// Unlike the score kernel, E at column -1 and F at row -1 are a true -inf so
+// that the continuation flags of the first row and column are 0, as with the
+// reference's SHORT_MIN+gapEx sentinel.
+//
+// Flag layout: each lane packs its K=8 nibbles of one step into one dword
+// (row i of the lane at bits 4i..4i+3) and the wave stores the 64 dwords of a
+// step contiguously: flags[(strip*(n+63) + t)*64 + lane]. A step is one
+// coalesced 256-byte store, 0.5 byte per cell. The traceback reads single
+// nibbles at random and walks the reference's state machine (:409-482) from
+// the last cell of the reversed inputs, i.e. from the first residues of the
+// forward sequences, emitting the path in forward order. Going up into row -1
+// continues left along row -1 to the corner (the reference stops there early).
+//
+// Score sources: SRC_SEQ looks the substitution scores up in a per-wave LDS
+// query profile (pairwise alignment); SRC_TAB reads them from a precomputed
+// byte table in the same skewed order (profile-profile alignment).
+#include "common.h"
+#include "nw_align.h"
+
+namespace mst {
+
+constexpr int AK = 8;                 // rows per lane
+constexpr int ASH = MST_WAVE * AK;    // rows per strip
+constexpr int AU = 8;                 // unrolled steps / prefetch distance
+
+__device__ __forceinline__ int shr1_const(int v, int c) {
+  return __builtin_amdgcn_update_dpp(c, v, DPP_WAVE_SHR1, 0xF, 0xF, false);
+}
+
+struct ALane {
+  int Hm[AK], E[AK];
+  int Fout, Hout, hdiag;
+};
+
+__device__ __forceinline__ void build_profile8(uint8_t* prof, const int8_t* S_lds, const int8_t* Y,
+                                               int m, int r0, int bonus) {
+  const int lane = lane_id();
+  int yrow[AK];
+#pragma unroll
+  for (int i = 0; i < AK; ++i) {
+    int r = r0 + lane * AK + i;
+    yrow[i] = (r < m) ? (int)Y[r] * MST_LETTERS : 0;
+  }
+  for (int x = 0; x < MST_LETTERS; ++x) {
+    uint32_t w[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      uint32_t v = 0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        int s = (int)S_lds[yrow[q * 4 + b] + x] + bonus;
+        v |= ((uint32_t)s & 0xFFu) << (8 * b);
+      }
+      w[q] = v;
+    }
+    *(uint2*)(prof + (x * MST_WAVE + lane) * AK) = make_uint2(w[0], w[1]);
+  }
+}
+
+// Column c of the lane's 8 rows; returns the packed nibbles.
+__device__ __forceinline__ uint32_t cell_column_flags(uint32_t w0, uint32_t w1, ALane& L, int& f,
+                                                      int& hu, int G) {
+  int hd = L.hdiag;
+  uint32_t word = 0;
+#pragma unroll
+  for (int i = 0; i < AK; ++i) {
+    const uint32_t wb = (i < 4) ? w0 : w1;
+    const int sb = (int)((wb >> (8 * (i & 3))) & 0xFFu);
+    const int eo = L.E[i], ho = L.Hm[i];
+    const int e = imax(eo, ho);
+    const uint32_t cl = eo >= ho;
+    const uint32_t cu = f >= hu;
+    f = imax(f, hu);
+    const int dg = hd + sb;
+    const int t1 = imax(e, f);
+    const uint32_t eqD = dg >= t1;
+    const uint32_t fe = f >= e;
+    const int h = imax(t1, dg) - G;
+    const uint32_t up = eqD | fe;
+    const uint32_t left = eqD | (fe ^ 1u);
+    word |= ((left << 3) | (up << 2) | (cu << 1) | cl) << (4 * i);
+    hd = ho;
+    L.Hm[i] = h;
+    L.E[i] = e;
+    hu = h;
+  }
+  return word;
+}
+
+template <int SRC, bool PRED, bool FIRST, bool STORE>
+__device__ __forceinline__ void astep(ALane& L, int t, int x, uint2 tb, int2 bv,
+                                      const uint8_t* prof_lane, int n, int G, int2* bout,
+                                      uint32_t* fl_strip) {
+  const int lane = lane_id();
+  int fin, hup;
+  if (FIRST) {
+    fin = shr1_const(L.Fout, MST_NEG_INF);
+    hup = shr1_const(L.Hout, 0);
+  } else {
+    fin = shr1(bv.x, L.Fout);
+    hup = shr1(bv.y, L.Hout);
+  }
+  const int c = t - lane;
+  const bool act = !PRED || (c >= 0 && c < n);
+  if (act) {
+    uint32_t w0, w1;
+    if (SRC == SRC_SEQ) {
+      uint2 v = *(const uint2*)(prof_lane + x * (MST_WAVE * AK));
+      w0 = v.x;
+      w1 = v.y;
+    } else {
+      w0 = tb.x;
+      w1 = tb.y;
+    }
+    int f = fin, hu = hup;
+    const uint32_t word = cell_column_flags(w0, w1, L, f, hu, G);
+    L.Fout = f;
+    L.Hout = hu;
+    fl_strip[(int64_t)t * MST_WAVE + lane] = word;
+    if (STORE && lane == MST_WAVE - 1) bout[c] = make_int2(f, hu);
+  }
+  L.hdiag = hup;
+}
+
+template <int SRC, bool FIRST>
+__device__ __forceinline__ void aload(int t0, const int8_t* __restrict__ X, int n, const int2* bin,
+                                      const uint2* tab_strip, int* xs, int2* bs, uint2* ts) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int j = 0; j < AU; ++j) {
+    const int t = t0 + j;
+    int c = t - lane;
+    c = c < 0 ? 0 : (c >= n ? n - 1 : c);
+    if (SRC == SRC_SEQ) {
+      xs[j] = X[c];
+      ts[j] = make_uint2(0, 0);
+    } else {
+      xs[j] = 0;
+      int tt = t < n + MST_WAVE - 1 ? t : n + MST_WAVE - 2;
+      ts[j] = tab_strip[(int64_t)tt * MST_WAVE + lane];
+    }
+    if (!FIRST) {
+      int tt = t >= n ? n - 1 : t;
+      bs[j] = bin[tt];
+    } else {
+      bs[j] = make_int2(0, 0);
+    }
+  }
+}
+
+template <int SRC, bool FIRST, bool STORE>
+__device__ void arun_strip(ALane& L, const int8_t* X, int n, const uint8_t* prof_lane,
+                           const uint2* tab_strip, const int2* bin, int2* bout, int G,
+                           uint32_t* fl_strip) {
+  const int T = n + MST_WAVE - 1;
+  int xa[AU], xb[AU];
+  int2 ba[AU], bb[AU];
+  uint2 ta[AU], tbb[AU];
+  aload<SRC, FIRST>(0, X, n, bin, tab_strip, xa, ba, ta);
+  for (int t0 = 0; t0 < T; t0 += AU) {
+    aload<SRC, FIRST>(t0 + AU, X, n, bin, tab_strip, xb, bb, tbb);
+    const bool full = (t0 >= MST_WAVE - 1) && (t0 + AU <= n);
+    if (full) {
+#pragma unroll
+      for (int j = 0; j < AU; ++j)
+        astep<SRC, false, FIRST, STORE>(L, t0 + j, xa[j], ta[j], ba[j], prof_lane, n, G, bout,
+                                        fl_strip);
+    } else {
+#pragma unroll
+      for (int j = 0; j < AU; ++j)
+        if (t0 + j < T)
+          astep<SRC, true, FIRST, STORE>(L, t0 + j, xa[j], ta[j], ba[j], prof_lane, n, G, bout,
+                                         fl_strip);
+    }
+#pragma unroll
+    for (int j = 0; j < AU; ++j) {
+      xa[j] = xb[j];
+      ba[j] = bb[j];
+      ta[j] = tbb[j];
+    }
+  }
+}
+
+template <int SRC>
+__global__ void __launch_bounds__(MST_WAVE) align_kernel(AlignArgs a) {
+  __shared__ int8_t S_lds[MST_LETTERS * MST_LETTERS];
+  __shared__ __attribute__((aligned(16))) uint8_t prof[MST_LETTERS * MST_WAVE * AK];
+  const int lane = lane_id();
+  if (SRC == SRC_SEQ) {
+    for (int i = lane; i < MST_LETTERS * MST_LETTERS; i += MST_WAVE) S_lds[i] = a.subst[i];
+  }
+  __syncthreads();
+  const int G = a.go - a.ge;
+  int2* buf = a.bufs + (int64_t)blockIdx.x * 2 * a.bufcap;
+  for (int64_t j = (int64_t)blockIdx.x + a.first; j < a.last; j += gridDim.x) {
+    const AlignJob job = a.jobs[j];
+    const int n = job.n, m = job.m;
+    uint32_t* fl = a.flags + job.flag_off;
+    const int nstrips = (m + ASH - 1) / ASH;
+    const int Tn = n + MST_WAVE - 1;
+    const int8_t* X = a.seqs + job.x_off;
+    const int8_t* Y = a.seqs + job.y_off;
+    ALane L;
+    int result = 0;
+    for (int s = 0; s < nstrips; ++s) {
+      const int r0 = s * ASH;
+      if (SRC == SRC_SEQ) {
+        __builtin_amdgcn_wave_barrier();
+        build_profile8(prof, S_lds, Y, m, r0, a.go + a.ge);
+        __builtin_amdgcn_wave_barrier();
+      }
+#pragma unroll
+      for (int i = 0; i < AK; ++i) {
+        L.Hm[i] = 0;
+        L.E[i] = MST_NEG_INF;
+      }
+      L.Fout = MST_NEG_INF;
+      L.Hout = 0;
+      L.hdiag = 0;
+      const int2* bin = buf + (int64_t)(s & 1) * a.bufcap;
+      int2* bout = buf + (int64_t)((s + 1) & 1) * a.bufcap;
+      uint32_t* fl_strip = fl + (int64_t)s * Tn * MST_WAVE;
+      const uint2* tab_strip =
+          (SRC == SRC_TAB) ? (const uint2*)(a.tab + job.tab_off) + (int64_t)s * Tn * MST_WAVE : nullptr;
+      const uint8_t* prof_lane = prof + lane * AK;
+      const bool last = (s == nstrips - 1);
+      if (s == 0) {
+        if (last)
+          arun_strip<SRC, true, false>(L, X, n, prof_lane, tab_strip, bin, bout, G, fl_strip);
+        else
+          arun_strip<SRC, true, true>(L, X, n, prof_lane, tab_strip, bin, bout, G, fl_strip);
+      } else {
+        if (last)
+          arun_strip<SRC, false, false>(L, X, n, prof_lane, tab_strip, bin, bout, G, fl_strip);
+        else
+          arun_strip<SRC, false, true>(L, X, n, prof_lane, tab_strip, bin, bout, G, fl_strip);
+      }
+      if (!last) agent_fence();
+      if (last) {
+        const int rl = (m - 1) - r0;
+        const int owner = rl / AK, idx = rl % AK;
+        int v = L.Hm[0];
+#pragma unroll
+        for (int i = 1; i < AK; ++i)
+          if (idx == i) v = L.Hm[i];
+        v = __shfl(v, owner);
+        result = v - a.ge * (m + n);
+      }
+    }
+    if (lane == 0) a.scores[j] = result;
+  }
+}
+
+// ---- traceback: one lane per job ----
+
+__device__ __forceinline__ int nib_at(const uint32_t* fl, int n, int x, int y) {
+  if (x < 0 && y < 0) return 0;
+  if (y < 0) return 9;  // LEFT + contLeft
+  if (x < 0) return 6;  // UP + contUp
+  const int Tn = n + MST_WAVE - 1;
+  const int s = y / ASH, rl = y - s * ASH;
+  const int lane = rl / AK, i = rl - lane * AK;
+  const int t = x + lane;
+  const uint32_t w = fl[((int64_t)s * Tn + t) * MST_WAVE + lane];
+  return (int)((w >> (4 * i)) & 15u);
+}
+
+__global__ void traceback_kernel(TraceArgs a) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x + a.first;
+  if (j >= a.last) return;
+  const AlignJob job = a.jobs[j];
+  const int n = job.n, m = job.m;
+  const uint32_t* fl = a.flags + job.flag_off;
+  uint8_t* out = a.ops + job.ops_off;
+  int x = n - 1, y = m - 1;
+  int prev = 12;
+  int cur = nib_at(fl, n, x, y);
+  int k = 0;
+  while (cur & 12) {
+    int todo;
+    if ((prev & 12) == 4 && (prev & 2))
+      todo = 4;
+    else if ((prev & 12) == 8 && (prev & 1))
+      todo = 8;
+    else if ((cur & 12) == 4)
+      todo = 4;
+    else if ((cur & 12) == 8)
+      todo = 8;
+    else
+      todo = 12;
+    out[k++] = (uint8_t)(todo == 12 ? OP_DIAG : (todo == 8 ? OP_LEFT : OP_UP));
+    if (todo == 8)
+      --x;
+    else if (todo == 4)
+      --y;
+    else {
+      --x;
+      --y;
+    }
+    prev = todo | (cur & 3);
+    cur = nib_at(fl, n, x, y);
+  }
+  a.lens[j] = k;
+}
+
+void launch_align(int src, const AlignArgs& a, int grid, hipStream_t st) {
+  if (src == SRC_SEQ)
+    hipLaunchKernelGGL(align_kernel<SRC_SEQ>, dim3(grid), dim3(MST_WAVE), 0, st, a);
+  else
+    hipLaunchKernelGGL(align_kernel<SRC_TAB>, dim3(grid), dim3(MST_WAVE), 0, st, a);
+}
+
+void launch_traceback(const TraceArgs& a, hipStream_t st) {
+  const int64_t n = a.last - a.first;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(traceback_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, a);
+}
+
+int64_t flag_words(int n, int m) {
+  const int64_t nstrips = (m + ASH - 1) / ASH;
+  return nstrips * (int64_t)(n + MST_WAVE - 1) * MST_WAVE;
+}
+
+}  // namespace mst

@@ -449,7 +449,8 @@ extern "C" int orc_nj_reference(const float* D, int n, int numBins, float ratio,
 }
 
 // Textbook NJ with the same arithmetic: the whole upper triangle of q is scanned
-// each step with q = (d*(n-2) - T_i) - T_j and the tie rule of
+// each step with q = (d*(n-2) - T_a) - T_b (operands in the order the
+// reference's candidate list holds the pair) and the tie rule of
 // SpeciesBlock.cpp:1169-1178; T follows the reference's incremental update and
 // rebuild schedule (:864,1225-1247) without the heap-size trigger.
 extern "C" int orc_nj_fullscan(const float* Din, int n0, float ratio,
@@ -470,7 +471,7 @@ extern "C" int orc_nj_fullscan(const float* Din, int n0, float ratio,
   };
   rowsums();
   int steps = std::max((int)(n0 * ratio), minRebuild);
-  int m = n0, rebuilds = 0;
+  int m = n0, rebuilds = 0, R = n0;
   for (int step = 0; step < n0 - 1; ++step) {
     double best = DBL_MAX;
     int bi = -1, bj = -1;
@@ -478,7 +479,11 @@ extern "C" int orc_nj_fullscan(const float* Din, int n0, float ratio,
       for (size_t y = x + 1; y < active.size(); ++y) {
         int I = active[x], Jd = active[y];
         int ri = store[I], rj = store[Jd];
-        double q = ((double)D[ri][rj]) * (m - 2) - T[ri] - T[rj];
+        // operand order of the reference's candidate list: id order for pairs
+        // made by the last bin rebuild, storage order for pairs made by the
+        // insertion of a node created after it (SpeciesBlock.cpp:1280-1287)
+        double dq = ((double)D[ri][rj]) * (m - 2);
+        double q = (Jd >= R && rj < ri) ? (dq - T[rj]) - T[ri] : (dq - T[ri]) - T[rj];
         if (q < best || (q == best && (Jd > bj || (Jd == bj && I > bi)))) {
           best = q;
           bi = I;
@@ -518,6 +523,7 @@ extern "C" int orc_nj_fullscan(const float* Din, int n0, float ratio,
     if (steps == 0) {
       rowsums();
       ++rebuilds;
+      R = J.node + 1;
       steps = (m < minRebuild) ? INT_MAX : (int)(m * ratio);
     } else {
       --steps;

@@ -1,0 +1,111 @@
+"""GPU parity of the all-pairs score kernels against the oracle (bit-exact
+int32). Sizes are chosen so the oracle finishes in seconds; the full-size
+configurations are checked through sampled pairs."""
+import numpy as np
+import pytest
+
+import mestrado_amd as M
+import oracle_bind as O
+
+pytestmark = pytest.mark.gpu
+S = O.blosum62()
+
+
+def _rand(rng, n, alpha=O.STANDARD20):
+    return "".join(rng.choice(list(alpha), n))
+
+
+@pytest.mark.parametrize("form", [M.FORM_PLAIN, M.FORM_BATCHED])
+def test_config0_all_pairs_bit_exact(ctx, form):
+    seqs = M.synth(0, 0)
+    st = ctx.upload(seqs)
+    g = st.all_pairs(form)
+    o, _ = O.all_pairs(seqs, S)
+    assert (g == o).all()
+    st.free()
+
+
+@pytest.mark.parametrize("form", [M.FORM_PLAIN, M.FORM_BATCHED])
+def test_edge_lengths(ctx, form):
+    # wave and strip boundaries: 64 lanes, K rows per lane (256/512/768/1024)
+    rng = np.random.default_rng(1)
+    lens = [1, 2, 3, 7, 8, 9, 63, 64, 65, 127, 128, 200, 255, 256, 257, 511, 512, 513, 767,
+            768, 769, 1023, 1024, 1025, 1500, 2049, 3100]
+    seqs = [_rand(rng, L) for L in lens]
+    st = ctx.upload(seqs)
+    g = st.all_pairs(form)
+    o, _ = O.all_pairs(seqs, S)
+    bad = np.argwhere(g != o)
+    assert len(bad) == 0, [(lens[i], lens[j], g[i, j], o[i, j]) for i, j in bad[:10]]
+
+
+def test_config1_sampled_pairs(ctx):
+    seqs = M.synth(1, 0)
+    st = ctx.upload(seqs)
+    rng = np.random.default_rng(5)
+    pi = rng.integers(0, len(seqs), 3000)
+    pj = rng.integers(0, len(seqs), 3000)
+    g = st.score_pairs(pi, pj, M.FORM_PLAIN)
+    codes, offs = O.pack_seqs(seqs)
+    o, _ = O.pair_list(codes, offs, pi, pj, S)
+    assert (g == o).all()
+    # the whole matrix through the batched form, checked on the same sample
+    full = st.all_pairs(M.FORM_BATCHED)
+    assert (full[pi, pj] == o).all()
+    assert (full == full.T).all()
+
+
+def test_config2_longest_pairs(ctx):
+    seqs = M.synth(2, 0)
+    L = np.array([len(s) for s in seqs])
+    st = ctx.upload(seqs)
+    order = np.argsort(-L)
+    top = order[:12]
+    rng = np.random.default_rng(9)
+    pi = np.concatenate([np.repeat(top, 3), rng.integers(0, len(seqs), 400)])
+    pj = np.concatenate([rng.choice(order[:200], len(top) * 3), rng.integers(0, len(seqs), 400)])
+    codes, offs = O.pack_seqs(seqs)
+    o, _ = O.pair_list(codes, offs, pi, pj, S)
+    for form in (M.FORM_PLAIN, M.FORM_BATCHED):
+        g = st.score_pairs(pi, pj, form)
+        assert (g == o).all()
+
+
+def test_unknown_and_lowercase_residues(ctx):
+    seqs = ["ACDJOUxyz*BZ", "acdefghik", "JJJJ", "WWWW*"]
+    st = ctx.upload(seqs)
+    g = st.all_pairs()
+    o, _ = O.all_pairs(seqs, S)
+    assert (g == o).all()
+    codes = st.codes()
+    ref, _ = O.pack_seqs(seqs)
+    assert (codes == ref[: len(codes)]).all()
+
+
+@pytest.mark.parametrize("go,ge", [(11, 1), (2, 2), (0, 0), (20, 5)])
+def test_other_gap_penalties(go, ge):
+    rng = np.random.default_rng(go * 31 + ge)
+    seqs = [_rand(rng, int(n)) for n in rng.integers(1, 700, 20)]
+    with M.Context(gap_open=go, gap_ext=ge) as c:
+        st = c.upload(seqs)
+        g = st.all_pairs()
+    o, _ = O.all_pairs(seqs, S, go=go, ge=ge)
+    assert (g == o).all()
+
+
+def test_asymmetric_matrix_keeps_roles(ctx):
+    # With an asymmetric S the kernel may not swap X and Y; score(X=i, Y=j)
+    # reads S[Y][X] (NeedlemanWunsch.cu:140).
+    S2 = S.copy()
+    L = O.LETTERS
+    S2[L.index("W"), L.index("A")] = 9
+    rng = np.random.default_rng(2)
+    seqs = [_rand(rng, int(n), "AW") for n in rng.integers(1, 300, 10)]
+    with M.Context(subst=S2) as c:
+        st = c.upload(seqs)
+        pi, pj = np.meshgrid(np.arange(10), np.arange(10), indexing="ij")
+        g = st.score_pairs(pi.ravel(), pj.ravel())
+    codes, offs = O.pack_seqs(seqs)
+    o, _ = O.pair_list(codes, offs, pi.ravel(), pj.ravel(), S2)
+    assert (g == o).all()
+    assert (g.reshape(10, 10) != g.reshape(10, 10).T).any()

@@ -1,0 +1,62 @@
+"""CPU-only checks of the native library boundary: it loads, exports every
+symbol the public headers declare, and its host-side logic (synthetic sets,
+argument validation without a device) behaves."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import mestrado_amd as M
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    out = set()
+    for h in ("mestrado.h", "mestrado_synth.h"):
+        txt = open(os.path.join(ROOT, "include", h)).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        for m in re.finditer(r"\b(mst_[a-z0-9_]+)\s*\(", txt):
+            out.add(m.group(1))
+    return out
+
+
+def test_library_exports_every_declared_symbol():
+    L = C.CDLL(M.LIB_PATH)
+    syms = declared_symbols()
+    assert len(syms) >= 10
+    missing = [s for s in sorted(syms) if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_synth_is_deterministic_and_shaped():
+    a = M.synth(0, 0)
+    b = M.synth(0, 0)
+    c = M.synth(0, 1)
+    assert a == b and a != c
+    assert len(a) == 64 and all(150 <= len(s) <= 300 for s in a)
+    assert set("".join(a)) <= set("ACDEFGHIKLMNPQRSTVWY")
+    d = M.synth(2, 0, n=300)
+    L = np.array([len(s) for s in d])
+    assert len(d) == 300 and L.min() >= 50 and L.max() <= 5000
+    assert 200 < np.median(L) < 500
+    e = M.synth(4, 7, n=20)
+    assert len(e) == 20 and all(100 <= len(s) <= 2000 for s in e)
+    sizes = M.job_sizes(0, 2000)
+    assert sizes.min() >= 8 and sizes.max() <= 512
+    assert 30 < sizes.mean() < 120
+
+
+def test_create_without_device_fails_cleanly():
+    # In a container without a GPU the context cannot be created; the call must
+    # return an error code, not crash. On a GPU box this test is skipped.
+    L = M.lib()
+    h = C.c_void_p()
+    rc = L.mst_create(10, 2, None, 0, C.byref(h))
+    if rc == 0:
+        L.mst_destroy(h)
+        pytest.skip("a device is present")
+    assert rc < 0
+    assert L.mst_create(2, 10, None, 0, C.byref(h)) < 0  # go < ge rejected
