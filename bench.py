@@ -1,0 +1,165 @@
+#!/usr/bin/env python3
+"""All-pairs Needleman-Wunsch throughput on one MI355X.
+
+Workload: the log-normal protein set (4,000 sequences, lengths log-normal with
+median 350 and sigma 0.8 clipped to [50, 5000], each from its own random parent
+with substitution 0.3), all 7,998,000 pairs i<j plus the 4,000 self pairs the
+distance matrix keeps on its diagonal, int32 scores, BLOSUM62, gap open 10,
+gap extension 2. A step is one pass of the score kernels over the whole set,
+with the sequences and the bucketed pair list already resident in HBM.
+
+value      GCUPS of the batched form = sum_{i<j} L_i*L_j / wall seconds per step
+plain      GCUPS of the plain drop-in form (one synchronous launch per bucket)
+roofline   the dominant launch's integer VALU rate (5 int32 ops per cell:
+           max, max, add, max3, sub) against 256 CU x 64 lanes x 2.4 GHz
+cpu_baseline  the oracle recurrence (oracle/liboracle.so, OpenMP over pairs)
+           on a bounded random sample of the same pairs
+
+Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_INT32_TOPS = 256 * 64 * 2.4e9 / 1e12  # 39.3 T int32 VALU ops/s
+OPS_PER_CELL = 5
+
+
+def cpu_baseline(seqs, target_seconds=12.0):
+    """Oracle recurrence on the host cores over a random sample of pairs."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as O
+
+    S = O.blosum62()
+    codes, offs = O.pack_seqs(seqs)
+    n = len(seqs)
+    L = np.array([len(s) for s in seqs], dtype=np.int64)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    rng = np.random.default_rng(123)
+    # calibrate on a small sample, then size the measured sample
+    pi = rng.integers(0, n, 256)
+    pj = rng.integers(0, n, 256)
+    keep = pi != pj
+    pi, pj = pi[keep], pj[keep]
+    _, secs = O.pair_list(codes, offs, pi, pj, S, threads=threads)
+    rate = (L[pi] * L[pj]).sum() / max(secs, 1e-6)
+    want_cells = rate * target_seconds
+    mean_cells = float((L.mean()) ** 2)
+    npairs = int(max(512, min(2_000_000, want_cells / mean_cells)))
+    pi = rng.integers(0, n, npairs)
+    pj = rng.integers(0, n, npairs)
+    keep = pi != pj
+    pi, pj = pi[keep], pj[keep]
+    _, secs = O.pair_list(codes, offs, pi, pj, S, threads=threads)
+    cells = int((L[pi] * L[pj]).sum())
+    return {
+        "value": round(cells / secs / 1e9, 3),
+        "unit": "GCUPS",
+        "cores": threads,
+        "kind": "port",
+        "sample": "%d random pairs (i != j) of the same 4,000-sequence set, %.3g cells, %.1f s wall"
+                  % (len(pi), cells, secs),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the all-pairs path runs on a single MI355X "
+              "(how it would shard across GPUs is described in DESIGN.md); use --gpus 1",
+              file=sys.stderr)
+        return 2
+
+    import mestrado_amd as M
+
+    seqs = M.synth(args.config, 0)
+    L = np.array([len(s) for s in seqs], dtype=np.int64)
+    cells = int((L.sum() ** 2 - (L * L).sum()) // 2)  # sum_{i<j} L_i L_j
+    self_cells = int((L * L).sum())
+
+    ctx = M.Context(device=0)
+    st = ctx.upload(seqs)
+    print("bench: %d sequences, %.4g cells (i<j), planning" % (len(seqs), cells), file=sys.stderr,
+          flush=True)
+
+    for _ in range(args.warmup):
+        st.all_pairs_resident(M.FORM_PLAIN)
+        st.all_pairs_resident(M.FORM_BATCHED)
+    print("bench: warmup done", file=sys.stderr, flush=True)
+
+    def timed(form):
+        walls, devs = [], []
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            devs.append(st.all_pairs_resident(form))
+            walls.append(time.perf_counter() - t0)
+        return walls, devs
+
+    walls_b, devs_b = timed(M.FORM_BATCHED)
+    stats_b = st.launch_stats()
+    walls_p, devs_p = timed(M.FORM_PLAIN)
+    print("bench: timed steps done", file=sys.stderr, flush=True)
+
+    sec_b = float(np.mean(walls_b))
+    sec_p = float(np.mean(walls_p))
+    gcups_b = cells / sec_b / 1e9
+    gcups_p = cells / sec_p / 1e9
+
+    # dominant launch of the batched form
+    k, npairs, lcells, lms = max(stats_b, key=lambda r: r[3])
+    achieved = lcells * OPS_PER_CELL / (lms * 1e-3) / 1e12
+    roof = {
+        "bound": "valu",
+        "kernel": "score_pairs_kernel<%d>" % k,
+        "achieved": round(achieved, 3),
+        "peak": round(PEAK_INT32_TOPS, 2),
+        "unit": "Tops/s (int32)",
+        "frac": round(achieved / PEAK_INT32_TOPS, 4),
+        "traffic": None,
+        "launch_ms": round(lms, 3),
+        "launch_cells": lcells,
+    }
+
+    line = {
+        "metric": "all-pairs NW GCUPS (bit-exact), batched form",
+        "value": round(gcups_b, 2),
+        "unit": "GCUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sec_b * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": "lognormal-4000 all-pairs NW scores (BLOSUM62, go=10, ge=2)",
+                   "sequences": len(seqs), "pairs_i_lt_j": len(seqs) * (len(seqs) - 1) // 2,
+                   "cells_i_lt_j": cells, "self_cells": self_cells},
+        "plain_gcups": round(gcups_p, 2),
+        "batched_over_plain": round(gcups_b / gcups_p, 3),
+        "device_ms_batched": round(float(np.mean(devs_b)), 3),
+        "device_ms_plain": round(float(np.mean(devs_p)), 3),
+        "roofline": roof,
+        "cpu_baseline": None if args.no_cpu else cpu_baseline(seqs),
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -42,6 +42,7 @@ def lib():
         L.mst_all_pairs.argtypes = [P, C.c_int, C.c_int, P, P]
         L.mst_score_pairs.argtypes = [P, C.c_int, P, P, C.c_int64, C.c_int, P]
         L.mst_all_pairs_resident.argtypes = [P, C.c_int, C.c_int, C.POINTER(C.c_float)]
+        L.mst_last_launch_stats.argtypes = [P, P, P, P, P, C.c_int, C.POINTER(C.c_int)]
         L.mst_align_pairs.argtypes = [P, C.c_int, P, P, C.c_int64, P, P, P, C.c_int64]
         L.mst_nj.argtypes = [P, P, C.c_int, C.c_float, P]
         L.mst_nj_scores.argtypes = [P, C.c_int, C.c_float, P]
@@ -208,7 +209,21 @@ class SeqSet:
         return list(joins)[: self.n - 1]
 
     def all_pairs_resident(self, form=FORM_BATCHED):
+        """All pairs with the matrix left on the device; returns device ms."""
         ms = C.c_float(0)
         self.ctx._ck(self.ctx._L.mst_all_pairs_resident(self.ctx.h, self.id, form, C.byref(ms)),
                      "mst_all_pairs_resident")
         return ms.value
+
+    def launch_stats(self):
+        """[(rows_per_lane, pairs, cells, ms)] of the last all_pairs_resident."""
+        L = self.ctx._L
+        cnt = C.c_int(0)
+        L.mst_last_launch_stats(self.ctx.h, None, None, None, None, 0, C.byref(cnt))
+        k = np.zeros(cnt.value, np.int32)
+        p = np.zeros(cnt.value, np.int64)
+        c = np.zeros(cnt.value, np.int64)
+        m = np.zeros(cnt.value, np.float32)
+        self.ctx._ck(L.mst_last_launch_stats(self.ctx.h, _p(k), _p(p), _p(c), _p(m), cnt.value,
+                                             C.byref(cnt)), "mst_last_launch_stats")
+        return list(zip(k.tolist(), p.tolist(), c.tolist(), m.tolist()))

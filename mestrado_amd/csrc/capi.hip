@@ -77,7 +77,8 @@ extern "C" int mst_create(int gap_open, int gap_ext, const int8_t* subst, int de
                           mst_ctx** out) {
   if (!out) return MST_E_ARG;
   *out = nullptr;
-  if (gap_open < 0 || gap_ext < 0 || gap_open < gap_ext) return MST_E_ARG;
+  // the kernels pack S + go + ge into signed bytes (S in [-128,127] shifted)
+  if (gap_open < 0 || gap_ext < 0 || gap_open < gap_ext || gap_open + gap_ext > 116) return MST_E_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return MST_E_HIP;
   if (device < 0 || device >= ndev) return MST_E_ARG;
@@ -87,6 +88,8 @@ extern "C" int mst_create(int gap_open, int gap_ext, const int8_t* subst, int de
   ctx->go = gap_open;
   ctx->ge = gap_ext;
   memcpy(ctx->S, subst ? subst : kBlosum62, 576);
+  for (int k = 0; k < 576; ++k)
+    if (ctx->S[k] + gap_open + gap_ext > 127 || ctx->S[k] + gap_open + gap_ext < -128) return MST_E_ARG;
   ctx->symmetric = true;
   for (int a = 0; a < 24; ++a)
     for (int b = 0; b < 24; ++b)
@@ -115,6 +118,7 @@ mst_ctx::~mst_ctx() {
                 d_nj_act, d_nj_joins, d_flags, d_ops, d_jobs,   d_jscore};
   for (void* p : ps)
     if (p) hipFree(p);
+  for (hipEvent_t e : launch_events) hipEventDestroy(e);
   if (ev0) hipEventDestroy(ev0);
   if (ev1) hipEventDestroy(ev1);
   if (stream) hipStreamDestroy(stream);
@@ -236,9 +240,10 @@ int build_plan(mst_ctx* ctx, const SeqSet& s, const int32_t* pi, const int32_t* 
     plan.index[t] = k;
     int K = k_for_rows(rows[k]);
     if (plan.buckets.empty() || plan.buckets.back().K != K)
-      plan.buckets.push_back(Bucket{K, t, t + 1});
+      plan.buckets.push_back(Bucket{K, t, t + 1, 0});
     else
       plan.buckets.back().end = t + 1;
+    plan.buckets.back().cells += cost[k];
   }
   plan.cells = 0;
   for (int64_t k = 0; k < np; ++k) plan.cells += cost[k];
@@ -262,7 +267,7 @@ int score_grid(mst_ctx* ctx, int64_t npairs) {
 }
 
 int run_plan(mst_ctx* ctx, const SeqSet& s, const PairPlan& plan, int form, int32_t* d_out_pair,
-             int32_t* d_out_mat) {
+             int32_t* d_out_mat, bool timed = false) {
   int grid_max = score_grid(ctx, (int64_t)plan.pairs.size());
   int64_t bufcap = std::max<int64_t>(s.maxlen, 1);
   int rc = ctx->ensure((void**)&ctx->d_bufs, &ctx->bufs_cap,
@@ -285,8 +290,11 @@ int run_plan(mst_ctx* ctx, const SeqSet& s, const PairPlan& plan, int form, int3
     a.ge = ctx->ge;
     a.transpose_ok = ctx->symmetric ? 1 : 0;
     int grid = score_grid(ctx, b.end - b.begin);
+    const size_t bi = &b - plan.buckets.data();
+    if (timed) CK(hipEventRecord(ctx->launch_events[2 * bi], ctx->stream));
     launch_score_pairs(b.K, a, grid, ctx->stream);
     CK(hipGetLastError());
+    if (timed) CK(hipEventRecord(ctx->launch_events[2 * bi + 1], ctx->stream));
     if (form == MST_FORM_PLAIN) CK(hipStreamSynchronize(ctx->stream));
   }
   return MST_OK;
@@ -357,16 +365,42 @@ extern "C" int mst_all_pairs_resident(mst_ctx* ctx, int id, int form, float* ms)
   size_t nn = (size_t)s.n * s.n;
   rc = ctx->ensure((void**)&ctx->d_mat, &ctx->mat_cap, sizeof(int32_t) * std::max<size_t>(nn, 1));
   if (rc) return rc;
+  while (ctx->launch_events.size() < 2 * plan->buckets.size()) {
+    hipEvent_t e;
+    CK(hipEventCreate(&e));
+    ctx->launch_events.push_back(e);
+  }
   CK(hipStreamSynchronize(ctx->stream));
   CK(hipEventRecord(ctx->ev0, ctx->stream));
-  rc = run_plan(ctx, s, *plan, form, nullptr, ctx->d_mat);
+  rc = run_plan(ctx, s, *plan, form, nullptr, ctx->d_mat, true);
   if (rc) return rc;
   CK(hipEventRecord(ctx->ev1, ctx->stream));
   CK(hipEventSynchronize(ctx->ev1));
   ctx->mat_set = id;
+  ctx->launch_stats.clear();
+  for (size_t b = 0; b < plan->buckets.size(); ++b) {
+    float t = 0;
+    CK(hipEventElapsedTime(&t, ctx->launch_events[2 * b], ctx->launch_events[2 * b + 1]));
+    const Bucket& bk = plan->buckets[b];
+    ctx->launch_stats.push_back(LaunchStat{bk.K, bk.end - bk.begin, bk.cells, t});
+  }
   float t = 0;
   CK(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
   if (ms) *ms = t;
+  return MST_OK;
+}
+
+extern "C" int mst_last_launch_stats(mst_ctx* ctx, int* rows_per_lane, int64_t* pairs,
+                                     int64_t* cells, float* ms, int cap, int* count) {
+  if (!ctx || !count) return MST_E_ARG;
+  *count = (int)ctx->launch_stats.size();
+  for (int i = 0; i < *count && i < cap; ++i) {
+    const LaunchStat& L = ctx->launch_stats[i];
+    if (rows_per_lane) rows_per_lane[i] = L.K;
+    if (pairs) pairs[i] = L.pairs;
+    if (cells) cells[i] = L.cells;
+    if (ms) ms[i] = L.ms;
+  }
   return MST_OK;
 }
 

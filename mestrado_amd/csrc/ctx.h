@@ -22,6 +22,13 @@ std::string canonical_letters(const char* ascii, int64_t n);
 struct Bucket {
   int K;
   int64_t begin, end;
+  int64_t cells = 0;
+};
+
+struct LaunchStat {
+  int K;
+  int64_t pairs, cells;
+  float ms;
 };
 
 // A bucketed pair list (pairs sorted by bucket, longest first in a bucket).
@@ -78,6 +85,10 @@ struct mst_ctx {
   size_t jobs_cap = 0;
   void* d_jscore = nullptr;
   size_t jscore_cap = 0;
+
+  // per-launch timing of the last resident all-pairs call
+  std::vector<hipEvent_t> launch_events;
+  std::vector<LaunchStat> launch_stats;
 
   ~mst_ctx();
   int fail(int code, const char* what, const char* detail = nullptr);

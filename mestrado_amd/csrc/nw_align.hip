@@ -78,7 +78,7 @@ __device__ __forceinline__ uint32_t cell_column_flags(uint32_t w0, uint32_t w1, 
 #pragma unroll
   for (int i = 0; i < AK; ++i) {
     const uint32_t wb = (i < 4) ? w0 : w1;
-    const int sb = (int)((wb >> (8 * (i & 3))) & 0xFFu);
+    const int sb = (int)(int8_t)((wb >> (8 * (i & 3))) & 0xFFu);
     const int eo = L.E[i], ho = L.Hm[i];
     const int e = imax(eo, ho);
     const uint32_t cl = eo >= ho;

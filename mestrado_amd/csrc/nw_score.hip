@@ -22,7 +22,7 @@
 // one step per lane (anti-diagonal wavefront). F' and Hm of a lane's last row
 // move to the next lane with v_mov_dpp wave_shr:1 (lane 0 reads zeros, or the
 // previous strip's last row, which lane 63 left in a per-wave double buffer).
-// The substitution scores of the lane's K rows, offset by go+ge into [8,23],
+// The substitution scores of the lane's K rows, offset by go+ge (signed bytes),
 // are packed K bytes per residue in LDS (query profile) laid out [x][lane][K],
 // so one ds_read_b32/b64/b128 per step and lane is bank-conflict free. X
 // residues are prefetched one block of U steps ahead.
@@ -69,8 +69,9 @@ struct ProfRead<16> {
   }
 };
 
+// Signed byte i of the packed profile (S + go + ge fits int8 for go+ge <= 116).
 __device__ __forceinline__ int pbyte(const uint32_t* w, int i) {
-  return (int)((w[i >> 2] >> (8 * (i & 3))) & 0xFFu);
+  return (int)(int8_t)((w[i >> 2] >> (8 * (i & 3))) & 0xFFu);
 }
 
 // wave_shr:1 with zero fill for lane 0.
