@@ -111,6 +111,19 @@ int mst_nj(mst_ctx* ctx, const float* D_nxn, int n, float rebuild_ratio, mst_joi
  * on the device (d = (float)score, SpeciesBlock.cpp:264 via AlignmentScore). */
 int mst_nj_scores(mst_ctx* ctx, int set_id, float rebuild_ratio, mst_join* joins);
 
+/* Progressive alignment along the guide tree: what AlignmentMatch::
+ * computeAlignment (AlignmentMatch.cu:133-434) leaves unfinished, made whole.
+ * Every join aligns the profile of its left node (columns) against the profile
+ * of its right node (rows) with the recurrence, flags and traceback of
+ * mst_align_pairs (profile column scores: rint of the fp32 frequency-weighted
+ * substitution score; exactly the substitution score for single sequences),
+ * then inserts the path's gap columns into every member. *msa_len receives the
+ * number of columns; the rows stay on the context for mst_msa_rows. */
+int mst_msa(mst_ctx* ctx, int set_id, const mst_join* joins, int* msa_len);
+/* Rows of the last mst_msa in input order, letters and '-', NUL-terminated,
+ * row r at offset r*stride (stride >= msa_len+1). */
+int mst_msa_rows(mst_ctx* ctx, char* rows, int64_t stride);
+
 #ifdef __cplusplus
 }
 #endif

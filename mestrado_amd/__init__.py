@@ -44,6 +44,8 @@ def lib():
         L.mst_all_pairs_resident.argtypes = [P, C.c_int, C.c_int, C.POINTER(C.c_float)]
         L.mst_last_launch_stats.argtypes = [P, P, P, P, P, C.c_int, C.POINTER(C.c_int)]
         L.mst_align_pairs.argtypes = [P, C.c_int, P, P, C.c_int64, P, P, P, C.c_int64]
+        L.mst_msa.argtypes = [P, C.c_int, P, C.POINTER(C.c_int)]
+        L.mst_msa_rows.argtypes = [P, P, C.c_int64]
         L.mst_nj.argtypes = [P, P, C.c_int, C.c_float, P]
         L.mst_nj_scores.argtypes = [P, C.c_int, C.c_float, P]
         L.mst_synth.argtypes = [C.c_int, C.c_uint64, C.c_int, P, C.c_int64, P, C.c_int64,
@@ -207,6 +209,17 @@ class SeqSet:
         joins = (Join * max(1, self.n - 1))()
         self.ctx._ck(self.ctx._L.mst_nj_scores(self.ctx.h, self.id, ratio, joins), "mst_nj_scores")
         return list(joins)[: self.n - 1]
+
+    def msa(self, joins):
+        """Progressive alignment along `joins` -> list of gapped rows."""
+        arr = (Join * max(1, len(joins)))(*joins)
+        L = C.c_int(0)
+        self.ctx._ck(self.ctx._L.mst_msa(self.ctx.h, self.id, arr, C.byref(L)), "mst_msa")
+        stride = L.value + 1
+        buf = C.create_string_buffer(self.n * stride)
+        self.ctx._ck(self.ctx._L.mst_msa_rows(self.ctx.h, buf, stride), "mst_msa_rows")
+        raw = buf.raw
+        return [raw[r * stride: r * stride + L.value].decode() for r in range(self.n)]
 
     def all_pairs_resident(self, form=FORM_BATCHED):
         """All pairs with the matrix left on the device; returns device ms."""

@@ -74,8 +74,14 @@ def build_cli():
         return None
     os.makedirs(BIN, exist_ok=True)
     out = os.path.join(BIN, "clustal")
-    _run(["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), "-I" + HOST,
-          "-o", out] + srcs + ["-L" + PKG, "-lmestrado", "-Wl,-rpath,$ORIGIN/.."])
+    inc = ["-I" + os.path.join(ROOT, "include"), "-I" + HOST]
+    _run(["g++", "-O2", "-std=c++17"] + inc + ["-o", out] + srcs +
+         ["-L" + PKG, "-lmestrado", "-Wl,-rpath,$ORIGIN/.."])
+    # device-free driver of the host classes, for the CPU tests
+    plain = [os.path.join(HOST, f) for f in ("FastaFile.cpp", "CmdLineOptions.cpp", "TreeNode.cpp",
+                                             "MatchesManager.cpp")]
+    _run(["g++", "-O2", "-std=c++17"] + inc + ["-o", os.path.join(BIN, "host_tool"),
+          os.path.join(HOST, "tools", "host_tool.cpp")] + plain)
     return out
 
 

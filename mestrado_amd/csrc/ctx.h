@@ -86,6 +86,10 @@ struct mst_ctx {
   void* d_jscore = nullptr;
   size_t jscore_cap = 0;
 
+  // result of the last mst_msa: msa_n rows of msa_len codes (24 = gap)
+  int msa_n = 0, msa_len = 0;
+  std::vector<uint8_t> msa_rows;
+
   // per-launch timing of the last resident all-pairs call
   std::vector<hipEvent_t> launch_events;
   std::vector<LaunchStat> launch_stats;

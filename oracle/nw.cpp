@@ -341,7 +341,7 @@ extern "C" double orc_all_pairs(const int8_t* seqs, const int64_t* off, int n,
 }
 
 // ---- profile-profile DP (build-defined; reduces to R1-R3 on one-hot) ----
-// Column frequencies f = count/k in fp32. P_j(a) = sum_b fY_j(b)*S[a][b] in b
+// Column frequencies f = count/k in fp32. P_j(a) = sum_b fY_j(b)*S[b][a] in b
 // order, then s(i,j) = sum_a fX_i(a)*P_j(a) in a order, every product and sum
 // rounded to fp32 separately, finally rounded half-to-even to int32.
 extern "C" void orc_profile_scores(const int32_t* cntX, int n, int kX,
@@ -356,7 +356,7 @@ extern "C" void orc_profile_scores(const int32_t* cntX, int n, int kX,
     for (int a = 0; a < 24; ++a) {
       float acc = 0.0f;
       for (int b = 0; b < 24; ++b) {
-        float p = fy[b] * (float)S[24 * a + b];
+        float p = fy[b] * (float)S[24 * b + a];
         acc = acc + p;
       }
       P[(size_t)j * 24 + a] = acc;

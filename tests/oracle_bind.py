@@ -193,3 +193,63 @@ def ref_tree_text(joins, names):
 
 def have_ref_tool():
     return os.path.exists(REF_TOOL)
+
+
+def profile_counts(rows):
+    """rows: list of equal-length strings (letters and '-') -> int32 [len][25]."""
+    L = len(rows[0]) if rows else 0
+    cnt = np.zeros((L, 25), dtype=np.int32)
+    idx = {c: k for k, c in enumerate(LETTERS)}
+    for r in rows:
+        for t, ch in enumerate(r):
+            cnt[t, 24 if ch == "-" else idx.get(ch.upper(), 22)] += 1
+    return cnt
+
+
+def profile_align(rowsX, rowsY, S, go=10, ge=2):
+    """Oracle profile-profile alignment (the DP runs on reversed columns).
+    Returns (score, ops) with ops in forward order: 'D', 'L' (X column vs gaps),
+    'U' (gaps vs Y column)."""
+    cx = profile_counts(rowsX)[::-1].copy()
+    cy = profile_counts(rowsY)[::-1].copy()
+    n, m = len(cx), len(cy)
+    sc = np.zeros(max(1, n * m), dtype=np.int32)
+    P32 = C.POINTER(C.c_int32)
+    lib().orc_profile_scores(cx.ctypes.data_as(P32), n, len(rowsX), cy.ctypes.data_as(P32), m,
+                             len(rowsY), _i8(S), sc.ctypes.data_as(P32))
+    nib = np.zeros(max(1, n * m), dtype=np.uint8)
+    score = lib().orc_nw_flags_sc(sc.ctypes.data_as(P32), n, m, go, ge,
+                                  nib.ctypes.data_as(C.POINTER(C.c_uint8)))
+    ops = C.create_string_buffer(n + m + 1)
+    lib().orc_nw_path(nib.ctypes.data_as(C.POINTER(C.c_uint8)), n, m, ops)
+    return score, ops.value.decode()
+
+
+def progressive_msa(seqs, joins, S, go=10, ge=2):
+    """Oracle progressive alignment along `joins` (objects with node/left/right);
+    returns the gapped rows in input order."""
+    canon = ["".join(LETTERS[c] for c in encode_rev(s)[::-1]) for s in seqs]
+    groups = {i: ([i], [canon[i]]) for i in range(len(seqs))}
+    for J in joins:
+        mx, rx = groups.pop(J.left)
+        my, ry = groups.pop(J.right)
+        if rx[0] == "" or ry[0] == "":
+            ops = ("U" * len(ry[0])) if rx[0] == "" else ("L" * len(rx[0]))
+        else:
+            _, ops = profile_align(rx, ry, S, go, ge)
+        nx = [""] * len(rx)
+        ny = [""] * len(ry)
+        ix = iy = 0
+        for op in ops:
+            for k in range(len(rx)):
+                nx[k] += "-" if op == "U" else rx[k][ix]
+            for k in range(len(ry)):
+                ny[k] += "-" if op == "L" else ry[k][iy]
+            ix += op != "U"
+            iy += op != "L"
+        groups[J.node] = (mx + my, nx + ny)
+    (members, rows), = groups.values()
+    out = [None] * len(seqs)
+    for i, r in zip(members, rows):
+        out[i] = r
+    return out
