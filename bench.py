@@ -10,8 +10,10 @@ with the sequences and the bucketed pair list already resident in HBM.
 
 value      GCUPS of the batched form = sum_{i<j} L_i*L_j / wall seconds per step
 plain      GCUPS of the plain drop-in form (one synchronous launch per bucket)
-roofline   the dominant launch's integer VALU rate (5 int32 ops per cell:
-           max, max, add, max3, sub) against 256 CU x 64 lanes x 2.4 GHz
+roofline   the dominant launch's integer VALU rate against 256 CU x 64 lanes
+           x 2.4 GHz: 5 int32 ops per cell (max, max, add, max3, sub) for the
+           int32 kernel, 6 int16 ops per cell at 2 per lane and cycle for the
+           packed 16-bit kernels
 cpu_baseline  the oracle recurrence (oracle/liboracle.so, OpenMP over pairs)
            on a bounded random sample of the same pairs
 
@@ -29,7 +31,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_INT32_TOPS = 256 * 64 * 2.4e9 / 1e12  # 39.3 T int32 VALU ops/s
-OPS_PER_CELL = 5
+PEAK_INT16_TOPS = 2 * PEAK_INT32_TOPS       # 78.6 T int16 ops/s (packed, 2 per lane)
+OPS_PER_CELL = 5       # int32 kernel: max, max, add, max3, sub
+PK16_OPS_PER_CELL = 6  # packed kernel, per cell: max, max, add, max, max, sub
 
 
 def cpu_baseline(seqs, target_seconds=12.0):
@@ -109,29 +113,39 @@ def main():
         return walls, devs
 
     walls_b, devs_b = timed(M.FORM_BATCHED)
-    stats_b = st.launch_stats()
     walls_p, devs_p = timed(M.FORM_PLAIN)
     print("bench: timed steps done", file=sys.stderr, flush=True)
+    stats_b = st.launch_stats(M.FORM_BATCHED)  # one extra pass, per-launch events
 
     sec_b = float(np.mean(walls_b))
     sec_p = float(np.mean(walls_p))
     gcups_b = cells / sec_b / 1e9
     gcups_p = cells / sec_p / 1e9
 
-    # dominant launch of the batched form
-    k, npairs, lcells, lms = max(stats_b, key=lambda r: r[3])
-    achieved = lcells * OPS_PER_CELL / (lms * 1e-3) / 1e12
+    def kernel_of(cls):
+        if cls <= -100:
+            return "score2_kernel<%d>" % (-100 - cls), PK16_OPS_PER_CELL, PEAK_INT16_TOPS, "int16x2"
+        if cls < 0:
+            return "score3_kernel<%d>" % (-cls), PK16_OPS_PER_CELL, PEAK_INT16_TOPS, "int16x2"
+        return "score_pairs_kernel<%d>" % cls, OPS_PER_CELL, PEAK_INT32_TOPS, "int32"
+
+    cls, npairs, lcells, lms = max(stats_b, key=lambda r: r[3])
+    kname, opc, peak, kind = kernel_of(cls)
+    achieved = lcells * opc / (lms * 1e-3) / 1e12
     roof = {
         "bound": "valu",
-        "kernel": "score_pairs_kernel<%d>" % k,
+        "kernel": kname,
         "achieved": round(achieved, 3),
-        "peak": round(PEAK_INT32_TOPS, 2),
-        "unit": "Tops/s (int32)",
-        "frac": round(achieved / PEAK_INT32_TOPS, 4),
+        "peak": round(peak, 2),
+        "unit": "Tops/s (%s VALU, %d ops per cell)" % (kind, opc),
+        "frac": round(achieved / peak, 4),
         "traffic": None,
         "launch_ms": round(lms, 3),
         "launch_cells": lcells,
     }
+    launches = [{"kernel": kernel_of(c)[0], "pairs": p, "cells": ce, "ms": round(m, 3),
+                 "gcups": round(ce / (m * 1e-3) / 1e9, 1) if m > 0 else None}
+                for c, p, ce, m in stats_b]
 
     line = {
         "metric": "all-pairs NW GCUPS (bit-exact), batched form",
@@ -154,6 +168,7 @@ def main():
         "device_ms_batched": round(float(np.mean(devs_b)), 3),
         "device_ms_plain": round(float(np.mean(devs_p)), 3),
         "roofline": roof,
+        "batched_launches": launches,
         "cpu_baseline": None if args.no_cpu else cpu_baseline(seqs),
     }
     print(json.dumps(line), flush=True)

@@ -70,9 +70,14 @@ int mst_score_pairs(mst_ctx* ctx, int set_id, const int32_t* pi,
  * device; *kernel_ms receives the device time of the score launches (HIP
  * events on the context's stream). */
 int mst_all_pairs_resident(mst_ctx* ctx, int set_id, int form, float* kernel_ms);
-/* Per-launch record of the last mst_all_pairs_resident call: rows per lane of
- * the kernel instance, pairs, DP cells and device milliseconds of each launch
- * (arrays of cap entries; *count receives the number of launches). */
+/* One all-pairs pass with an event pair around every launch (the batched
+ * form's launches are not graph-replayed here), for per-launch timing. */
+int mst_all_pairs_launches(mst_ctx* ctx, int set_id, int form);
+/* Per-launch record of the last mst_all_pairs_launches call: rows per lane of
+ * the kernel instance (negative: -K for the shared-profile 16-bit kernel,
+ * -100-K for the multi-strip 16-bit kernel), pairs, DP cells and device
+ * milliseconds of each launch (arrays of cap entries; *count receives the
+ * number of launches). */
 int mst_last_launch_stats(mst_ctx* ctx, int* rows_per_lane, int64_t* pairs, int64_t* cells,
                           float* ms, int cap, int* count);
 

@@ -42,6 +42,7 @@ def lib():
         L.mst_all_pairs.argtypes = [P, C.c_int, C.c_int, P, P]
         L.mst_score_pairs.argtypes = [P, C.c_int, P, P, C.c_int64, C.c_int, P]
         L.mst_all_pairs_resident.argtypes = [P, C.c_int, C.c_int, C.POINTER(C.c_float)]
+        L.mst_all_pairs_launches.argtypes = [P, C.c_int, C.c_int]
         L.mst_last_launch_stats.argtypes = [P, P, P, P, P, C.c_int, C.POINTER(C.c_int)]
         L.mst_align_pairs.argtypes = [P, C.c_int, P, P, C.c_int64, P, P, P, C.c_int64]
         L.mst_msa.argtypes = [P, C.c_int, P, C.POINTER(C.c_int)]
@@ -228,9 +229,12 @@ class SeqSet:
                      "mst_all_pairs_resident")
         return ms.value
 
-    def launch_stats(self):
-        """[(rows_per_lane, pairs, cells, ms)] of the last all_pairs_resident."""
+    def launch_stats(self, form=FORM_BATCHED):
+        """One pass with per-launch events -> [(kernel_class, pairs, cells, ms)]
+        (kernel_class: K rows per lane of the int32 kernel, -K of the shared-
+        profile 16-bit kernel, -100-K of the multi-strip 16-bit kernel)."""
         L = self.ctx._L
+        self.ctx._ck(L.mst_all_pairs_launches(self.ctx.h, self.id, form), "mst_all_pairs_launches")
         cnt = C.c_int(0)
         L.mst_last_launch_stats(self.ctx.h, None, None, None, None, 0, C.byref(cnt))
         k = np.zeros(cnt.value, np.int32)

@@ -1,6 +1,7 @@
 // C ABI of libmestrado (include/mestrado.h): context, resident sequence sets
 // and the all-pairs score path.
 #include <algorithm>
+#include <climits>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -16,6 +17,11 @@
 using namespace mst;
 
 namespace {
+
+__global__ void fill_i32(int32_t* p, int64_t n, int32_t v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
 
 // BLOSUM62 in the letter order ARNDCQEGHILKMFPSTWYVBZX* (the table of
 // definitions.cuh:96-121, which is the published BLOSUM62 matrix).
@@ -110,6 +116,9 @@ extern "C" int mst_create(int gap_open, int gap_ext, const int8_t* subst, int de
 }
 
 mst_ctx::~mst_ctx() {
+  for (auto& kv : bplans) free_batched(kv.second);
+  for (auto& kv : fbplans)
+    if (kv.second.d_pairs) hipFree(kv.second.d_pairs);
   for (auto& kv : sets) {
     hipFree(kv.second.d_codes);
     hipFree(kv.second.d_off);
@@ -186,6 +195,16 @@ extern "C" int mst_seqset_free(mst_ctx* ctx, int id) {
   hipFree(it->second.d_off);
   ctx->sets.erase(it);
   ctx->plans.erase(id);
+  auto bp = ctx->bplans.find(id);
+  if (bp != ctx->bplans.end()) {
+    free_batched(bp->second);
+    ctx->bplans.erase(bp);
+  }
+  auto fp = ctx->fbplans.find(id);
+  if (fp != ctx->fbplans.end()) {
+    if (fp->second.d_pairs) hipFree(fp->second.d_pairs);
+    ctx->fbplans.erase(fp);
+  }
   return MST_OK;
 }
 
@@ -266,18 +285,19 @@ int score_grid(mst_ctx* ctx, int64_t npairs) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(g, npairs));
 }
 
-int run_plan(mst_ctx* ctx, const SeqSet& s, const PairPlan& plan, int form, int32_t* d_out_pair,
-             int32_t* d_out_mat, bool timed = false) {
+int run_plan_on(mst_ctx* ctx, const SeqSet& s, const PairPlan& plan, const int2* d_pairs, int form,
+                int32_t* d_out_pair, int32_t* d_out_mat, bool timed, int event_base) {
   int grid_max = score_grid(ctx, (int64_t)plan.pairs.size());
   int64_t bufcap = std::max<int64_t>(s.maxlen, 1);
   int rc = ctx->ensure((void**)&ctx->d_bufs, &ctx->bufs_cap,
                        sizeof(int2) * 2 * bufcap * (size_t)grid_max);
   if (rc) return rc;
-  for (const Bucket& b : plan.buckets) {
+  for (size_t bi = 0; bi < plan.buckets.size(); ++bi) {
+    const Bucket& b = plan.buckets[bi];
     ScoreArgs a;
     a.seqs = s.d_codes;
     a.off = s.d_off;
-    a.pairs = ctx->d_pairs;
+    a.pairs = d_pairs;
     a.first = b.begin;
     a.last = b.end;
     a.out_pair = d_out_pair;
@@ -290,15 +310,23 @@ int run_plan(mst_ctx* ctx, const SeqSet& s, const PairPlan& plan, int form, int3
     a.ge = ctx->ge;
     a.transpose_ok = ctx->symmetric ? 1 : 0;
     int grid = score_grid(ctx, b.end - b.begin);
-    const size_t bi = &b - plan.buckets.data();
-    if (timed) CK(hipEventRecord(ctx->launch_events[2 * bi], ctx->stream));
+    if (timed) CK(hipEventRecord(ctx->launch_events[event_base + 2 * bi], ctx->stream));
     launch_score_pairs(b.K, a, grid, ctx->stream);
     CK(hipGetLastError());
-    if (timed) CK(hipEventRecord(ctx->launch_events[2 * bi + 1], ctx->stream));
+    if (timed) CK(hipEventRecord(ctx->launch_events[event_base + 2 * bi + 1], ctx->stream));
     if (form == MST_FORM_PLAIN) CK(hipStreamSynchronize(ctx->stream));
   }
   return MST_OK;
 }
+
+int run_plan(mst_ctx* ctx, const SeqSet& s, const PairPlan& plan, int form, int32_t* d_out_pair,
+             int32_t* d_out_mat, bool timed = false) {
+  return run_plan_on(ctx, s, plan, ctx->d_pairs, form, d_out_pair, d_out_mat, timed, 0);
+}
+
+bool pk16_usable(const mst_ctx* ctx);
+int run_batched(mst_ctx* ctx, int id, int32_t* d_out, bool timed);
+void free_batched(BatchedPlan& P);
 
 int all_pairs_plan(mst_ctx* ctx, int id, PairPlan** out) {
   auto it = ctx->sets.find(id);
@@ -322,20 +350,65 @@ int all_pairs_plan(mst_ctx* ctx, int id, PairPlan** out) {
   return MST_OK;
 }
 
+// All pairs of the set into ctx->d_mat, in the requested form.
+static int all_pairs_device(mst_ctx* ctx, int id, int form, bool timed) {
+  auto it = ctx->sets.find(id);
+  if (it == ctx->sets.end()) return ctx->fail(MST_E_STATE, "unknown sequence set");
+  const SeqSet& s = it->second;
+  size_t nn = (size_t)s.n * s.n;
+  int rc = ctx->ensure((void**)&ctx->d_mat, &ctx->mat_cap, sizeof(int32_t) * std::max<size_t>(nn, 1));
+  if (rc) return rc;
+  if (form == MST_FORM_BATCHED && pk16_usable(ctx)) return run_batched(ctx, id, ctx->d_mat, timed);
+  PairPlan* plan = nullptr;
+  rc = all_pairs_plan(ctx, id, &plan);
+  if (rc) return rc;
+  if (!plan->uploaded || ctx->resident_plan != id) {
+    rc = upload_plan(ctx, *plan);
+    if (rc) return rc;
+    ctx->resident_plan = id;
+  }
+  if (timed) {
+    while (ctx->launch_events.size() < 2 * plan->buckets.size()) {
+      hipEvent_t e;
+      CK(hipEventCreate(&e));
+      ctx->launch_events.push_back(e);
+    }
+  }
+  rc = run_plan(ctx, s, *plan, form, nullptr, ctx->d_mat, timed);
+  if (rc) return rc;
+  if (timed) {
+    CK(hipStreamSynchronize(ctx->stream));
+    ctx->launch_stats.clear();
+    for (size_t b = 0; b < plan->buckets.size(); ++b) {
+      float t = 0;
+      CK(hipEventElapsedTime(&t, ctx->launch_events[2 * b], ctx->launch_events[2 * b + 1]));
+      const Bucket& bk = plan->buckets[b];
+      ctx->launch_stats.push_back(LaunchStat{bk.K, bk.end - bk.begin, bk.cells, t});
+    }
+  }
+  return MST_OK;
+}
+
 extern "C" int mst_all_pairs(mst_ctx* ctx, int id, int form, int32_t* scores, float* dist) {
   if (!ctx || (form != MST_FORM_PLAIN && form != MST_FORM_BATCHED)) return MST_E_ARG;
   CK(hipSetDevice(ctx->device));
-  PairPlan* plan = nullptr;
-  int rc = all_pairs_plan(ctx, id, &plan);
+  {
+    // every entry is written by exactly one pair; start from a sentinel so a
+    // pair the plan missed cannot hide behind an earlier call's value
+    auto it = ctx->sets.find(id);
+    if (it == ctx->sets.end()) return ctx->fail(MST_E_STATE, "unknown sequence set");
+    const int64_t nn = (int64_t)it->second.n * it->second.n;
+    int rc0 = ctx->ensure((void**)&ctx->d_mat, &ctx->mat_cap, sizeof(int32_t) * std::max<int64_t>(nn, 1));
+    if (rc0) return rc0;
+    if (nn > 0)
+      hipLaunchKernelGGL(fill_i32, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, ctx->stream,
+                         ctx->d_mat, nn, (int32_t)INT32_MIN);
+    CK(hipGetLastError());
+  }
+  int rc = all_pairs_device(ctx, id, form, false);
   if (rc) return rc;
   const SeqSet& s = ctx->sets[id];
-  rc = upload_plan(ctx, *plan);
-  if (rc) return rc;
   size_t nn = (size_t)s.n * s.n;
-  rc = ctx->ensure((void**)&ctx->d_mat, &ctx->mat_cap, sizeof(int32_t) * std::max<size_t>(nn, 1));
-  if (rc) return rc;
-  rc = run_plan(ctx, s, *plan, form, nullptr, ctx->d_mat);
-  if (rc) return rc;
   ctx->mat_set = id;
   std::vector<int32_t> tmp;
   int32_t* host = scores;
@@ -353,40 +426,29 @@ extern "C" int mst_all_pairs(mst_ctx* ctx, int id, int form, int32_t* scores, fl
 extern "C" int mst_all_pairs_resident(mst_ctx* ctx, int id, int form, float* ms) {
   if (!ctx || (form != MST_FORM_PLAIN && form != MST_FORM_BATCHED)) return MST_E_ARG;
   CK(hipSetDevice(ctx->device));
-  PairPlan* plan = nullptr;
-  int rc = all_pairs_plan(ctx, id, &plan);
+  if (ctx->sets.find(id) == ctx->sets.end()) return ctx->fail(MST_E_STATE, "unknown sequence set");
+  // plans and device copies are built outside the timed region
+  int rc = all_pairs_device(ctx, id, form, false);
   if (rc) return rc;
-  const SeqSet& s = ctx->sets[id];
-  if (!plan->uploaded || ctx->resident_plan != id) {
-    rc = upload_plan(ctx, *plan);
-    if (rc) return rc;
-    ctx->resident_plan = id;
-  }
-  size_t nn = (size_t)s.n * s.n;
-  rc = ctx->ensure((void**)&ctx->d_mat, &ctx->mat_cap, sizeof(int32_t) * std::max<size_t>(nn, 1));
-  if (rc) return rc;
-  while (ctx->launch_events.size() < 2 * plan->buckets.size()) {
-    hipEvent_t e;
-    CK(hipEventCreate(&e));
-    ctx->launch_events.push_back(e);
-  }
   CK(hipStreamSynchronize(ctx->stream));
   CK(hipEventRecord(ctx->ev0, ctx->stream));
-  rc = run_plan(ctx, s, *plan, form, nullptr, ctx->d_mat, true);
+  rc = all_pairs_device(ctx, id, form, false);
   if (rc) return rc;
   CK(hipEventRecord(ctx->ev1, ctx->stream));
   CK(hipEventSynchronize(ctx->ev1));
   ctx->mat_set = id;
-  ctx->launch_stats.clear();
-  for (size_t b = 0; b < plan->buckets.size(); ++b) {
-    float t = 0;
-    CK(hipEventElapsedTime(&t, ctx->launch_events[2 * b], ctx->launch_events[2 * b + 1]));
-    const Bucket& bk = plan->buckets[b];
-    ctx->launch_stats.push_back(LaunchStat{bk.K, bk.end - bk.begin, bk.cells, t});
-  }
   float t = 0;
   CK(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
   if (ms) *ms = t;
+  return MST_OK;
+}
+
+extern "C" int mst_all_pairs_launches(mst_ctx* ctx, int id, int form) {
+  if (!ctx || (form != MST_FORM_PLAIN && form != MST_FORM_BATCHED)) return MST_E_ARG;
+  CK(hipSetDevice(ctx->device));
+  int rc = all_pairs_device(ctx, id, form, true);
+  if (rc) return rc;
+  ctx->mat_set = id;
   return MST_OK;
 }
 

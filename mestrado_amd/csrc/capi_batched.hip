@@ -1,0 +1,372 @@
+// Batched form of the all-pairs scores: planning, graph capture and replay.
+//
+// Plan (symmetric substitution matrix): sequences are sorted by length; row
+// sequences are paired consecutively (Ya = s[2k], Yb = s[2k+1]) and run against
+// every column sequence at or after s[2k+1] in that order, so each pair i<=j is
+// scored exactly once with the shorter sequence in the rows. A row pair's
+// column list is cut into chunks: one chunk is one work item of the shared-
+// profile kernel (score3), in the rows-per-lane class K = ceil(rows/64). Pairs
+// whose 16-bit bound fails, and row pairs longer than 1024 rows, go to the
+// per-wave multi-strip 16-bit kernel (score2) when their bound allows, else to
+// the int32 kernel. Items are ordered longest first inside every launch; the
+// launches of all classes are captured once into a hipGraph and replayed.
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "../../include/mestrado.h"
+#include "batched.h"
+#include "ctx.h"
+#include "nw_score.h"
+#include "nw_score2.h"
+
+using namespace mst;
+
+#define CK(call)                                                                      \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess) return ctx->fail(MST_E_HIP, #call, hipGetErrorString(e_)); \
+  } while (0)
+
+namespace {
+
+__global__ void zero_u32(unsigned int* p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+}  // namespace
+
+// Largest value a packed 16-bit lane can reach for R padded rows and n columns.
+int64_t pk16_bound(const mst_ctx* ctx, int64_t R, int64_t n) {
+  int smax = 0;
+  for (int k = 0; k < 576; ++k) smax = std::max(smax, (int)ctx->S[k]);
+  return (int64_t)smax * (std::min(R, n) + 1) + (int64_t)ctx->ge * (R + n + 2) + smax + ctx->go + ctx->ge;
+}
+
+bool pk16_usable(const mst_ctx* ctx) {
+  int smin = 127, smax = -128;
+  for (int k = 0; k < 576; ++k) {
+    smin = std::min(smin, (int)ctx->S[k]);
+    smax = std::max(smax, (int)ctx->S[k]);
+  }
+  // profile bytes S+go+ge are zero-extended: they must lie in [0, 255]
+  return ctx->symmetric && smin + ctx->go + ctx->ge >= 0 && smax + ctx->go + ctx->ge <= 255;
+}
+
+static int rows_per_lane(int m, int* nstrips) {
+  int s = (m + 1023) / 1024;
+  if (s < 1) s = 1;
+  *nstrips = s;
+  int K = (m + 64 * s - 1) / (64 * s);
+  return K < 1 ? 1 : K;
+}
+
+int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
+  const int N = s.n;
+  auto len = [&](int i) { return (int)(s.h_off[i + 1] - s.h_off[i]); };
+  P = BatchedPlan();
+  std::vector<int32_t> fb_i, fb_j;  // int32 fallback pairs
+  std::vector<int> order(N);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return len(a) < len(b); });
+  P.xlist = order;
+  const int C = ctx->tune.chunk;
+  struct It3 {
+    Item3 it;
+    int K;
+    int64_t cost, cells, pairs;
+  };
+  struct It2 {
+    Pair2 it;
+    int K;
+    int64_t cost, cells, pairs;
+  };
+  std::vector<It3> v3;
+  std::vector<It2> v2;
+  for (int k = 0; 2 * k < N; ++k) {
+    const int ya = order[2 * k];
+    const bool single = (2 * k + 1 >= N);
+    const int yb = single ? -1 : order[2 * k + 1];
+    const int m = single ? len(ya) : len(yb);
+    if (m == 0) {  // empty rows: score is -ge*n, no DP needed
+      for (int t = 2 * k; t < N; ++t) {
+        fb_i.push_back(order[t]);
+        fb_j.push_back(ya);
+        if (!single && t >= 2 * k + 1) {
+          fb_i.push_back(order[t]);
+          fb_j.push_back(yb);
+        }
+      }
+      continue;
+    }
+    int nstr;
+    const int K = rows_per_lane(m, &nstr);
+    const int64_t R = (int64_t)nstr * 64 * K;
+    // rows (ya, yb) against columns s[first..last]; yb < 0: ya alone
+    auto add_cols = [&](int first, int last, int ya_, int yb_) {
+      std::vector<int> elig;
+      for (int t = first; t <= last; ++t) {
+        const int x = order[t];
+        const int n = len(x);
+        if (n == 0 || pk16_bound(ctx, R, n) > 32767) {
+          fb_i.push_back(x);
+          fb_j.push_back(ya_);
+          if (yb_ >= 0) {
+            fb_i.push_back(x);
+            fb_j.push_back(yb_);
+          }
+        } else {
+          elig.push_back(t);
+        }
+      }
+      if (nstr == 1) {
+        size_t c0 = 0;
+        while (c0 < elig.size()) {
+          size_t c1 = c0 + 1;  // a chunk is a run of consecutive positions
+          while (c1 < elig.size() && (int)(c1 - c0) < C && elig[c1] == elig[c1 - 1] + 1) ++c1;
+          Item3 it;
+          it.ya = ya_;
+          it.yb = yb_;
+          it.xbegin = elig[c0];
+          it.xcount = (int)(c1 - c0);
+          int64_t cost = 0, cells = 0;
+          const int64_t rows = len(ya_) + (yb_ >= 0 ? len(yb_) : 0);
+          for (size_t q = c0; q < c1; ++q) {
+            cost += (int64_t)(len(order[elig[q]]) + 63) * 64 * K;
+            cells += (int64_t)len(order[elig[q]]) * rows;
+          }
+          v3.push_back(It3{it, K, cost, cells, (int64_t)(c1 - c0) * (yb_ >= 0 ? 2 : 1)});
+          c0 = c1;
+        }
+      } else {
+        for (int t : elig) {
+          Pair2 it;
+          it.x = order[t];
+          it.ya = ya_;
+          it.yb = yb_;
+          it.pad = 0;
+          it.oa = it.ob = -1;
+          const int64_t rows = len(ya_) + (yb_ >= 0 ? len(yb_) : 0);
+          v2.push_back(It2{it, K, (int64_t)(len(order[t]) + 63) * R, (int64_t)len(order[t]) * rows,
+                           yb_ >= 0 ? 2 : 1});
+        }
+      }
+    };
+    if (single) {
+      add_cols(2 * k, N - 1, ya, -1);
+    } else {
+      add_cols(2 * k, 2 * k, ya, -1);          // (ya, ya)
+      add_cols(2 * k + 1, N - 1, ya, yb);      // every x at or after yb, both rows
+    }
+  }
+  auto by_class = [](auto& v) {
+    std::stable_sort(v.begin(), v.end(), [](const auto& a, const auto& b) {
+      if (a.K != b.K) return a.K < b.K;
+      return a.cost > b.cost;
+    });
+  };
+  by_class(v3);
+  by_class(v2);
+  for (size_t q = 0; q < v3.size(); ++q) {
+    P.items3.push_back(v3[q].it);
+    if (P.c3.empty() || P.c3.back().K != v3[q].K)
+      P.c3.push_back(BClass{v3[q].K, (int64_t)q, (int64_t)q + 1, 0});
+    else
+      P.c3.back().end = q + 1;
+    P.c3.back().work += v3[q].cost;
+    P.c3.back().cells += v3[q].cells;
+    P.c3.back().pairs += v3[q].pairs;
+  }
+  for (size_t q = 0; q < v2.size(); ++q) {
+    P.items2.push_back(v2[q].it);
+    if (P.c2.empty() || P.c2.back().K != v2[q].K)
+      P.c2.push_back(BClass{v2[q].K, (int64_t)q, (int64_t)q + 1, 0});
+    else
+      P.c2.back().end = q + 1;
+    P.c2.back().work += v2[q].cost;
+    P.c2.back().cells += v2[q].cells;
+    P.c2.back().pairs += v2[q].pairs;
+  }
+  P.fb_i = fb_i;
+  P.fb_j = fb_j;
+  P.built = true;
+  return MST_OK;
+}
+
+// defined in capi.hip
+int build_plan(mst_ctx* ctx, const SeqSet& s, const int32_t* pi, const int32_t* pj, int64_t np,
+               PairPlan& plan);
+int run_plan_on(mst_ctx* ctx, const SeqSet& s, const PairPlan& plan, const int2* d_pairs, int form,
+                int32_t* d_out_pair, int32_t* d_out_mat, bool timed, int event_base);
+
+static int grid3(mst_ctx* ctx, int K, int64_t items) {
+  int per = ctx->tune.wg_per_cu3[K < 17 ? K : 16];
+  if (per <= 0) per = K <= 4 ? 8 : (K <= 8 ? 6 : 3);
+  return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ctx->num_cus * per, items));
+}
+
+static int grid2(mst_ctx* ctx, int64_t items) {
+  int per = ctx->tune.wg_per_cu2 > 0 ? ctx->tune.wg_per_cu2 : 8;
+  return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ctx->num_cus * per, items));
+}
+
+int upload_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P) {
+  if (P.uploaded) return MST_OK;
+  CK(hipMalloc(&P.d_items3, sizeof(Item3) * std::max<size_t>(P.items3.size(), 1)));
+  CK(hipMalloc(&P.d_items2, sizeof(Pair2) * std::max<size_t>(P.items2.size(), 1)));
+  CK(hipMalloc(&P.d_xlist, sizeof(int32_t) * std::max<size_t>(P.xlist.size(), 1)));
+  P.ncounters = (int)(P.c3.size() + P.c2.size()) + 1;
+  CK(hipMalloc(&P.d_counters, sizeof(unsigned int) * P.ncounters));
+  if (!P.items3.empty())
+    CK(hipMemcpy(P.d_items3, P.items3.data(), sizeof(Item3) * P.items3.size(), hipMemcpyHostToDevice));
+  if (!P.items2.empty())
+    CK(hipMemcpy(P.d_items2, P.items2.data(), sizeof(Pair2) * P.items2.size(), hipMemcpyHostToDevice));
+  if (!P.xlist.empty())
+    CK(hipMemcpy(P.d_xlist, P.xlist.data(), sizeof(int32_t) * P.xlist.size(), hipMemcpyHostToDevice));
+  PairPlan& fb = ctx->fbplans[id];
+  fb = PairPlan();
+  if (!P.fb_i.empty()) {
+    int rc = build_plan(ctx, s, P.fb_i.data(), P.fb_j.data(), (int64_t)P.fb_i.size(), fb);
+    if (rc) return rc;
+    CK(hipMalloc(&fb.d_pairs, sizeof(int2) * fb.pairs.size()));
+    CK(hipMemcpy(fb.d_pairs, fb.pairs.data(), sizeof(int2) * fb.pairs.size(), hipMemcpyHostToDevice));
+  }
+  P.uploaded = true;
+  return MST_OK;
+}
+
+// Enqueues every launch of the batched plan on ctx->stream. With timed, an
+// event pair brackets each launch (ctx->launch_events from event_base).
+int enqueue_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P, int32_t* d_out,
+                    bool timed) {
+  hipLaunchKernelGGL(zero_u32, dim3((P.ncounters + 63) / 64), dim3(64), 0, ctx->stream, P.d_counters,
+                     P.ncounters);
+  CK(hipGetLastError());
+  int ev = 0, ctr = 0;
+  for (const BClass& c : P.c3) {
+    Score3Args a;
+    a.seqs = s.d_codes;
+    a.off = s.d_off;
+    a.items = P.d_items3;
+    a.first = c.begin;
+    a.last = c.end;
+    a.counter = P.d_counters + ctr++;
+    a.xlist = P.d_xlist;
+    a.out_mat = d_out;
+    a.nseq = s.n;
+    a.subst = ctx->d_S;
+    a.go = ctx->go;
+    a.ge = ctx->ge;
+    if (timed) CK(hipEventRecord(ctx->launch_events[ev++], ctx->stream));
+    launch_score3(c.K, a, grid3(ctx, c.K, c.end - c.begin), ctx->stream);
+    CK(hipGetLastError());
+    if (timed) CK(hipEventRecord(ctx->launch_events[ev++], ctx->stream));
+  }
+  if (!P.c2.empty()) {
+    int64_t bufcap = std::max(s.maxlen, 1);
+    int g = grid2(ctx, (int64_t)P.items2.size());
+    int rc = ctx->ensure((void**)&ctx->d_bufs, &ctx->bufs_cap, sizeof(int2) * 2 * bufcap * (size_t)g);
+    if (rc) return rc;
+    for (const BClass& c : P.c2) {
+      Score2Args a;
+      a.seqs = s.d_codes;
+      a.off = s.d_off;
+      a.items = P.d_items2;
+      a.first = c.begin;
+      a.last = c.end;
+      a.counter = P.d_counters + ctr++;
+      a.out_mat = d_out;
+      a.nseq = s.n;
+      a.out_pair = nullptr;
+      a.bufs = ctx->d_bufs;
+      a.bufcap = bufcap;
+      a.subst = ctx->d_S;
+      a.go = ctx->go;
+      a.ge = ctx->ge;
+      if (timed) CK(hipEventRecord(ctx->launch_events[ev++], ctx->stream));
+      launch_score2(c.K, a, std::min(g, grid2(ctx, c.end - c.begin)), ctx->stream);
+      CK(hipGetLastError());
+      if (timed) CK(hipEventRecord(ctx->launch_events[ev++], ctx->stream));
+    }
+  }
+  const PairPlan& fb = ctx->fbplans[id];
+  if (!fb.pairs.empty()) {
+    int rc = run_plan_on(ctx, s, fb, fb.d_pairs, MST_FORM_BATCHED, nullptr, d_out, timed, ev);
+    if (rc) return rc;
+  }
+  return MST_OK;
+}
+
+int batched_launch_count(mst_ctx* ctx, int id) {
+  const BatchedPlan& P = ctx->bplans[id];
+  return (int)(P.c3.size() + P.c2.size() + ctx->fbplans[id].buckets.size());
+}
+
+// All pairs of a set in the batched form into d_out (n x n). timed: plain
+// enqueue with per-launch events; otherwise the captured graph is replayed.
+int run_batched(mst_ctx* ctx, int id, int32_t* d_out, bool timed) {
+  auto it = ctx->sets.find(id);
+  if (it == ctx->sets.end()) return ctx->fail(MST_E_STATE, "unknown sequence set");
+  const SeqSet& s = it->second;
+  BatchedPlan& P = ctx->bplans[id];
+  if (!P.built) {
+    int rc = build_batched_plan(ctx, s, P);
+    if (rc) return rc;
+  }
+  int rc = upload_batched(ctx, s, id, P);
+  if (rc) return rc;
+  if (timed) {
+    const int nl = batched_launch_count(ctx, id);
+    while ((int)ctx->launch_events.size() < 2 * nl) {
+      hipEvent_t e;
+      CK(hipEventCreate(&e));
+      ctx->launch_events.push_back(e);
+    }
+    rc = enqueue_batched(ctx, s, id, P, d_out, true);
+    if (rc) return rc;
+    CK(hipStreamSynchronize(ctx->stream));
+    ctx->launch_stats.clear();
+    int ev = 0;
+    auto rec = [&](int K, int64_t pairs, int64_t cells) -> int {
+      float t = 0;
+      CK(hipEventElapsedTime(&t, ctx->launch_events[ev], ctx->launch_events[ev + 1]));
+      ev += 2;
+      ctx->launch_stats.push_back(LaunchStat{K, pairs, cells, t});
+      return MST_OK;
+    };
+    for (const BClass& c : P.c3)
+      if ((rc = rec(-c.K, c.pairs, c.cells))) return rc;
+    for (const BClass& c : P.c2)
+      if ((rc = rec(-100 - c.K, c.pairs, c.cells))) return rc;
+    for (const Bucket& b : ctx->fbplans[id].buckets)
+      if ((rc = rec(b.K, b.end - b.begin, b.cells))) return rc;
+    return MST_OK;
+  }
+  if (!P.graph || P.graph_out != d_out) {
+    if (P.graph) {
+      CK(hipGraphExecDestroy(P.graph));
+      P.graph = nullptr;
+    }
+    hipGraph_t g;
+    CK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    rc = enqueue_batched(ctx, s, id, P, d_out, false);
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc) return rc;
+    if (e != hipSuccess) return ctx->fail(MST_E_HIP, "hipStreamEndCapture", hipGetErrorString(e));
+    CK(hipGraphInstantiate(&P.graph, g, nullptr, nullptr, 0));
+    CK(hipGraphDestroy(g));
+    P.graph_out = d_out;
+  }
+  CK(hipGraphLaunch(P.graph, ctx->stream));
+  return MST_OK;
+}
+
+void free_batched(BatchedPlan& P) {
+  if (P.graph) hipGraphExecDestroy(P.graph);
+  void* ps[] = {P.d_items3, P.d_items2, P.d_xlist, P.d_counters};
+  for (void* p : ps)
+    if (p) hipFree(p);
+  P = BatchedPlan();
+}

@@ -7,6 +7,8 @@
 #include <string>
 #include <vector>
 
+#include "batched.h"
+
 struct SeqSet {
   int n = 0;
   int64_t total = 0;
@@ -26,7 +28,7 @@ struct Bucket {
 };
 
 struct LaunchStat {
-  int K;
+  int K;  // rows per lane; negative for the packed 16-bit kernels
   int64_t pairs, cells;
   float ms;
 };
@@ -34,6 +36,7 @@ struct LaunchStat {
 // A bucketed pair list (pairs sorted by bucket, longest first in a bucket).
 struct PairPlan {
   bool built = false, uploaded = false;
+  int2* d_pairs = nullptr;  // own device copy (fallback plans of the batched form)
   std::vector<int2> pairs;
   std::vector<int64_t> index;  // plan position -> caller's pair index
   std::vector<Bucket> buckets;
@@ -52,7 +55,10 @@ struct mst_ctx {
   int8_t* d_S = nullptr;
   std::string err;
   std::map<int, SeqSet> sets;
-  std::map<int, PairPlan> plans;  // all-pairs plan per set
+  std::map<int, PairPlan> plans;  // all-pairs plan per set (plain form)
+  std::map<int, BatchedPlan> bplans;  // all-pairs plan per set (batched form)
+  std::map<int, PairPlan> fbplans;    // int32 fallback pairs of the batched plan
+  Tune tune;
   int next_set = 1;
   int resident_plan = -1;
 

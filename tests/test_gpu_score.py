@@ -109,3 +109,31 @@ def test_asymmetric_matrix_keeps_roles(ctx):
     o, _ = O.pair_list(codes, offs, pi.ravel(), pj.ravel(), S2)
     assert (g == o).all()
     assert (g.reshape(10, 10) != g.reshape(10, 10).T).any()
+
+
+def test_config2_subset_all_pairs_batched(ctx):
+    # the 10 longest sequences (up to 5,000 residues: multi-strip and int32
+    # fallback pairs) plus 250 random ones, every pair, both forms
+    seqs = M.synth(2, 0)
+    L = np.array([len(s) for s in seqs])
+    rng = np.random.default_rng(21)
+    idx = np.concatenate([np.argsort(-L)[:10], rng.choice(len(seqs), 250, replace=False)])
+    idx = np.unique(idx)
+    sub = [seqs[i] for i in idx]
+    st = ctx.upload(sub)
+    o, _ = O.all_pairs(sub, S)
+    for form in (M.FORM_BATCHED, M.FORM_PLAIN):
+        g = st.all_pairs(form)
+        bad = np.argwhere(g != o)
+        assert len(bad) == 0, [(len(sub[i]), len(sub[j]), g[i, j], o[i, j]) for i, j in bad[:8]]
+    st.free()
+
+
+def test_batched_launch_stats_cover_all_cells(ctx):
+    seqs = M.synth(0, 1)
+    st = ctx.upload(seqs)
+    stats = st.launch_stats(M.FORM_BATCHED)
+    L = np.array([len(s) for s in seqs], dtype=np.int64)
+    total = (L.sum() ** 2 + (L * L).sum()) // 2  # pairs i <= j
+    assert sum(r[2] for r in stats) == total
+    assert any(r[0] < 0 for r in stats)  # the packed kernels ran
