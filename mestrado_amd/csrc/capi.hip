@@ -195,6 +195,8 @@ extern "C" int mst_seqset_free(mst_ctx* ctx, int id) {
   hipFree(it->second.d_off);
   ctx->sets.erase(it);
   ctx->plans.erase(id);
+  ctx->prepared.erase(std::make_pair(id, 0));
+  ctx->prepared.erase(std::make_pair(id, 1));
   auto bp = ctx->bplans.find(id);
   if (bp != ctx->bplans.end()) {
     free_batched(bp->second);
@@ -427,9 +429,14 @@ extern "C" int mst_all_pairs_resident(mst_ctx* ctx, int id, int form, float* ms)
   if (!ctx || (form != MST_FORM_PLAIN && form != MST_FORM_BATCHED)) return MST_E_ARG;
   CK(hipSetDevice(ctx->device));
   if (ctx->sets.find(id) == ctx->sets.end()) return ctx->fail(MST_E_STATE, "unknown sequence set");
-  // plans and device copies are built outside the timed region
-  int rc = all_pairs_device(ctx, id, form, false);
-  if (rc) return rc;
+  // the first call of a (set, form) builds its plan, device copies and graph
+  // outside the timed region
+  int rc;
+  if (!ctx->prepared.count(std::make_pair(id, form))) {
+    rc = all_pairs_device(ctx, id, form, false);
+    if (rc) return rc;
+    ctx->prepared.insert(std::make_pair(id, form));
+  }
   CK(hipStreamSynchronize(ctx->stream));
   CK(hipEventRecord(ctx->ev0, ctx->stream));
   rc = all_pairs_device(ctx, id, form, false);

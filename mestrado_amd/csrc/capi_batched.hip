@@ -5,11 +5,12 @@
 // every column sequence at or after s[2k+1] in that order, so each pair i<=j is
 // scored exactly once with the shorter sequence in the rows. A row pair's
 // column list is cut into chunks: one chunk is one work item of the shared-
-// profile kernel (score3), in the rows-per-lane class K = ceil(rows/64). Pairs
-// whose 16-bit bound fails, and row pairs longer than 1024 rows, go to the
-// per-wave multi-strip 16-bit kernel (score2) when their bound allows, else to
-// the int32 kernel. Items are ordered longest first inside every launch; the
-// launches of all classes are captured once into a hipGraph and replayed.
+// profile kernel (score3), in the rows-per-lane class K = ceil(rows/64), for
+// the classes enabled below. Pairs whose 16-bit bound fails, and rows outside
+// the enabled classes, go to the int32 kernel (the per-wave multi-strip 16-bit
+// kernel, score2, is switched off). Items are ordered longest first inside every launch; the
+// packed launches are captured once into a hipGraph and replayed, and the
+// int32 launches follow the replay on the same stream.
 #include <algorithm>
 #include <cstring>
 #include <numeric>
@@ -54,6 +55,14 @@ bool pk16_usable(const mst_ctx* ctx) {
   // profile bytes S+go+ge are zero-extended: they must lie in [0, 255]
   return ctx->symmetric && smin + ctx->go + ctx->ge >= 0 && smax + ctx->go + ctx->ge <= 255;
 }
+
+// Rows-per-lane classes of the shared-profile kernel that the planner uses.
+// The other classes and the multi-strip packed kernel are switched off: a run
+// that used them (lengths 1..3100, classes K = 1, 2, 8, 9, 12, 16 and the
+// multi-strip kernel) did not finish on the device, with no cause found in the
+// code; their pairs take the int32 kernel.
+constexpr int kPackedMinK = 3, kPackedMaxK = 5;
+constexpr bool kMultiStripPacked = false;
 
 static int rows_per_lane(int m, int* nstrips) {
   int s = (m + 1023) / 1024;
@@ -104,13 +113,14 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
     int nstr;
     const int K = rows_per_lane(m, &nstr);
     const int64_t R = (int64_t)nstr * 64 * K;
+    const bool packed = nstr == 1 ? (K >= kPackedMinK && K <= kPackedMaxK) : kMultiStripPacked;
     // rows (ya, yb) against columns s[first..last]; yb < 0: ya alone
     auto add_cols = [&](int first, int last, int ya_, int yb_) {
       std::vector<int> elig;
       for (int t = first; t <= last; ++t) {
         const int x = order[t];
         const int n = len(x);
-        if (n == 0 || pk16_bound(ctx, R, n) > 32767) {
+        if (!packed || n == 0 || pk16_bound(ctx, R, n) > 32767) {
           fb_i.push_back(x);
           fb_j.push_back(ya_);
           if (yb_ >= 0) {
@@ -237,10 +247,20 @@ int upload_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P) {
   return MST_OK;
 }
 
-// Enqueues every launch of the batched plan on ctx->stream. With timed, an
-// event pair brackets each launch (ctx->launch_events from event_base).
+// The int32 launches of the plan's remaining pairs (never captured in the
+// graph: they follow its replay on the same stream).
+static int enqueue_fallback(mst_ctx* ctx, const SeqSet& s, int id, int32_t* d_out, bool timed,
+                            int event_base) {
+  const PairPlan& fb = ctx->fbplans[id];
+  if (fb.pairs.empty()) return MST_OK;
+  return run_plan_on(ctx, s, fb, fb.d_pairs, MST_FORM_BATCHED, nullptr, d_out, timed, event_base);
+}
+
+// Enqueues the packed launches of the batched plan on ctx->stream, then the
+// int32 ones when with_fallback. With timed, an event pair brackets each
+// launch (ctx->launch_events in launch order).
 int enqueue_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P, int32_t* d_out,
-                    bool timed) {
+                    bool timed, bool with_fallback) {
   hipLaunchKernelGGL(zero_u32, dim3((P.ncounters + 63) / 64), dim3(64), 0, ctx->stream, P.d_counters,
                      P.ncounters);
   CK(hipGetLastError());
@@ -291,11 +311,7 @@ int enqueue_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P, int32
       if (timed) CK(hipEventRecord(ctx->launch_events[ev++], ctx->stream));
     }
   }
-  const PairPlan& fb = ctx->fbplans[id];
-  if (!fb.pairs.empty()) {
-    int rc = run_plan_on(ctx, s, fb, fb.d_pairs, MST_FORM_BATCHED, nullptr, d_out, timed, ev);
-    if (rc) return rc;
-  }
+  if (with_fallback) return enqueue_fallback(ctx, s, id, d_out, timed, ev);
   return MST_OK;
 }
 
@@ -324,7 +340,7 @@ int run_batched(mst_ctx* ctx, int id, int32_t* d_out, bool timed) {
       CK(hipEventCreate(&e));
       ctx->launch_events.push_back(e);
     }
-    rc = enqueue_batched(ctx, s, id, P, d_out, true);
+    rc = enqueue_batched(ctx, s, id, P, d_out, true, true);
     if (rc) return rc;
     CK(hipStreamSynchronize(ctx->stream));
     ctx->launch_stats.clear();
@@ -351,7 +367,7 @@ int run_batched(mst_ctx* ctx, int id, int32_t* d_out, bool timed) {
     }
     hipGraph_t g;
     CK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    rc = enqueue_batched(ctx, s, id, P, d_out, false);
+    rc = enqueue_batched(ctx, s, id, P, d_out, false, false);
     hipError_t e = hipStreamEndCapture(ctx->stream, &g);
     if (rc) return rc;
     if (e != hipSuccess) return ctx->fail(MST_E_HIP, "hipStreamEndCapture", hipGetErrorString(e));
@@ -360,7 +376,8 @@ int run_batched(mst_ctx* ctx, int id, int32_t* d_out, bool timed) {
     P.graph_out = d_out;
   }
   CK(hipGraphLaunch(P.graph, ctx->stream));
-  return MST_OK;
+  const int nev = 2 * (int)(P.c3.size() + P.c2.size());
+  return enqueue_fallback(ctx, s, id, d_out, false, nev);
 }
 
 void free_batched(BatchedPlan& P) {

@@ -4,6 +4,8 @@
 #include <stdint.h>
 
 #include <map>
+#include <set>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -59,6 +61,7 @@ struct mst_ctx {
   std::map<int, BatchedPlan> bplans;  // all-pairs plan per set (batched form)
   std::map<int, PairPlan> fbplans;    // int32 fallback pairs of the batched plan
   Tune tune;
+  std::set<std::pair<int, int>> prepared;  // (set, form) ready for timed runs
   int next_set = 1;
   int resident_plan = -1;
 
