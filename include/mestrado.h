@@ -75,11 +75,17 @@ int mst_all_pairs_resident(mst_ctx* ctx, int set_id, int form, float* kernel_ms)
 int mst_all_pairs_launches(mst_ctx* ctx, int set_id, int form);
 /* Per-launch record of the last mst_all_pairs_launches call: rows per lane of
  * the kernel instance (negative: -K for the shared-profile 16-bit kernel,
- * -100-K for the multi-strip 16-bit kernel), pairs, DP cells and device
+ * -100-K for the multi-strip 16-bit kernel, -(1000+100*W+K) for the pipelined
+ * 16-bit kernel with W waves of K rows per lane), pairs, DP cells and device
  * milliseconds of each launch (arrays of cap entries; *count receives the
  * number of launches). */
 int mst_last_launch_stats(mst_ctx* ctx, int* rows_per_lane, int64_t* pairs, int64_t* cells,
                           float* ms, int cap, int* count);
+/* Measurement helper (no counterpart in the reference): integer VALU rate of
+ * the context's device in T ops/s, int32 (packed = 0) or packed int16
+ * (packed = 1, two ops per lane and instruction), from a kernel of
+ * independent add/max chains. */
+int mst_valu_peak(mst_ctx* ctx, int packed, double* tops);
 
 
 /* Replaces AlignmentMatch::computePairwise (AlignmentMatch.cu:140-317) with

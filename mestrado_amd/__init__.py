@@ -44,6 +44,7 @@ def lib():
         L.mst_all_pairs_resident.argtypes = [P, C.c_int, C.c_int, C.POINTER(C.c_float)]
         L.mst_all_pairs_launches.argtypes = [P, C.c_int, C.c_int]
         L.mst_last_launch_stats.argtypes = [P, P, P, P, P, C.c_int, C.POINTER(C.c_int)]
+        L.mst_valu_peak.argtypes = [P, C.c_int, C.POINTER(C.c_double)]
         L.mst_align_pairs.argtypes = [P, C.c_int, P, P, C.c_int64, P, P, P, C.c_int64]
         L.mst_msa.argtypes = [P, C.c_int, P, C.POINTER(C.c_int)]
         L.mst_msa_rows.argtypes = [P, P, C.c_int64]
@@ -143,6 +144,12 @@ class Context:
         hbm = C.c_int64(0)
         self._ck(self._L.mst_device_info(self.h, C.byref(cus), C.byref(hbm)), "mst_device_info")
         return cus.value, hbm.value
+
+    def valu_peak(self, packed=False):
+        """Measured integer VALU rate of the device in T ops/s (int32, or packed int16)."""
+        t = C.c_double(0)
+        self._ck(self._L.mst_valu_peak(self.h, 1 if packed else 0, C.byref(t)), "mst_valu_peak")
+        return t.value
 
     def nj(self, D, ratio=0.5):
         """Neighbour joining of an n x n float matrix -> list of Join."""

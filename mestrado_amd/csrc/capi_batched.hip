@@ -21,6 +21,7 @@
 #include "ctx.h"
 #include "nw_score.h"
 #include "nw_score2.h"
+#include "nw_score5.h"
 
 using namespace mst;
 
@@ -35,6 +36,18 @@ namespace {
 __global__ void zero_u32(unsigned int* p, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0u;
+}
+
+// codes of the sequence at sorted position blockIdx.x into its place in the
+// length-ordered copy
+__global__ void gather_codes(const int8_t* codes, const int64_t* off, const int32_t* order,
+                             const int64_t* soff, int8_t* out) {
+  const int k = blockIdx.x;
+  const int id = order[k];
+  const int64_t n = soff[k + 1] - soff[k];
+  const int8_t* src = codes + off[id];
+  int8_t* dst = out + soff[k];
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
 }
 
 }  // namespace
@@ -63,6 +76,9 @@ bool pk16_usable(const mst_ctx* ctx) {
 // code; their pairs take the int32 kernel.
 constexpr int kPackedMinK = 3, kPackedMaxK = 5;
 constexpr bool kMultiStripPacked = false;
+// Single-strip rows beyond the classes above (321..1024 rows) go to the
+// pipelined packed kernel (score5): W = ceil(rows/256) waves of 64*K rows.
+constexpr bool kPipelined = true;
 
 static int rows_per_lane(int m, int* nstrips) {
   int s = (m + 1023) / 1024;
@@ -84,7 +100,7 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
   const int C = ctx->tune.chunk;
   struct It3 {
     Item3 it;
-    int K;
+    int K, W;
     int64_t cost, cells, pairs;
   };
   struct It2 {
@@ -92,7 +108,7 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
     int K;
     int64_t cost, cells, pairs;
   };
-  std::vector<It3> v3;
+  std::vector<It3> v3, v5;
   std::vector<It2> v2;
   for (int k = 0; 2 * k < N; ++k) {
     const int ya = order[2 * k];
@@ -111,9 +127,22 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
       continue;
     }
     int nstr;
-    const int K = rows_per_lane(m, &nstr);
-    const int64_t R = (int64_t)nstr * 64 * K;
-    const bool packed = nstr == 1 ? (K >= kPackedMinK && K <= kPackedMaxK) : kMultiStripPacked;
+    int K = rows_per_lane(m, &nstr);
+    int W = 1;   // waves per pipeline (score5)
+    int kind;    // 3: score3, 5: score5, 2: score2, 0: int32 kernel
+    if (nstr == 1 && K >= kPackedMinK && K <= kPackedMaxK) {
+      kind = 3;
+    } else if (nstr == 1 && m > 64 * kPackedMaxK && kPipelined) {
+      W = (m + 255) / 256;
+      K = (m + 64 * W - 1) / (64 * W);
+      kind = 5;
+    } else if (nstr > 1 && kMultiStripPacked) {
+      kind = 2;
+    } else {
+      kind = 0;
+    }
+    const int64_t R = kind == 5 ? (int64_t)W * 64 * K : (int64_t)nstr * 64 * K;
+    const bool packed = kind != 0;
     // rows (ya, yb) against columns s[first..last]; yb < 0: ya alone
     auto add_cols = [&](int first, int last, int ya_, int yb_) {
       std::vector<int> elig;
@@ -131,7 +160,7 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
           elig.push_back(t);
         }
       }
-      if (nstr == 1) {
+      if (kind == 3 || kind == 5) {
         size_t c0 = 0;
         while (c0 < elig.size()) {
           size_t c1 = c0 + 1;  // a chunk is a run of consecutive positions
@@ -147,7 +176,11 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
             cost += (int64_t)(len(order[elig[q]]) + 63) * 64 * K;
             cells += (int64_t)len(order[elig[q]]) * rows;
           }
-          v3.push_back(It3{it, K, cost, cells, (int64_t)(c1 - c0) * (yb_ >= 0 ? 2 : 1)});
+          const int64_t npairs = (int64_t)(c1 - c0) * (yb_ >= 0 ? 2 : 1);
+          if (kind == 3)
+            v3.push_back(It3{it, K, 1, cost, cells, npairs});
+          else
+            v5.push_back(It3{it, K, W, cost * W, cells, npairs});
           c0 = c1;
         }
       } else {
@@ -179,6 +212,24 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
   };
   by_class(v3);
   by_class(v2);
+  std::stable_sort(v5.begin(), v5.end(), [](const It3& a, const It3& b) {
+    if (a.W != b.W) return a.W < b.W;
+    if (a.K != b.K) return a.K < b.K;
+    return a.cost > b.cost;
+  });
+  for (size_t q = 0; q < v5.size(); ++q) {
+    P.items5.push_back(v5[q].it);
+    if (P.c5.empty() || P.c5.back().K != v5[q].K || P.c5.back().W != v5[q].W) {
+      BClass c{v5[q].K, (int64_t)q, (int64_t)q + 1};
+      c.W = v5[q].W;
+      P.c5.push_back(c);
+    } else {
+      P.c5.back().end = q + 1;
+    }
+    P.c5.back().work += v5[q].cost;
+    P.c5.back().cells += v5[q].cells;
+    P.c5.back().pairs += v5[q].pairs;
+  }
   for (size_t q = 0; q < v3.size(); ++q) {
     P.items3.push_back(v3[q].it);
     if (P.c3.empty() || P.c3.back().K != v3[q].K)
@@ -227,7 +278,18 @@ int upload_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P) {
   CK(hipMalloc(&P.d_items3, sizeof(Item3) * std::max<size_t>(P.items3.size(), 1)));
   CK(hipMalloc(&P.d_items2, sizeof(Pair2) * std::max<size_t>(P.items2.size(), 1)));
   CK(hipMalloc(&P.d_xlist, sizeof(int32_t) * std::max<size_t>(P.xlist.size(), 1)));
-  P.ncounters = (int)(P.c3.size() + P.c2.size()) + 1;
+  P.ncounters = (int)(P.c3.size() + P.c2.size() + P.c5.size()) + 1;
+  if (!P.c5.empty()) {
+    const int N = s.n;
+    std::vector<int64_t> soff(N + 1, 0);
+    for (int k = 0; k < N; ++k)
+      soff[k + 1] = soff[k] + (s.h_off[P.xlist[k] + 1] - s.h_off[P.xlist[k]]);
+    CK(hipMalloc(&P.d_items5, sizeof(Item3) * P.items5.size()));
+    CK(hipMemcpy(P.d_items5, P.items5.data(), sizeof(Item3) * P.items5.size(), hipMemcpyHostToDevice));
+    CK(hipMalloc(&P.d_soff, sizeof(int64_t) * (N + 1)));
+    CK(hipMemcpy(P.d_soff, soff.data(), sizeof(int64_t) * (N + 1), hipMemcpyHostToDevice));
+    CK(hipMalloc(&P.d_scodes, std::max<int64_t>(soff[N], 1)));
+  }
   CK(hipMalloc(&P.d_counters, sizeof(unsigned int) * P.ncounters));
   if (!P.items3.empty())
     CK(hipMemcpy(P.d_items3, P.items3.data(), sizeof(Item3) * P.items3.size(), hipMemcpyHostToDevice));
@@ -235,6 +297,12 @@ int upload_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P) {
     CK(hipMemcpy(P.d_items2, P.items2.data(), sizeof(Pair2) * P.items2.size(), hipMemcpyHostToDevice));
   if (!P.xlist.empty())
     CK(hipMemcpy(P.d_xlist, P.xlist.data(), sizeof(int32_t) * P.xlist.size(), hipMemcpyHostToDevice));
+  if (!P.c5.empty()) {
+    hipLaunchKernelGGL(gather_codes, dim3(s.n), dim3(256), 0, ctx->stream, s.d_codes, s.d_off,
+                       P.d_xlist, P.d_soff, P.d_scodes);
+    CK(hipGetLastError());
+    CK(hipStreamSynchronize(ctx->stream));
+  }
   PairPlan& fb = ctx->fbplans[id];
   fb = PairPlan();
   if (!P.fb_i.empty()) {
@@ -284,6 +352,29 @@ int enqueue_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P, int32
     CK(hipGetLastError());
     if (timed) CK(hipEventRecord(ctx->launch_events[ev++], ctx->stream));
   }
+  for (const BClass& c : P.c5) {
+    Score5Args a;
+    a.scodes = P.d_scodes;
+    a.soff = P.d_soff;
+    a.order = P.d_xlist;
+    a.seqs = s.d_codes;
+    a.off = s.d_off;
+    a.items = P.d_items5;
+    a.first = c.begin;
+    a.last = c.end;
+    a.counter = P.d_counters + ctr++;
+    a.out_mat = d_out;
+    a.nseq = s.n;
+    a.subst = ctx->d_S;
+    a.go = ctx->go;
+    a.ge = ctx->ge;
+    const int64_t g = std::min<int64_t>((int64_t)ctx->num_cus * score5_groups_per_cu(c.K, c.W),
+                                        c.end - c.begin);
+    if (timed) CK(hipEventRecord(ctx->launch_events[ev++], ctx->stream));
+    launch_score5(c.K, c.W, a, (int)std::max<int64_t>(g, 1), ctx->stream);
+    CK(hipGetLastError());
+    if (timed) CK(hipEventRecord(ctx->launch_events[ev++], ctx->stream));
+  }
   if (!P.c2.empty()) {
     int64_t bufcap = std::max(s.maxlen, 1);
     int g = grid2(ctx, (int64_t)P.items2.size());
@@ -317,7 +408,7 @@ int enqueue_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P, int32
 
 int batched_launch_count(mst_ctx* ctx, int id) {
   const BatchedPlan& P = ctx->bplans[id];
-  return (int)(P.c3.size() + P.c2.size() + ctx->fbplans[id].buckets.size());
+  return (int)(P.c3.size() + P.c5.size() + P.c2.size() + ctx->fbplans[id].buckets.size());
 }
 
 // All pairs of a set in the batched form into d_out (n x n). timed: plain
@@ -354,6 +445,8 @@ int run_batched(mst_ctx* ctx, int id, int32_t* d_out, bool timed) {
     };
     for (const BClass& c : P.c3)
       if ((rc = rec(-c.K, c.pairs, c.cells))) return rc;
+    for (const BClass& c : P.c5)
+      if ((rc = rec(-(1000 + 100 * c.W + c.K), c.pairs, c.cells))) return rc;
     for (const BClass& c : P.c2)
       if ((rc = rec(-100 - c.K, c.pairs, c.cells))) return rc;
     for (const Bucket& b : ctx->fbplans[id].buckets)
@@ -376,13 +469,13 @@ int run_batched(mst_ctx* ctx, int id, int32_t* d_out, bool timed) {
     P.graph_out = d_out;
   }
   CK(hipGraphLaunch(P.graph, ctx->stream));
-  const int nev = 2 * (int)(P.c3.size() + P.c2.size());
+  const int nev = 2 * (int)(P.c3.size() + P.c5.size() + P.c2.size());
   return enqueue_fallback(ctx, s, id, d_out, false, nev);
 }
 
 void free_batched(BatchedPlan& P) {
   if (P.graph) hipGraphExecDestroy(P.graph);
-  void* ps[] = {P.d_items3, P.d_items2, P.d_xlist, P.d_counters};
+  void* ps[] = {P.d_items3, P.d_items2, P.d_xlist, P.d_counters, P.d_items5, P.d_scodes, P.d_soff};
   for (void* p : ps)
     if (p) hipFree(p);
   P = BatchedPlan();

@@ -59,6 +59,21 @@ def test_batched_packed_class_edges(ctx):
     st.free()
 
 
+@pytest.mark.parametrize("seed,n,lo,hi", [(4, 12, 321, 1024), (5, 40, 300, 1030), (6, 70, 700, 1024)])
+def test_batched_pipelined_rows(ctx, seed, n, lo, hi):
+    # rows 321..1024: two to four waves per row pair, column streams of many
+    # sequences split between pipelines
+    rng = np.random.default_rng(seed)
+    lens = list(rng.integers(lo, hi + 1, n)) + [321, 512, 513, 768, 769, 1024]
+    seqs = [_rand(rng, int(L)) for L in lens]
+    st = ctx.upload(seqs)
+    g = st.all_pairs(M.FORM_BATCHED)
+    o, _ = O.all_pairs(seqs, S)
+    bad = np.argwhere(g != o)
+    assert len(bad) == 0, [(lens[i], lens[j], g[i, j], o[i, j]) for i, j in bad[:10]]
+    st.free()
+
+
 def test_config1_sampled_pairs(ctx):
     seqs = M.synth(1, 0)
     st = ctx.upload(seqs)
