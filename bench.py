@@ -82,6 +82,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batched-only", action="store_true",
                     help="time only the batched form (for a profile of its launches)")
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="a tunable of the batched form (mst_tune_set), repeatable")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the all-pairs path runs on a single MI355X "
@@ -97,6 +99,11 @@ def main():
     self_cells = int((L * L).sum())
 
     ctx = M.Context(device=0)
+    tunes = {}
+    for kv in args.tune:
+        k, v = kv.split("=", 1)
+        ctx.tune_set(k, int(v))
+        tunes[k] = int(v)
     st = ctx.upload(seqs)
     print("bench: %d sequences, %.4g cells (i<j), planning" % (len(seqs), cells), file=sys.stderr,
           flush=True)
@@ -126,6 +133,8 @@ def main():
     gcups_p = cells / sec_p / 1e9 if sec_p else None
 
     def kernel_of(cls):
+        if cls <= -2000:
+            return "score6_kernel", OPS_PER_CELL, PEAK_INT16_TOPS, "int16x2"
         if cls <= -1000:
             w, k = divmod(-cls - 1000, 100)
             return "score5_kernel<%d, %d>" % (k, w), OPS_PER_CELL, PEAK_INT16_TOPS, "int16x2"
@@ -177,6 +186,7 @@ def main():
         "device_ms_plain": round(float(np.mean(devs_p)), 3) if devs_p else None,
         "roofline": roof,
         "batched_launches": launches,
+        "tunables": tunes,
         "cpu_baseline": None if args.no_cpu else cpu_baseline(seqs),
     }
     print(json.dumps(line), flush=True)

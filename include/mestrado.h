@@ -76,11 +76,21 @@ int mst_all_pairs_launches(mst_ctx* ctx, int set_id, int form);
 /* Per-launch record of the last mst_all_pairs_launches call: rows per lane of
  * the kernel instance (negative: -K for the shared-profile 16-bit kernel,
  * -100-K for the multi-strip 16-bit kernel, -(1000+100*W+K) for the pipelined
- * 16-bit kernel with W waves of K rows per lane), pairs, DP cells and device
+ * 16-bit kernel with W waves of K rows per lane, -2000-K for the multi-sweep
+ * 16-bit kernel), pairs, DP cells and device
  * milliseconds of each launch (arrays of cap entries; *count receives the
  * number of launches). */
 int mst_last_launch_stats(mst_ctx* ctx, int* rows_per_lane, int64_t* pairs, int64_t* cells,
                           float* ms, int cap, int* count);
+/* Tunables of the batched form (no counterpart in the reference; the
+ * reference's window size wSize becomes such a hint): "chunk" (column
+ * sequences per work item, 1..4096; default 32), "shared3" (rows 129..320
+ * by the shared-profile kernel 1, or by the streamed kernel 0, the default),
+ * "wg5", "wg2" and
+ * "wg3.<K>" (workgroups per CU, 0 = default). Setting a value drops the cached
+ * batched plans. Unknown keys or values out of range return MST_E_ARG. */
+int mst_tune_set(mst_ctx* ctx, const char* key, int value);
+int mst_tune_get(mst_ctx* ctx, const char* key, int* value);
 /* Measurement helper (no counterpart in the reference): integer VALU rate of
  * the context's device in T ops/s, int32 (packed = 0) or packed int16
  * (packed = 1, two ops per lane and instruction), from a kernel of

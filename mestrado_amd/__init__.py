@@ -45,6 +45,8 @@ def lib():
         L.mst_all_pairs_launches.argtypes = [P, C.c_int, C.c_int]
         L.mst_last_launch_stats.argtypes = [P, P, P, P, P, C.c_int, C.POINTER(C.c_int)]
         L.mst_valu_peak.argtypes = [P, C.c_int, C.POINTER(C.c_double)]
+        L.mst_tune_set.argtypes = [P, C.c_char_p, C.c_int]
+        L.mst_tune_get.argtypes = [P, C.c_char_p, C.POINTER(C.c_int)]
         L.mst_align_pairs.argtypes = [P, C.c_int, P, P, C.c_int64, P, P, P, C.c_int64]
         L.mst_msa.argtypes = [P, C.c_int, P, C.POINTER(C.c_int)]
         L.mst_msa_rows.argtypes = [P, P, C.c_int64]
@@ -144,6 +146,15 @@ class Context:
         hbm = C.c_int64(0)
         self._ck(self._L.mst_device_info(self.h, C.byref(cus), C.byref(hbm)), "mst_device_info")
         return cus.value, hbm.value
+
+    def tune_set(self, key, value):
+        """Set a tunable of the batched form (drops cached batched plans)."""
+        self._ck(self._L.mst_tune_set(self.h, key.encode(), int(value)), "mst_tune_set")
+
+    def tune_get(self, key):
+        v = C.c_int(0)
+        self._ck(self._L.mst_tune_get(self.h, key.encode(), C.byref(v)), "mst_tune_get")
+        return v.value
 
     def valu_peak(self, packed=False):
         """Measured integer VALU rate of the device in T ops/s (int32, or packed int16)."""

@@ -15,11 +15,14 @@ struct BClass {
   int64_t pairs = 0;
   int W = 1;          // waves per pipeline (pipelined packed kernel)
 };
-// Tunables of the batched form (the autotuner writes them, see mst_autotune).
+// Tunables of the batched form (set through mst_tune_set; the autotuner in
+// mestrado_amd/autotune.py searches them and persists the best per device).
 struct Tune {
-  int chunk = 32;            // column sequences per shared-profile item
+  int chunk = 32;            // column sequences per work item
+  int shared3 = 0;           // rows 129..320: 1 shared-profile kernel, 0 streamed kernel
   int wg_per_cu3[17] = {0};  // score3 workgroups per CU by rows-per-lane K (0: default)
   int wg_per_cu2 = 0;        // score2 workgroups per CU (0: default)
+  int wg5 = 0;               // score5 workgroups per CU (0: as many as LDS allows)
 };
 
 struct BatchedPlan {
@@ -31,6 +34,9 @@ struct BatchedPlan {
   std::vector<BClass> c2;
   std::vector<mst::Item3> items5;  // pipelined packed kernel (score5)
   std::vector<BClass> c5;
+  std::vector<mst::Item3> items6;  // multi-sweep packed kernel (score6)
+  std::vector<BClass> c6;
+  int64_t carry_cols = 0;          // longest score6 column stream
   std::vector<int32_t> fb_i, fb_j;  // int32 fallback pairs (column, row)
   // device copies
   bool uploaded = false;
@@ -38,6 +44,9 @@ struct BatchedPlan {
   mst::Pair2* d_items2 = nullptr;
   int32_t* d_xlist = nullptr;
   mst::Item3* d_items5 = nullptr;
+  mst::Item3* d_items6 = nullptr;
+  int2* d_carry = nullptr;      // score6 row buffers, one per workgroup
+  int grid6 = 0;
   int8_t* d_scodes = nullptr;   // codes in length order
   int64_t* d_soff = nullptr;    // their offsets by sorted position
   unsigned int* d_counters = nullptr;

@@ -69,16 +69,24 @@ bool pk16_usable(const mst_ctx* ctx) {
   return ctx->symmetric && smin + ctx->go + ctx->ge >= 0 && smax + ctx->go + ctx->ge <= 255;
 }
 
-// Rows-per-lane classes of the shared-profile kernel that the planner uses.
-// The other classes and the multi-strip packed kernel are switched off: a run
-// that used them (lengths 1..3100, classes K = 1, 2, 8, 9, 12, 16 and the
-// multi-strip kernel) did not finish on the device, with no cause found in the
-// code; their pairs take the int32 kernel.
+// Classes of the planner, by rows m of the row pair (the longer of Ya, Yb):
+//  * m <= 1024: the pipelined packed kernel (score5), W = ceil(m/256) waves of
+//    64*K rows, K >= 3 (so rows 1..320 are one wave of K = 3..5);
+//  * with tune.shared3 = 1, rows 129..320 take the shared-profile kernel
+//    (score3) instead, in its classes K = 3..5;
+//  * m > 1024: the multi-sweep packed kernel (score6), up to the 16-bit bound;
+//  * every pair beyond the bound: the int32 kernel.
+// score3's other classes and the multi-strip packed kernel (score2) are
+// switched off: a run that used them (lengths 1..3100, classes K = 1, 2, 8, 9,
+// 12, 16 and the multi-strip kernel) did not finish on the device, with no
+// cause found in the code.
 constexpr int kPackedMinK = 3, kPackedMaxK = 5;
 constexpr bool kMultiStripPacked = false;
-// Single-strip rows beyond the classes above (321..1024 rows) go to the
-// pipelined packed kernel (score5): W = ceil(rows/256) waves of 64*K rows.
 constexpr bool kPipelined = true;
+// Rows beyond 1024 (up to the 16-bit bound) go to the multi-sweep kernel
+// (score6); a work item's column stream stays within kScore6MaxCols.
+constexpr bool kSweeps = true;
+constexpr int64_t kScore6MaxCols = 16384;
 
 static int rows_per_lane(int m, int* nstrips) {
   int s = (m + 1023) / 1024;
@@ -108,7 +116,7 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
     int K;
     int64_t cost, cells, pairs;
   };
-  std::vector<It3> v3, v5;
+  std::vector<It3> v3, v5, v6;
   std::vector<It2> v2;
   for (int k = 0; 2 * k < N; ++k) {
     const int ya = order[2 * k];
@@ -129,22 +137,36 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
     int nstr;
     int K = rows_per_lane(m, &nstr);
     int W = 1;   // waves per pipeline (score5)
-    int kind;    // 3: score3, 5: score5, 2: score2, 0: int32 kernel
-    if (nstr == 1 && K >= kPackedMinK && K <= kPackedMaxK) {
-      kind = 3;
-    } else if (nstr == 1 && m > 64 * kPackedMaxK && kPipelined) {
+    int kind;    // 3: score3, 5: score5, 6: score6, 2: score2, 0: int32 kernel
+    if (nstr == 1 && m <= 64 * kPackedMaxK) {
+      if (ctx->tune.shared3 && K >= kPackedMinK) {
+        kind = 3;
+      } else if (kPipelined) {
+        K = std::max(K, kPackedMinK);  // one wave, columns streamed
+        kind = 5;
+      } else {
+        kind = 0;
+      }
+    } else if (nstr == 1 && kPipelined) {
       W = (m + 255) / 256;
       K = (m + 64 * W - 1) / (64 * W);
       kind = 5;
+    } else if (nstr > 1 && kSweeps) {
+      K = kScore6K;
+      kind = 6;
     } else if (nstr > 1 && kMultiStripPacked) {
       kind = 2;
     } else {
       kind = 0;
     }
-    const int64_t R = kind == 5 ? (int64_t)W * 64 * K : (int64_t)nstr * 64 * K;
+    int64_t R = (int64_t)nstr * 64 * K;  // padded rows (bound of the 16-bit values)
+    if (kind == 5) R = (int64_t)W * 64 * K;
+    if (kind == 6) R = (int64_t)(m + 64 * K - 1) / (64 * K) * 64 * K;
     const bool packed = kind != 0;
-    // rows (ya, yb) against columns s[first..last]; yb < 0: ya alone
+    // rows (ya, yb) against columns s[first..last]; yb < 0: ya alone. The
+    // streamed kernels name rows by position (pa, pb), the others by id.
     auto add_cols = [&](int first, int last, int ya_, int yb_) {
+      const int pa = 2 * k, pb = yb_ >= 0 ? 2 * k + 1 : -1;
       std::vector<int> elig;
       for (int t = first; t <= last; ++t) {
         const int x = order[t];
@@ -160,14 +182,21 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
           elig.push_back(t);
         }
       }
-      if (kind == 3 || kind == 5) {
+      if (kind == 3 || kind == 5 || kind == 6) {
         size_t c0 = 0;
         while (c0 < elig.size()) {
-          size_t c1 = c0 + 1;  // a chunk is a run of consecutive positions
-          while (c1 < elig.size() && (int)(c1 - c0) < C && elig[c1] == elig[c1 - 1] + 1) ++c1;
+          // a chunk is a run of consecutive positions; score6 keeps its column
+          // stream within the per-workgroup row buffer
+          size_t c1 = c0 + 1;
+          int64_t cols = len(order[elig[c0]]);
+          while (c1 < elig.size() && (int)(c1 - c0) < C && elig[c1] == elig[c1 - 1] + 1 &&
+                 (kind != 6 || cols + len(order[elig[c1]]) <= kScore6MaxCols)) {
+            cols += len(order[elig[c1]]);
+            ++c1;
+          }
           Item3 it;
-          it.ya = ya_;
-          it.yb = yb_;
+          it.ya = kind == 3 ? ya_ : pa;
+          it.yb = kind == 3 ? yb_ : pb;
           it.xbegin = elig[c0];
           it.xcount = (int)(c1 - c0);
           int64_t cost = 0, cells = 0;
@@ -179,8 +208,10 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
           const int64_t npairs = (int64_t)(c1 - c0) * (yb_ >= 0 ? 2 : 1);
           if (kind == 3)
             v3.push_back(It3{it, K, 1, cost, cells, npairs});
-          else
+          else if (kind == 5)
             v5.push_back(It3{it, K, W, cost * W, cells, npairs});
+          else
+            v6.push_back(It3{it, K, (int)(R / 256), cost * (R / (64 * K)), cells, npairs});
           c0 = c1;
         }
       } else {
@@ -217,6 +248,18 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
     if (a.K != b.K) return a.K < b.K;
     return a.cost > b.cost;
   });
+  std::stable_sort(v6.begin(), v6.end(), [](const It3& a, const It3& b) { return a.cost > b.cost; });
+  for (size_t q = 0; q < v6.size(); ++q) {
+    P.items6.push_back(v6[q].it);
+    if (P.c6.empty()) P.c6.push_back(BClass{kScore6K, 0, 0});
+    P.c6.back().end = q + 1;
+    P.c6.back().work += v6[q].cost;
+    P.c6.back().cells += v6[q].cells;
+    P.c6.back().pairs += v6[q].pairs;
+    int64_t cols = 0;
+    for (int t = v6[q].it.xbegin; t < v6[q].it.xbegin + v6[q].it.xcount; ++t) cols += len(order[t]);
+    P.carry_cols = std::max(P.carry_cols, cols);
+  }
   for (size_t q = 0; q < v5.size(); ++q) {
     P.items5.push_back(v5[q].it);
     if (P.c5.empty() || P.c5.back().K != v5[q].K || P.c5.back().W != v5[q].W) {
@@ -278,14 +321,24 @@ int upload_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P) {
   CK(hipMalloc(&P.d_items3, sizeof(Item3) * std::max<size_t>(P.items3.size(), 1)));
   CK(hipMalloc(&P.d_items2, sizeof(Pair2) * std::max<size_t>(P.items2.size(), 1)));
   CK(hipMalloc(&P.d_xlist, sizeof(int32_t) * std::max<size_t>(P.xlist.size(), 1)));
-  P.ncounters = (int)(P.c3.size() + P.c2.size() + P.c5.size()) + 1;
+  P.ncounters = (int)(P.c3.size() + P.c2.size() + P.c5.size() + P.c6.size()) + 1;
+  const bool streamed = !P.c5.empty() || !P.c6.empty();
+  if (!P.c6.empty()) {
+    CK(hipMalloc(&P.d_items6, sizeof(Item3) * P.items6.size()));
+    CK(hipMemcpy(P.d_items6, P.items6.data(), sizeof(Item3) * P.items6.size(), hipMemcpyHostToDevice));
+    P.grid6 = (int)std::max<int64_t>(
+        1, std::min<int64_t>((int64_t)ctx->num_cus * score6_groups_per_cu(), (int64_t)P.items6.size()));
+    CK(hipMalloc(&P.d_carry, sizeof(int2) * (size_t)P.grid6 * std::max<int64_t>(P.carry_cols, 1)));
+  }
   if (!P.c5.empty()) {
+    CK(hipMalloc(&P.d_items5, sizeof(Item3) * P.items5.size()));
+    CK(hipMemcpy(P.d_items5, P.items5.data(), sizeof(Item3) * P.items5.size(), hipMemcpyHostToDevice));
+  }
+  if (streamed) {
     const int N = s.n;
     std::vector<int64_t> soff(N + 1, 0);
     for (int k = 0; k < N; ++k)
       soff[k + 1] = soff[k] + (s.h_off[P.xlist[k] + 1] - s.h_off[P.xlist[k]]);
-    CK(hipMalloc(&P.d_items5, sizeof(Item3) * P.items5.size()));
-    CK(hipMemcpy(P.d_items5, P.items5.data(), sizeof(Item3) * P.items5.size(), hipMemcpyHostToDevice));
     CK(hipMalloc(&P.d_soff, sizeof(int64_t) * (N + 1)));
     CK(hipMemcpy(P.d_soff, soff.data(), sizeof(int64_t) * (N + 1), hipMemcpyHostToDevice));
     CK(hipMalloc(&P.d_scodes, std::max<int64_t>(soff[N], 1)));
@@ -297,7 +350,7 @@ int upload_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P) {
     CK(hipMemcpy(P.d_items2, P.items2.data(), sizeof(Pair2) * P.items2.size(), hipMemcpyHostToDevice));
   if (!P.xlist.empty())
     CK(hipMemcpy(P.d_xlist, P.xlist.data(), sizeof(int32_t) * P.xlist.size(), hipMemcpyHostToDevice));
-  if (!P.c5.empty()) {
+  if (streamed) {
     hipLaunchKernelGGL(gather_codes, dim3(s.n), dim3(256), 0, ctx->stream, s.d_codes, s.d_off,
                        P.d_xlist, P.d_soff, P.d_scodes);
     CK(hipGetLastError());
@@ -352,14 +405,15 @@ int enqueue_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P, int32
     CK(hipGetLastError());
     if (timed) CK(hipEventRecord(ctx->launch_events[ev++], ctx->stream));
   }
-  for (const BClass& c : P.c5) {
+  auto args5 = [&](const BClass& c, const Item3* items) {
     Score5Args a;
     a.scodes = P.d_scodes;
     a.soff = P.d_soff;
     a.order = P.d_xlist;
-    a.seqs = s.d_codes;
-    a.off = s.d_off;
-    a.items = P.d_items5;
+    a.items = items;
+    a.item_job = nullptr;
+    a.job_out = nullptr;
+    a.job_ld = nullptr;
     a.first = c.begin;
     a.last = c.end;
     a.counter = P.d_counters + ctr++;
@@ -368,8 +422,21 @@ int enqueue_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P, int32
     a.subst = ctx->d_S;
     a.go = ctx->go;
     a.ge = ctx->ge;
-    const int64_t g = std::min<int64_t>((int64_t)ctx->num_cus * score5_groups_per_cu(c.K, c.W),
-                                        c.end - c.begin);
+    a.carry = P.d_carry;
+    a.carry_cap = std::max<int64_t>(P.carry_cols, 1);
+    return a;
+  };
+  for (const BClass& c : P.c6) {
+    const Score5Args a = args5(c, P.d_items6);
+    if (timed) CK(hipEventRecord(ctx->launch_events[ev++], ctx->stream));
+    launch_score6(a, P.grid6, ctx->stream);
+    CK(hipGetLastError());
+    if (timed) CK(hipEventRecord(ctx->launch_events[ev++], ctx->stream));
+  }
+  for (const BClass& c : P.c5) {
+    const Score5Args a = args5(c, P.d_items5);
+    const int per = ctx->tune.wg5 > 0 ? ctx->tune.wg5 : score5_groups_per_cu(c.K, c.W);
+    const int64_t g = std::min<int64_t>((int64_t)ctx->num_cus * per, c.end - c.begin);
     if (timed) CK(hipEventRecord(ctx->launch_events[ev++], ctx->stream));
     launch_score5(c.K, c.W, a, (int)std::max<int64_t>(g, 1), ctx->stream);
     CK(hipGetLastError());
@@ -408,7 +475,8 @@ int enqueue_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P, int32
 
 int batched_launch_count(mst_ctx* ctx, int id) {
   const BatchedPlan& P = ctx->bplans[id];
-  return (int)(P.c3.size() + P.c5.size() + P.c2.size() + ctx->fbplans[id].buckets.size());
+  return (int)(P.c3.size() + P.c6.size() + P.c5.size() + P.c2.size() +
+               ctx->fbplans[id].buckets.size());
 }
 
 // All pairs of a set in the batched form into d_out (n x n). timed: plain
@@ -445,6 +513,8 @@ int run_batched(mst_ctx* ctx, int id, int32_t* d_out, bool timed) {
     };
     for (const BClass& c : P.c3)
       if ((rc = rec(-c.K, c.pairs, c.cells))) return rc;
+    for (const BClass& c : P.c6)
+      if ((rc = rec(-2000 - c.K, c.pairs, c.cells))) return rc;
     for (const BClass& c : P.c5)
       if ((rc = rec(-(1000 + 100 * c.W + c.K), c.pairs, c.cells))) return rc;
     for (const BClass& c : P.c2)
@@ -469,14 +539,61 @@ int run_batched(mst_ctx* ctx, int id, int32_t* d_out, bool timed) {
     P.graph_out = d_out;
   }
   CK(hipGraphLaunch(P.graph, ctx->stream));
-  const int nev = 2 * (int)(P.c3.size() + P.c5.size() + P.c2.size());
+  const int nev = 2 * (int)(P.c3.size() + P.c6.size() + P.c5.size() + P.c2.size());
   return enqueue_fallback(ctx, s, id, d_out, false, nev);
 }
 
 void free_batched(BatchedPlan& P) {
   if (P.graph) hipGraphExecDestroy(P.graph);
-  void* ps[] = {P.d_items3, P.d_items2, P.d_xlist, P.d_counters, P.d_items5, P.d_scodes, P.d_soff};
+  void* ps[] = {P.d_items3, P.d_items2, P.d_xlist, P.d_counters, P.d_items5,
+                P.d_scodes, P.d_soff, P.d_items6, P.d_carry};
   for (void* p : ps)
     if (p) hipFree(p);
   P = BatchedPlan();
+}
+
+// Tunables of the batched form by name. A change drops every cached batched
+// plan (and its graph), so the next call plans with the new values.
+static int* tune_slot(mst_ctx* ctx, const char* key) {
+  if (!strcmp(key, "chunk")) return &ctx->tune.chunk;
+  if (!strcmp(key, "shared3")) return &ctx->tune.shared3;
+  if (!strcmp(key, "wg5")) return &ctx->tune.wg5;
+  if (!strcmp(key, "wg2")) return &ctx->tune.wg_per_cu2;
+  if (!strncmp(key, "wg3.", 4)) {
+    const int k = atoi(key + 4);
+    if (k >= 1 && k <= 16) return &ctx->tune.wg_per_cu3[k];
+  }
+  return nullptr;
+}
+
+extern "C" int mst_tune_set(mst_ctx* ctx, const char* key, int value) {
+  if (!ctx || !key) return MST_E_ARG;
+  int* slot = tune_slot(ctx, key);
+  if (!slot) return ctx->fail(MST_E_ARG, "mst_tune_set", "unknown key");
+  if (!strcmp(key, "chunk") && (value < 1 || value > 4096))
+    return ctx->fail(MST_E_ARG, "mst_tune_set", "chunk out of range 1..4096");
+  if (!strcmp(key, "shared3") && value != 0 && value != 1)
+    return ctx->fail(MST_E_ARG, "mst_tune_set", "shared3 is 0 or 1");
+  if (strcmp(key, "chunk") && strcmp(key, "shared3") && (value < 0 || value > 64))
+    return ctx->fail(MST_E_ARG, "mst_tune_set", "workgroups per CU out of range 0..64");
+  if (*slot == value) return MST_OK;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  *slot = value;
+  for (auto& kv : ctx->bplans) free_batched(kv.second);
+  ctx->bplans.clear();
+  for (auto& kv : ctx->fbplans)
+    if (kv.second.d_pairs) hipFree(kv.second.d_pairs);
+  ctx->fbplans.clear();
+  for (auto it = ctx->prepared.begin(); it != ctx->prepared.end();)
+    it = (it->second == MST_FORM_BATCHED) ? ctx->prepared.erase(it) : std::next(it);
+  return MST_OK;
+}
+
+extern "C" int mst_tune_get(mst_ctx* ctx, const char* key, int* value) {
+  if (!ctx || !key || !value) return MST_E_ARG;
+  int* slot = tune_slot(ctx, key);
+  if (!slot) return ctx->fail(MST_E_ARG, "mst_tune_get", "unknown key");
+  *value = *slot;
+  return MST_OK;
 }

@@ -6,31 +6,42 @@
 
 namespace mst {
 
-// Pipelined packed form: the rows of a (Ya, Yb) pair are cut into W strips of
-// 64*K rows, one wave per strip; the W waves of a pipeline sweep the same
-// column stream, each strip's last row passing to the next wave through an LDS
-// ring. A work item (Item3) is a run of consecutive sorted positions; its
-// column sequences are concatenated into one stream per pipeline.
+// Streamed packed forms (nw_score5.hip, nw_score6.hip). Rows and columns are
+// positions in a length-ordered copy of the codes; a work item (Item3) is a
+// row pair (ya, yb: positions, yb < 0 for ya alone) and a run of consecutive
+// positions xbegin .. xbegin+xcount-1 whose codes are contiguous, swept as one
+// column stream.
 struct Score5Args {
-  const int8_t* scodes;   // codes of every sequence in length order, concatenated
-  const int64_t* soff;    // offsets into scodes by sorted position (nseq + 1)
-  const int32_t* order;   // sorted position -> sequence id
-  const int8_t* seqs;     // codes by sequence id (row sequences)
-  const int64_t* off;
-  const Item3* items;     // xbegin/xcount are sorted positions
-  int64_t first, last;
-  unsigned int* counter;  // work queue head (zero before the launch)
-  int32_t* out_mat;       // nseq x nseq, written at (x, y) and (y, x)
+  const int8_t* scodes;    // codes in length order, concatenated
+  const int64_t* soff;     // offsets into scodes by position (positions + 1)
+  const int32_t* order;    // position -> index of the sequence in its set
+  const Item3* items;
+  const int* item_job;     // set of each item (null: one set, ld = nseq)
+  const int64_t* job_out;  // offset of each set's matrix in out_mat
+  const int* job_ld;       // and its leading dimension
+  int64_t first, last;     // item range of this launch
+  unsigned int* counter;   // work queue head (zero before the launch)
+  int32_t* out_mat;        // scores, written at (x, y) and (y, x)
   int nseq;
   const int8_t* subst;
   int go, ge;
+  int2* carry;             // score6: per-workgroup row buffer between sweeps
+  int64_t carry_cap;       // its columns per workgroup
 };
 
-// Rows per lane K and waves per pipeline W supported: K in 3..5 for W = 1,
-// K in 3..4 for W = 2..4.
+// score5: rows up to 1024 in W strips of 64*K rows, one wave per strip.
+// K in 3..5 for W = 1, K in 3..4 for W = 2..4.
 bool score5_supported(int K, int W);
 int score5_block_threads(int W);
 int score5_groups_per_cu(int K, int W);
 void launch_score5(int K, int W, const Score5Args& a, int grid, hipStream_t st);
+
+// score6: any number of 256-row strips (K = 4), four waves sweeping the
+// stream once per four strips; the last strip of a sweep hands its row to the
+// first of the next through `carry`.
+constexpr int kScore6K = 4;
+int score6_block_threads();
+int score6_groups_per_cu();
+void launch_score6(const Score5Args& a, int grid, hipStream_t st);
 
 }  // namespace mst

@@ -1,0 +1,139 @@
+// Batched all-pairs scores (R1) in packed 16-bit arithmetic for row pairs of
+// any length the 16-bit bound admits (in practice up to ~2,100 residues with
+// BLOSUM62 and go = 10, ge = 2): the streamed form of nw_score5.hip with more
+// strips than waves.
+//
+// A workgroup of four waves takes one item. The rows are cut into NS strips of
+// 256 rows (K = 4 per lane). Wave j sweeps the item's column stream once for
+// each of its strips j, j+4, j+8, ...: in sweep q its strip is s = j + 4q.
+// Strip s reads the row above it from strip s-1: from wave j-1 through an LDS
+// ring in the same sweep, or, for wave 0, from wave 3's previous sweep through
+// a full-length row buffer in HBM (`carry`, one per workgroup). Progress
+// counters count columns in global units q*ncol + column, so the rings and
+// counters run on across sweeps without a reset.
+//
+// Every wait ends: a wave waits only for data from an earlier strip (wave j-1
+// in the same sweep, or wave 3 one sweep earlier) and for a later strip to have
+// read a ring slot; the later strip never waits on the writer's next columns,
+// and wave 3 never waits for the carry to be read, because wave 0 reads carry
+// column c of a sweep before the data of that column can reach wave 3 in the
+// same sweep. Each wave keeps its own LDS profile slot and rebuilds it at the
+// start of each sweep.
+#include "nw_stream.h"
+
+namespace mst {
+
+constexpr int W6 = 4;
+
+__global__ void __launch_bounds__(MST_WAVE * W6) score6_kernel(Score5Args a) {
+  constexpr int K = kScore6K;
+  constexpr int D = KD<K>::v;
+  constexpr int STRIP = MST_LETTERS * MST_WAVE * D * 4;
+  constexpr int ROWS = MST_WAVE * K;  // rows per strip
+  __shared__ int8_t S_lds[MST_LETTERS * MST_LETTERS];
+  __shared__ __attribute__((aligned(16))) uint8_t prof[W6 * STRIP];
+  __shared__ int2 ring[(W6 - 1) * RING5];
+  __shared__ int s_prog[W6 + 1], s_cons[W6 + 1];  // [W6]: the carry between sweeps
+  __shared__ int s_item;
+  const int lane = lane_id();
+  const int j = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave index (uniform)
+  for (int i = threadIdx.x; i < MST_LETTERS * MST_LETTERS; i += MST_WAVE * W6) S_lds[i] = a.subst[i];
+  const int G = a.go - a.ge;
+  const s2 G2 = {(short)G, (short)G};
+  const int bonus = a.go + a.ge;
+  uint8_t* slot = prof + j * STRIP;
+  const uint8_t* prof_lane = slot + lane * D * 4;
+  int2* carry = a.carry + (int64_t)blockIdx.x * a.carry_cap;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = (int)atomicAdd(a.counter, 1u);
+    if (threadIdx.x < W6 + 1) {
+      s_prog[threadIdx.x] = 0;
+      s_cons[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    const int64_t it = (int64_t)s_item + a.first;
+    if (it >= a.last) break;
+    const Item3 item = a.items[it];
+    Stream S = item_stream(item, it, a);
+    const int m = S.ma > S.mb ? S.ma : S.mb;
+    const int ns = (m + ROWS - 1) / ROWS;
+    S.base = a.soff[item.xbegin];
+    S.ncol = (int)(a.soff[item.xbegin + item.xcount] - S.base);
+    const int ncol = S.ncol;
+    for (int q = 0;; ++q) {
+      const int s = j + q * W6;
+      if (s >= ns) break;
+      // this wave's previous sweep is over and no other wave reads its slot
+      __builtin_amdgcn_wave_barrier();
+      build_strip_profile<K>(slot, S_lds, S, a, s * ROWS, 0, 1, bonus);
+      __builtin_amdgcn_wave_barrier();
+      const int g = q * ncol;
+      const bool first = (s == 0), last = (s == ns - 1);
+      Link in, out;
+      int gin;
+      const int* prog_in;
+      int* cons_out;
+      if (j > 0) {
+        in = Link{ring + (j - 1) * RING5, g, RING5 - 1};
+        gin = g;
+        prog_in = s_prog + j - 1;
+        cons_out = s_cons + j;
+      } else {
+        in = Link{carry, 0, -1};
+        gin = g - ncol;
+        prog_in = s_prog + W6;
+        cons_out = s_cons + W6;
+      }
+      int* prog_out;
+      const int* cons_in;
+      int span, pub;
+      if (j < W6 - 1) {
+        out = Link{ring + j * RING5, g, RING5 - 1};
+        prog_out = s_prog + j;
+        cons_in = s_cons + j + 1;
+        span = RING5;
+        pub = 1;
+      } else {
+        out = Link{carry, 0, -1};
+        prog_out = s_prog + W6;
+        cons_in = nullptr;
+        span = 0;
+        pub = 16;
+      }
+      const int r0 = s * ROWS;
+      if (first) {
+        if (last)
+          sweep5<K, true, false>(item.xbegin, r0, S, prof_lane, G2, in, gin, prog_in, cons_out, out,
+                                 g, prog_out, cons_in, span, pub, a);
+        else
+          sweep5<K, true, true>(item.xbegin, r0, S, prof_lane, G2, in, gin, prog_in, cons_out, out,
+                                g, prog_out, cons_in, span, pub, a);
+      } else {
+        if (last)
+          sweep5<K, false, false>(item.xbegin, r0, S, prof_lane, G2, in, gin, prog_in, cons_out,
+                                  out, g, prog_out, cons_in, span, pub, a);
+        else
+          sweep5<K, false, true>(item.xbegin, r0, S, prof_lane, G2, in, gin, prog_in, cons_out,
+                                 out, g, prog_out, cons_in, span, pub, a);
+      }
+    }
+    __syncthreads();  // rings, counters and carry are reused by the next item
+  }
+}
+
+int score6_block_threads() { return MST_WAVE * W6; }
+
+int score6_groups_per_cu() {
+  const int D = KD<kScore6K>::v;
+  const int lds = W6 * MST_LETTERS * MST_WAVE * D * 4 + (W6 - 1) * RING5 * 8 + 8 * (W6 + 1) + 4 +
+                  MST_LETTERS * MST_LETTERS;
+  int g = (160 * 1024) / (lds + 512);
+  if (g * W6 > 32) g = 32 / W6;
+  return g < 1 ? 1 : g;
+}
+
+void launch_score6(const Score5Args& a, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(score6_kernel, dim3(grid), dim3(score6_block_threads()), 0, st, a);
+}
+
+}  // namespace mst

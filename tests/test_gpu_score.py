@@ -59,6 +59,21 @@ def test_batched_packed_class_edges(ctx):
     st.free()
 
 
+def test_batched_short_rows_streams(ctx):
+    # rows of 1..128 residues: one wave, columns streamed through sequences of
+    # every length down to 1, so lanes cross several sequence ends per block
+    rng = np.random.default_rng(8)
+    lens = [1, 1, 2, 3, 4, 5, 7, 8, 9, 13, 21, 34, 55, 63, 64, 65, 89, 100, 127, 128, 129, 300, 700]
+    lens += list(rng.integers(1, 40, 30))
+    seqs = [_rand(rng, int(L)) for L in lens]
+    st = ctx.upload(seqs)
+    g = st.all_pairs(M.FORM_BATCHED)
+    o, _ = O.all_pairs(seqs, S)
+    bad = np.argwhere(g != o)
+    assert len(bad) == 0, [(lens[i], lens[j], g[i, j], o[i, j]) for i, j in bad[:10]]
+    st.free()
+
+
 @pytest.mark.parametrize("seed,n,lo,hi", [(4, 12, 321, 1024), (5, 40, 300, 1030), (6, 70, 700, 1024)])
 def test_batched_pipelined_rows(ctx, seed, n, lo, hi):
     # rows 321..1024: two to four waves per row pair, column streams of many
@@ -71,6 +86,49 @@ def test_batched_pipelined_rows(ctx, seed, n, lo, hi):
     o, _ = O.all_pairs(seqs, S)
     bad = np.argwhere(g != o)
     assert len(bad) == 0, [(lens[i], lens[j], g[i, j], o[i, j]) for i, j in bad[:10]]
+    st.free()
+
+
+@pytest.mark.parametrize("shared3,chunk", [(1, 32), (1, 1), (0, 5)])
+def test_batched_tunables_keep_results(ctx, shared3, chunk):
+    # every routing / item size of the batched planner gives the same matrix
+    rng = np.random.default_rng(11)
+    lens = list(rng.integers(1, 900, 45)) + [128, 129, 320, 321, 1024]
+    seqs = [_rand(rng, int(L)) for L in lens]
+    o, _ = O.all_pairs(seqs, S)
+    st = ctx.upload(seqs)
+    saved = {k: ctx.tune_get(k) for k in ("shared3", "chunk")}
+    try:
+        ctx.tune_set("shared3", shared3)
+        ctx.tune_set("chunk", chunk)
+        assert ctx.tune_get("chunk") == chunk
+        g = st.all_pairs(M.FORM_BATCHED)
+        assert (g == o).all()
+    finally:
+        for k, v in saved.items():
+            ctx.tune_set(k, v)
+        st.free()
+    with pytest.raises(M.MestradoError):
+        ctx.tune_set("no-such-key", 1)
+    with pytest.raises(M.MestradoError):
+        ctx.tune_set("chunk", 0)
+
+
+@pytest.mark.parametrize("seed,n,lo,hi", [(21, 9, 1025, 2100), (22, 14, 800, 1600)])
+def test_batched_multi_sweep_rows(ctx, seed, n, lo, hi):
+    # rows beyond 1024: four waves sweep the column stream once per four
+    # 256-row strips, the row between sweeps kept in HBM; the longest pairs
+    # exceed the 16-bit bound and take the int32 kernel
+    rng = np.random.default_rng(seed)
+    lens = list(rng.integers(lo, hi + 1, n)) + [1025, 1280, 1281, 2048]
+    seqs = [_rand(rng, int(L)) for L in lens]
+    st = ctx.upload(seqs)
+    g = st.all_pairs(M.FORM_BATCHED)
+    o, _ = O.all_pairs(seqs, S)
+    bad = np.argwhere(g != o)
+    assert len(bad) == 0, [(lens[i], lens[j], g[i, j], o[i, j]) for i, j in bad[:10]]
+    kinds = {r[0] for r in st.launch_stats(M.FORM_BATCHED)}
+    assert -2004 in kinds
     st.free()
 
 
