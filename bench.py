@@ -18,7 +18,10 @@ roofline   the dominant launch's integer VALU rate: 7 algorithmic int ops per
 cpu_baseline  the oracle recurrence (oracle/liboracle.so, OpenMP over pairs)
            on a bounded random sample of the same pairs
 
-Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu]
+served     the served form (continuous batching) on the job workload: offline
+           and loaded throughput, p50/p99 latency loaded and unloaded
+
+Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu] [--no-serve]
 """
 import argparse
 import json
@@ -73,6 +76,87 @@ def cpu_baseline(seqs, target_seconds=12.0):
     }
 
 
+def served_bench(M, ctx, njobs=2000, load=0.75, seed=0, sets=()):
+    """Served form on the job workload: njobs jobs, sizes power-law (exponent
+    1.5) on 8..512 sequences, each a tree-evolved family of lengths 100..2000.
+    Offline: every job submitted at once, drained. Unloaded: one job at a time.
+    Loaded: Poisson arrivals at `load` x the offline job rate; latency runs
+    from a job's scheduled arrival to the end of its last round."""
+    def note(msg):
+        print("served: " + msg, file=sys.stderr, flush=True)
+
+    sizes = M.job_sizes(seed, njobs)
+    jobs = [M.synth(4, seed * 1000003 + k, int(n)) for k, n in enumerate(sizes)]
+    note("%d jobs generated" % njobs)
+    jcells = []
+    for j in jobs:
+        L = np.array([len(x) for x in j], dtype=np.int64)
+        jcells.append(int((L.sum() ** 2 - (L * L).sum()) // 2))
+    total_cells = float(sum(jcells))
+    srv = M.Server(ctx, 16 << 30)
+    for k, v in sets:
+        srv.set(k, v)
+    # offline
+    t0 = time.perf_counter()
+    ids = [srv.submit(j) for j in jobs]
+    srv.drain()
+    t_off = time.perf_counter() - t0
+    for jid in ids:
+        srv.release(jid)
+    srv.completed(1 << 16)
+    note("offline %.2f s, %.3g cells/s estimate" % (t_off, srv.stats()["cells_per_s"]))
+    # unloaded
+    lat_u = []
+    for j in jobs:
+        jid = srv.submit(j)
+        srv.drain()
+        info = srv.job(jid)
+        lat_u.append(info["t_done"] - info["t_submit"])
+        srv.release(jid)
+        if len(lat_u) % 500 == 0:
+            note("unloaded %d jobs" % len(lat_u))
+    srv.completed(1 << 16)
+    # loaded
+    rng = np.random.default_rng(seed + 1)
+    rate_off = njobs / t_off
+    arr = np.cumsum(rng.exponential(1.0 / (load * rate_off), njobs))
+    perm = rng.permutation(njobs)
+    k, done, lat, first_of = 0, 0, [], {}
+    t0m = time.monotonic()
+    t_last = t0m
+    while done < njobs:
+        now = time.monotonic() - t0m
+        while k < njobs and arr[k] <= now:
+            first_of[srv.submit(jobs[perm[k]])] = k
+            k += 1
+        srv.step()
+        for jid in srv.completed():
+            info = srv.job(jid)
+            lat.append(info["t_done"] - (t0m + arr[first_of[jid]]))
+            t_last = max(t_last, info["t_done"])
+            srv.release(jid)
+            done += 1
+            if done % 500 == 0:
+                note("loaded %d jobs, %.2f s" % (done, time.monotonic() - t0m))
+    t_load = t_last - t0m
+    srv.close()
+    ms = lambda v, q: round(float(np.percentile(v, q)) * 1e3, 3)
+    return {
+        "workload": "%d jobs, sizes power-law 1.5 on 8..512, lengths 100..2000, %.3g cells (i<j)"
+                    % (njobs, total_cells),
+        "offline_jobs_per_s": round(rate_off, 1),
+        "offline_gcups": round(total_cells / t_off / 1e9, 1),
+        "load": load,
+        "jobs_per_s": round(njobs / t_load, 1),
+        "gcups": round(total_cells / t_load / 1e9, 1),
+        "frac_of_offline": round((njobs / t_load) / rate_off, 3),
+        "p50_ms": ms(lat, 50), "p99_ms": ms(lat, 99),
+        "unloaded_p50_ms": ms(lat_u, 50), "unloaded_p99_ms": ms(lat_u, 99),
+        "p99_over_unloaded_p99": round(float(np.percentile(lat, 99) / np.percentile(lat_u, 99)), 3),
+        "params": dict(sets),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +168,10 @@ def main():
                     help="time only the batched form (for a profile of its launches)")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="a tunable of the batched form (mst_tune_set), repeatable")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-form measurement")
+    ap.add_argument("--serve-jobs", type=int, default=2000)
+    ap.add_argument("--serve-set", action="append", default=[], metavar="KEY=VALUE",
+                    help="a parameter of the served form (mst_server_set), repeatable")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the all-pairs path runs on a single MI355X "
@@ -187,6 +275,9 @@ def main():
         "roofline": roof,
         "batched_launches": launches,
         "tunables": tunes,
+        "served": None if (args.no_serve or args.batched_only) else served_bench(
+            M, ctx, args.serve_jobs,
+            sets=[(kv.split("=", 1)[0], float(kv.split("=", 1)[1])) for kv in args.serve_set]),
         "cpu_baseline": None if args.no_cpu else cpu_baseline(seqs),
     }
     print(json.dumps(line), flush=True)

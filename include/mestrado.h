@@ -91,6 +91,47 @@ int mst_last_launch_stats(mst_ctx* ctx, int* rows_per_lane, int64_t* pairs, int6
  * batched plans. Unknown keys or values out of range return MST_E_ARG. */
 int mst_tune_set(mst_ctx* ctx, const char* key, int value);
 int mst_tune_get(mst_ctx* ctx, const char* key, int* value);
+/* Served form (no counterpart in the reference, whose runs are one job each):
+ * a request loop with continuous batching over one context. Jobs are sets of
+ * sequences whose all-pairs score matrix is wanted; mst_server_step retires
+ * finished rounds, admits waiting jobs in arrival order while their device
+ * footprint (N*N*4 B of scores plus twice the residues) fits the budget, and
+ * launches the next round: a quantum of work items of the admitted jobs,
+ * earliest deadline first, packed into shared launches of the batched form's
+ * kernels. Results stay on the device until mst_server_result copies them
+ * out and frees the job (finished jobs hold their memory until then). A job
+ * larger than the budget is rejected at submission. Jobs are independent: a
+ * job's matrix is identical alone and in any mix. Not thread-safe. */
+typedef struct mst_server mst_server;
+enum { MST_JOB_WAITING = 0, MST_JOB_RUNNING = 1, MST_JOB_DONE = 2, MST_JOB_CANCELLED = 3,
+       MST_JOB_REJECTED = 4 };
+int mst_server_create(mst_ctx* ctx, int64_t budget_bytes, mst_server** srv);
+int mst_server_destroy(mst_server* srv);
+/* The job's sequences are ascii[offsets[i] .. offsets[i+1]) (copied). */
+int mst_server_submit(mst_server* srv, const char* ascii, const int64_t* offsets, int n,
+                      int64_t* job);
+/* One scheduling pass (never blocks on the device); *ncompleted (may be NULL)
+ * receives the number of jobs that finished in it. */
+int mst_server_step(mst_server* srv, int* ncompleted);
+/* Steps until every admitted and waiting job is done (blocks). */
+int mst_server_drain(mst_server* srv);
+/* State, size and host steady-clock times (seconds) of a job. */
+int mst_server_job(mst_server* srv, int64_t job, int* state, int* n, double* t_submit,
+                   double* t_admit, double* t_done);
+/* Copies a done job's n x n scores out (may be NULL) and forgets the job. */
+int mst_server_result(mst_server* srv, int64_t job, int32_t* scores_nxn);
+int mst_server_cancel(mst_server* srv, int64_t job);
+/* Up to cap ids of jobs that finished since the previous call, oldest first;
+ * *count receives how many were written. */
+int mst_server_completed(mst_server* srv, int64_t* ids, int cap, int* count);
+int mst_server_stats(mst_server* srv, int* waiting, int* running, int* inflight,
+                     int64_t* arena_used, double* cells_per_s);
+/* "round_cells" (DP cells per round, default 2e10, about 4 ms of the device),
+ * "chunk" (column sequences per item, default 8), "slack" (a job's deadline
+ * is its submission plus slack times its estimated run time, default 2;
+ * jobs go earliest deadline first), "rate" (initial throughput estimate in
+ * cells/s, for those estimates). */
+int mst_server_set(mst_server* srv, const char* key, double value);
 /* Measurement helper (no counterpart in the reference): integer VALU rate of
  * the context's device in T ops/s, int32 (packed = 0) or packed int16
  * (packed = 1, two ops per lane and instruction), from a kernel of

@@ -46,6 +46,21 @@ def lib():
         L.mst_last_launch_stats.argtypes = [P, P, P, P, P, C.c_int, C.POINTER(C.c_int)]
         L.mst_valu_peak.argtypes = [P, C.c_int, C.POINTER(C.c_double)]
         L.mst_tune_set.argtypes = [P, C.c_char_p, C.c_int]
+        L.mst_server_create.argtypes = [P, C.c_int64, C.POINTER(P)]
+        L.mst_server_destroy.argtypes = [P]
+        L.mst_server_submit.argtypes = [P, P, P, C.c_int, C.POINTER(C.c_int64)]
+        L.mst_server_step.argtypes = [P, C.POINTER(C.c_int)]
+        L.mst_server_drain.argtypes = [P]
+        L.mst_server_job.argtypes = [P, C.c_int64, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double)]
+        L.mst_server_result.argtypes = [P, C.c_int64, P]
+        L.mst_server_cancel.argtypes = [P, C.c_int64]
+        L.mst_server_completed.argtypes = [P, P, C.c_int, C.POINTER(C.c_int)]
+        L.mst_server_stats.argtypes = [P, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                       C.POINTER(C.c_int), C.POINTER(C.c_int64),
+                                       C.POINTER(C.c_double)]
+        L.mst_server_set.argtypes = [P, C.c_char_p, C.c_double]
         L.mst_tune_get.argtypes = [P, C.c_char_p, C.POINTER(C.c_int)]
         L.mst_align_pairs.argtypes = [P, C.c_int, P, P, C.c_int64, P, P, P, C.c_int64]
         L.mst_msa.argtypes = [P, C.c_int, P, C.POINTER(C.c_int)]
@@ -68,11 +83,19 @@ def synth(config, seed=0, n=0):
     L = lib()
     cnt = C.c_int(0)
     tot = C.c_int64(0)
-    if L.mst_synth(config, seed, n, None, 0, None, 0, C.byref(cnt), C.byref(tot)) != 0:
-        raise MestradoError("bad synthetic configuration %r" % config)
-    buf = np.zeros(max(1, tot.value), dtype=np.uint8)
-    off = np.zeros(cnt.value + 1, dtype=np.int64)
-    rc = L.mst_synth(config, seed, n, _p(buf), len(buf), _p(off), len(off), C.byref(cnt), C.byref(tot))
+    rc = -2
+    if config == 4 and n > 0:  # lengths are at most 2000: one pass with room to spare
+        buf = np.zeros(n * 2048, dtype=np.uint8)
+        off = np.zeros(n + 1, dtype=np.int64)
+        rc = L.mst_synth(config, seed, n, _p(buf), len(buf), _p(off), len(off), C.byref(cnt),
+                         C.byref(tot))
+    if rc == -2:
+        if L.mst_synth(config, seed, n, None, 0, None, 0, C.byref(cnt), C.byref(tot)) != 0:
+            raise MestradoError("bad synthetic configuration %r" % config)
+        buf = np.zeros(max(1, tot.value), dtype=np.uint8)
+        off = np.zeros(cnt.value + 1, dtype=np.int64)
+        rc = L.mst_synth(config, seed, n, _p(buf), len(buf), _p(off), len(off), C.byref(cnt),
+                         C.byref(tot))
     if rc != 0:
         raise MestradoError("mst_synth failed (%d)" % rc)
     raw = buf.tobytes()
@@ -262,3 +285,83 @@ class SeqSet:
         self.ctx._ck(L.mst_last_launch_stats(self.ctx.h, _p(k), _p(p), _p(c), _p(m), cnt.value,
                                              C.byref(cnt)), "mst_last_launch_stats")
         return list(zip(k.tolist(), p.tolist(), c.tolist(), m.tolist()))
+
+
+JOB_WAITING, JOB_RUNNING, JOB_DONE, JOB_CANCELLED, JOB_REJECTED = range(5)
+
+
+class Server:
+    """Served form (mst_server_* in include/mestrado.h): continuous batching of
+    all-pairs jobs on one context."""
+
+    def __init__(self, ctx, budget_bytes=8 << 30):
+        self.ctx = ctx
+        self._L = ctx._L
+        h = C.c_void_p()
+        ctx._ck(self._L.mst_server_create(ctx.h, int(budget_bytes), C.byref(h)), "mst_server_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self._L.mst_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, seqs):
+        buf, off = pack(seqs)
+        j = C.c_int64(0)
+        self.ctx._ck(self._L.mst_server_submit(self.h, _p(buf), _p(off), len(seqs), C.byref(j)),
+                     "mst_server_submit")
+        return j.value
+
+    def step(self):
+        n = C.c_int(0)
+        self.ctx._ck(self._L.mst_server_step(self.h, C.byref(n)), "mst_server_step")
+        return n.value
+
+    def drain(self):
+        self.ctx._ck(self._L.mst_server_drain(self.h), "mst_server_drain")
+
+    def job(self, jid):
+        st, n = C.c_int(0), C.c_int(0)
+        ts, ta, td = C.c_double(0), C.c_double(0), C.c_double(0)
+        self.ctx._ck(self._L.mst_server_job(self.h, jid, C.byref(st), C.byref(n), C.byref(ts),
+                                            C.byref(ta), C.byref(td)), "mst_server_job")
+        return {"state": st.value, "n": n.value, "t_submit": ts.value, "t_admit": ta.value,
+                "t_done": td.value}
+
+    def result(self, jid):
+        n = self.job(jid)["n"]
+        out = np.zeros((n, n), dtype=np.int32)
+        self.ctx._ck(self._L.mst_server_result(self.h, jid, _p(out)), "mst_server_result")
+        return out
+
+    def release(self, jid):
+        self.ctx._ck(self._L.mst_server_result(self.h, jid, None), "mst_server_result")
+
+    def completed(self, cap=4096):
+        """Ids of the jobs finished since the previous call."""
+        ids = np.zeros(cap, dtype=np.int64)
+        n = C.c_int(0)
+        self.ctx._ck(self._L.mst_server_completed(self.h, _p(ids), cap, C.byref(n)),
+                     "mst_server_completed")
+        return ids[: n.value].tolist()
+
+    def cancel(self, jid):
+        self.ctx._ck(self._L.mst_server_cancel(self.h, jid), "mst_server_cancel")
+
+    def stats(self):
+        w, r, i = C.c_int(0), C.c_int(0), C.c_int(0)
+        used, rate = C.c_int64(0), C.c_double(0)
+        self.ctx._ck(self._L.mst_server_stats(self.h, C.byref(w), C.byref(r), C.byref(i),
+                                              C.byref(used), C.byref(rate)), "mst_server_stats")
+        return {"waiting": w.value, "running": r.value, "inflight": i.value,
+                "arena_used": used.value, "cells_per_s": rate.value}
+
+    def set(self, key, value):
+        self.ctx._ck(self._L.mst_server_set(self.h, key.encode(), float(value)), "mst_server_set")

@@ -97,6 +97,8 @@ extern "C" int mst_create(int gap_open, int gap_ext, const int8_t* subst, int de
   for (int k = 0; k < 576; ++k)
     if (ctx->S[k] + gap_open + gap_ext > 127 || ctx->S[k] + gap_open + gap_ext < -128) return MST_E_ARG;
   ctx->symmetric = true;
+  ctx->smax = 0;
+  for (int k = 0; k < 576; ++k) ctx->smax = std::max(ctx->smax, (int)ctx->S[k]);
   for (int a = 0; a < 24; ++a)
     for (int b = 0; b < 24; ++b)
       if (ctx->S[a * 24 + b] != ctx->S[b * 24 + a]) ctx->symmetric = false;

@@ -54,9 +54,8 @@ __global__ void gather_codes(const int8_t* codes, const int64_t* off, const int3
 
 // Largest value a packed 16-bit lane can reach for R padded rows and n columns.
 int64_t pk16_bound(const mst_ctx* ctx, int64_t R, int64_t n) {
-  int smax = 0;
-  for (int k = 0; k < 576; ++k) smax = std::max(smax, (int)ctx->S[k]);
-  return (int64_t)smax * (std::min(R, n) + 1) + (int64_t)ctx->ge * (R + n + 2) + smax + ctx->go + ctx->ge;
+  const int64_t smax = ctx->smax;
+  return smax * (std::min(R, n) + 1) + (int64_t)ctx->ge * (R + n + 2) + smax + ctx->go + ctx->ge;
 }
 
 bool pk16_usable(const mst_ctx* ctx) {

@@ -1,0 +1,835 @@
+// Served form of the all-pairs scores: a request loop with continuous
+// batching. Jobs (sets of sequences) are submitted at any time; each call of
+// mst_server_step retires finished rounds, admits waiting jobs whose device
+// footprint fits the budget (FIFO), and launches the next round: a quantum of
+// work items drawn from the admitted jobs, earliest deadline first, packed
+// into shared launches of the streamed packed kernels (nw_score5.hip,
+// nw_score6.hip) whatever job they belong to.
+//
+// Device state: one arena of the budget's size holds, per admitted job, its
+// N x N int32 score matrix, its residue codes in length order and the ASCII
+// they are encoded from; a position table (soff: code offset, order: index in
+// the job) names every admitted sequence; a job table gives each job's matrix
+// offset and leading dimension, so a kernel writes a pair's score into its own
+// job's matrix. Rounds are enqueued on the context's stream, at most three in
+// flight. Inside a round the launches of different kernel classes run side by
+// side on four side streams (default priority) after the round's copies on
+// the context's stream; a round is over when every stream it used has passed
+// its end event, and the next round's copies do not wait for it.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <numeric>
+#include <vector>
+
+#include "../../include/mestrado.h"
+#include "ctx.h"
+#include "encode.h"
+#include "nw_score.h"
+#include "nw_score5.h"
+
+using namespace mst;
+
+#define CK(call)                                                                      \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess) return ctx->fail(MST_E_HIP, #call, hipGetErrorString(e_)); \
+  } while (0)
+
+int64_t pk16_bound(const mst_ctx* ctx, int64_t R, int64_t n);
+bool pk16_usable(const mst_ctx* ctx);
+
+namespace {
+
+__global__ void zero_u32s(unsigned int* p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+// scores of int32-kernel pairs (positions x, y) into their jobs' matrices
+__global__ void scatter_pairs(const int32_t* vals, const int2* pairs, const int* pair_job,
+                              const int64_t* job_out, const int* job_ld, const int32_t* order,
+                              int32_t* out, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int j = pair_job[i];
+  const int64_t base = job_out[j];
+  const int64_t ld = job_ld[j];
+  const int ox = order[pairs[i].x], oy = order[pairs[i].y];
+  out[base + ox * ld + oy] = vals[i];
+  out[base + (int64_t)oy * ld + ox] = vals[i];
+}
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// First-fit allocator over [0, cap) in units of `align`.
+struct Pool {
+  int64_t cap = 0, align = 1, used = 0;
+  std::map<int64_t, int64_t> free_;  // offset -> size
+  void init(int64_t c, int64_t a) {
+    cap = c;
+    align = a;
+    used = 0;
+    free_.clear();
+    if (c > 0) free_[0] = c;
+  }
+  int64_t round(int64_t n) const { return (n + align - 1) / align * align; }
+  int64_t alloc(int64_t n) {
+    n = round(std::max<int64_t>(n, 1));
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= n) {
+        const int64_t off = it->first, rest = it->second - n;
+        free_.erase(it);
+        if (rest > 0) free_[off + n] = rest;
+        used += n;
+        return off;
+      }
+    }
+    return -1;
+  }
+  void release(int64_t off, int64_t n) {
+    n = round(std::max<int64_t>(n, 1));
+    used -= n;
+    auto it = free_.emplace(off, n).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+};
+
+// Classes of the streamed kernels: 0 = score6, 1 + (W-1)*3 + (K-3) = score5<K, W>.
+constexpr int kClasses = 13;
+int class_of(int kind, int K, int W) { return kind == 6 ? 0 : 1 + (W - 1) * 3 + (K - 3); }
+
+struct Job {
+  int64_t id = 0;
+  int n = 0;
+  int state = MST_JOB_WAITING;
+  std::vector<char> ascii;        // input letters
+  std::vector<int64_t> off;       // input offsets (n + 1)
+  double t_submit = 0, t_admit = 0, t_done = 0, deadline = 0;
+  // device placement
+  int64_t arena_off = -1, arena_bytes = 0;  // one block: scores, codes, ascii, offsets
+  int64_t pos0 = -1;                        // positions pos0 .. pos0 + n (n + 1)
+  int slot = -1;
+  // plan: streamed items (positions) and int32 pairs (positions)
+  std::vector<Item3> items;
+  std::vector<int> item_cls;
+  std::vector<int64_t> item_cells;
+  std::vector<int2> fb;
+  size_t next_item = 0, next_fb = 0;
+  int64_t cells_total = 0, cells_left = 0;
+  int rounds_ref = 0;  // in-flight rounds holding work of this job
+  bool cancel = false;
+};
+
+struct RoundBuf {
+  Item3* d_items = nullptr;
+  int* d_item_job = nullptr;
+  int64_t cap_items = 0;
+  int2* d_fb = nullptr;
+  int* d_fb_job = nullptr;
+  int32_t* d_fb_val = nullptr;
+  int64_t cap_fb = 0;
+  unsigned int* d_counters = nullptr;
+  // pinned host staging of the round's items and int32 pairs
+  Item3* h_items = nullptr;
+  int* h_item_job = nullptr;
+  int2* h_fb = nullptr;
+  int* h_fb_job = nullptr;
+  // events of the round using this buffer: device time (start, end), the
+  // point the side streams start from, and each side stream's end
+  hipEvent_t ev_start = nullptr, ev_end = nullptr, ev_go = nullptr;
+  hipEvent_t ev_side[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+struct Round {
+  std::vector<hipEvent_t> evs;  // the round is over when all of these are
+  int buf = 0;
+  std::vector<int64_t> jobs;        // jobs with work in this round
+  std::vector<int64_t> finishing;   // jobs whose last work is in this round
+  int64_t cells = 0;
+  double t_launch = 0;
+};
+
+}  // namespace
+
+struct mst_server {
+  mst_ctx* ctx = nullptr;
+  int64_t budget = 0;
+  uint8_t* d_arena = nullptr;
+  Pool arena, pos, slots;
+  int64_t* d_soff = nullptr;
+  int32_t* d_order = nullptr;
+  int64_t* d_job_out = nullptr;
+  int* d_job_ld = nullptr;
+  int2* d_carry = nullptr;
+  int grid6 = 1;
+  int2* d_bufs = nullptr;  // int32 kernel strip buffers
+  int64_t bufs_cap_cols = 0;
+  int grid32 = 1;
+  static constexpr int kBufs = 3;  // rounds in flight at most
+  RoundBuf rb[kBufs];
+  hipStream_t side[4] = {nullptr, nullptr, nullptr, nullptr};  // class launches of a round
+  std::map<int64_t, Job> jobs;
+  std::deque<int64_t> waiting;
+  std::vector<int64_t> running;
+  std::deque<Round> inflight;
+  int64_t next_id = 1;
+  double rate = 4e12;          // cells per second, running estimate
+  double round_cells = 2e10;   // work per round (about 4 ms of the device)
+  double slack = 2.0;          // deadline = submission + slack x estimated run time
+  double t_retire = 0;         // when the last round was seen finished
+  int chunk = 8;
+  int64_t done_count = 0, cells_done = 0;
+  std::vector<int64_t> completed;  // finished since the last mst_server_completed
+};
+
+namespace {
+
+int64_t job_footprint(const Job& J) {
+  const int64_t L = J.off[J.n] - J.off[0];
+  auto r = [](int64_t x) { return (x + 255) / 256 * 256; };
+  return r((int64_t)J.n * J.n * 4) + r(L) + r(L) + r((int64_t)(J.n + 1) * 8);
+}
+
+// Streamed plan of one job whose sequences sit at positions pos0 .. pos0+n-1
+// in length order (lens ascending): the row pairs (2k, 2k+1) against every
+// later position, cut into runs of `chunk` column sequences.
+void plan_job(const mst_ctx* ctx, Job& J, const std::vector<int>& lens, int chunk) {
+  const int N = J.n;
+  const int p0 = (int)J.pos0;
+  auto add = [&](int kind, int K, int W, int64_t R, int pa, int pb, int first, int last) {
+    std::vector<int> elig;
+    for (int t = first; t <= last; ++t) {
+      const int64_t nx = lens[t];
+      if (kind == 0 || nx == 0 || pk16_bound(ctx, R, nx) > 32767 || (kind == 6 && nx > 16384)) {
+        J.fb.push_back(make_int2(p0 + t, p0 + pa));
+        if (pb >= 0) J.fb.push_back(make_int2(p0 + t, p0 + pb));
+      } else {
+        elig.push_back(t);
+      }
+    }
+    const int64_t rows = lens[pa] + (pb >= 0 ? lens[pb] : 0);
+    size_t c0 = 0;
+    while (c0 < elig.size()) {
+      size_t c1 = c0 + 1;
+      int64_t cols = lens[elig[c0]];
+      while (c1 < elig.size() && (int)(c1 - c0) < chunk && elig[c1] == elig[c1 - 1] + 1 &&
+             (kind != 6 || cols + lens[elig[c1]] <= 16384)) {
+        cols += lens[elig[c1]];
+        ++c1;
+      }
+      Item3 it;
+      it.ya = p0 + pa;
+      it.yb = pb >= 0 ? p0 + pb : -1;
+      it.xbegin = p0 + elig[c0];
+      it.xcount = (int)(c1 - c0);
+      J.items.push_back(it);
+      J.item_cls.push_back(class_of(kind, K, W));
+      J.item_cells.push_back(cols * rows);
+      c0 = c1;
+    }
+  };
+  for (int k = 0; 2 * k < N; ++k) {
+    const bool single = 2 * k + 1 >= N;
+    const int pa = 2 * k, pb = single ? -1 : 2 * k + 1;
+    const int m = single ? lens[pa] : lens[pb];
+    int kind, K, W = 1;
+    if (m == 0) {
+      kind = 0;
+      K = 3;
+    } else if (m <= 320) {  // one wave of K = 3..5 rows per lane
+      K = std::max(3, (m + 63) / 64);
+      kind = 5;
+    } else if (m <= 1024) {
+      W = (m + 255) / 256;
+      K = (m + 64 * W - 1) / (64 * W);
+      kind = 5;
+    } else {
+      kind = 6;
+      K = kScore6K;
+    }
+    const int64_t R = kind == 6 ? (int64_t)(m + 255) / 256 * 256 : (int64_t)W * 64 * K;
+    if (single) {
+      add(kind, K, W, R, pa, -1, pa, N - 1);
+    } else {
+      add(kind, K, W, R, pa, -1, pa, pa);
+      add(kind, K, W, R, pa, pb, pb, N - 1);
+    }
+  }
+  J.cells_total = 0;
+  for (int64_t c : J.item_cells) J.cells_total += c;
+  for (const int2& p : J.fb) J.cells_total += (int64_t)lens[p.x - p0] * lens[p.y - p0];
+  J.cells_left = J.cells_total;
+}
+
+void release_job(mst_server* S, Job& J) {
+  if (J.arena_off >= 0) S->arena.release(J.arena_off, J.arena_bytes);
+  if (J.pos0 >= 0) S->pos.release(J.pos0, J.n + 1);
+  if (J.slot >= 0) S->slots.release(J.slot, 1);
+  J.arena_off = J.pos0 = -1;
+  J.slot = -1;
+  J.items.clear();
+  J.items.shrink_to_fit();
+  J.item_cls.clear();
+  J.item_cells.clear();
+  J.fb.clear();
+  J.ascii.clear();
+  J.ascii.shrink_to_fit();
+}
+
+// Moves job J to the device and plans it. Returns MST_OK, or 1 when it does
+// not fit now (nothing changed), or an error.
+int admit(mst_server* S, Job& J) {
+  mst_ctx* ctx = S->ctx;
+  const int N = J.n;
+  const int64_t bytes = job_footprint(J);
+  const int64_t a = S->arena.alloc(bytes);
+  if (a < 0) return 1;
+  const int64_t p = S->pos.alloc(N + 1);
+  if (p < 0) {
+    S->arena.release(a, bytes);
+    return 1;
+  }
+  const int64_t sl = S->slots.alloc(1);
+  if (sl < 0) {
+    S->arena.release(a, bytes);
+    S->pos.release(p, N + 1);
+    return 1;
+  }
+  J.arena_off = a;
+  J.arena_bytes = bytes;
+  J.pos0 = p;
+  J.slot = (int)sl;
+  // length order, then the layout of the block
+  std::vector<int> ord(N);
+  std::iota(ord.begin(), ord.end(), 0);
+  auto len = [&](int i) { return (int)(J.off[i + 1] - J.off[i]); };
+  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return len(x) < len(y); });
+  const int64_t L = J.off[N] - J.off[0];
+  auto r = [](int64_t x) { return (x + 255) / 256 * 256; };
+  const int64_t o_scores = a, o_codes = a + r((int64_t)N * N * 4), o_ascii = o_codes + r(L),
+                o_roff = o_ascii + r(L);
+  std::vector<char> asc(std::max<int64_t>(L, 1));
+  std::vector<int64_t> roff(N + 1, 0), soff(N + 1, 0);
+  std::vector<int32_t> order(N + 1, 0);
+  std::vector<int> lens(N);
+  for (int i = 0; i < N; ++i) {
+    const int s = ord[i];
+    memcpy(asc.data() + roff[i], J.ascii.data() + (J.off[s] - J.off[0]), len(s));
+    roff[i + 1] = roff[i] + len(s);
+    lens[i] = len(s);
+    order[i] = s;
+  }
+  for (int i = 0; i <= N; ++i) soff[i] = o_codes + roff[i];
+  CK(hipMemcpy(S->d_arena + o_ascii, asc.data(), std::max<int64_t>(L, 1), hipMemcpyHostToDevice));
+  CK(hipMemcpy(S->d_arena + o_roff, roff.data(), sizeof(int64_t) * (N + 1), hipMemcpyHostToDevice));
+  CK(hipMemcpy(S->d_soff + p, soff.data(), sizeof(int64_t) * (N + 1), hipMemcpyHostToDevice));
+  CK(hipMemcpy(S->d_order + p, order.data(), sizeof(int32_t) * (N + 1), hipMemcpyHostToDevice));
+  const int64_t jo = o_scores / 4;
+  const int ld = N;
+  CK(hipMemcpy(S->d_job_out + sl, &jo, sizeof jo, hipMemcpyHostToDevice));
+  CK(hipMemcpy(S->d_job_ld + sl, &ld, sizeof ld, hipMemcpyHostToDevice));
+  launch_encode((const char*)(S->d_arena + o_ascii), (const int64_t*)(S->d_arena + o_roff), N,
+                (int8_t*)(S->d_arena + o_codes), ctx->stream);
+  CK(hipGetLastError());
+  plan_job(ctx, J, lens, S->chunk);
+  J.state = MST_JOB_RUNNING;
+  J.t_admit = now_s();
+  J.deadline = J.t_submit + S->slack * ((double)J.cells_total / S->rate + 0.0005);
+  J.ascii.clear();
+  J.ascii.shrink_to_fit();
+  return MST_OK;
+}
+
+int ensure_round_buf(mst_server* S, RoundBuf& B, int64_t nitems, int64_t nfb) {
+  mst_ctx* ctx = S->ctx;
+  if (!B.d_counters) {
+    CK(hipMalloc(&B.d_counters, sizeof(unsigned int) * (kClasses + 1)));
+    CK(hipEventCreate(&B.ev_start));
+    CK(hipEventCreate(&B.ev_end));
+    CK(hipEventCreateWithFlags(&B.ev_go, hipEventDisableTiming));
+    for (auto& e : B.ev_side) CK(hipEventCreate(&e));
+  }
+  // growing frees buffers, which waits for the device; capacities double
+  if (nitems > B.cap_items) {
+    if (B.d_items) CK(hipFree(B.d_items));
+    if (B.d_item_job) CK(hipFree(B.d_item_job));
+    if (B.h_items) CK(hipHostFree(B.h_items));
+    if (B.h_item_job) CK(hipHostFree(B.h_item_job));
+    B.cap_items = std::max<int64_t>(std::max<int64_t>(nitems, 4096), 2 * B.cap_items);
+    CK(hipMalloc(&B.d_items, sizeof(Item3) * B.cap_items));
+    CK(hipMalloc(&B.d_item_job, sizeof(int) * B.cap_items));
+    CK(hipHostMalloc(&B.h_items, sizeof(Item3) * B.cap_items, hipHostMallocDefault));
+    CK(hipHostMalloc(&B.h_item_job, sizeof(int) * B.cap_items, hipHostMallocDefault));
+  }
+  if (nfb > B.cap_fb) {
+    if (B.d_fb) CK(hipFree(B.d_fb));
+    if (B.d_fb_job) CK(hipFree(B.d_fb_job));
+    if (B.d_fb_val) CK(hipFree(B.d_fb_val));
+    if (B.h_fb) CK(hipHostFree(B.h_fb));
+    if (B.h_fb_job) CK(hipHostFree(B.h_fb_job));
+    B.cap_fb = std::max<int64_t>(std::max<int64_t>(nfb, 1024), 2 * B.cap_fb);
+    CK(hipMalloc(&B.d_fb, sizeof(int2) * B.cap_fb));
+    CK(hipMalloc(&B.d_fb_job, sizeof(int) * B.cap_fb));
+    CK(hipMalloc(&B.d_fb_val, sizeof(int32_t) * B.cap_fb));
+    CK(hipHostMalloc(&B.h_fb, sizeof(int2) * B.cap_fb, hipHostMallocDefault));
+    CK(hipHostMalloc(&B.h_fb_job, sizeof(int) * B.cap_fb, hipHostMallocDefault));
+  }
+  return MST_OK;
+}
+
+// Builds and enqueues one round. *launched = 0 when there is no work.
+int launch_round(mst_server* S, int* launched) {
+  mst_ctx* ctx = S->ctx;
+  *launched = 0;
+  std::vector<int64_t> order;
+  for (int64_t id : S->running) {
+    Job& J = S->jobs[id];
+    if (!J.cancel && (J.next_item < J.items.size() || J.next_fb < J.fb.size())) order.push_back(id);
+  }
+  if (order.empty()) return MST_OK;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int64_t x, int64_t y) { return S->jobs[x].deadline < S->jobs[y].deadline; });
+  int bi = 0;  // a buffer no in-flight round uses
+  for (; bi < mst_server::kBufs; ++bi) {
+    bool busy = false;
+    for (const Round& q : S->inflight) busy |= q.buf == bi;
+    if (!busy) break;
+  }
+  if (bi == mst_server::kBufs) return MST_OK;
+  // a round buffer is rewritten only after the round that used it retired
+  RoundBuf& B = S->rb[bi];
+  const int64_t budget = (int64_t)S->round_cells;
+  int64_t cells = 0;
+  std::vector<std::vector<std::pair<Item3, int>>> by_cls(kClasses);
+  std::vector<int2> fb;
+  std::vector<int> fb_job;
+  Round R;
+  R.buf = bi;
+  for (int64_t id : order) {
+    if (cells >= budget) break;
+    Job& J = S->jobs[id];
+    while (J.next_item < J.items.size() && cells < budget) {
+      const size_t q = J.next_item++;
+      by_cls[J.item_cls[q]].push_back(std::make_pair(J.items[q], J.slot));
+      cells += J.item_cells[q];
+      J.cells_left -= J.item_cells[q];
+    }
+    while (J.next_item == J.items.size() && J.next_fb < J.fb.size() && cells < budget) {
+      const int2 p = J.fb[J.next_fb++];
+      fb.push_back(p);
+      fb_job.push_back(J.slot);
+      cells += 1;  // the int32 kernel's share is small; it is timed with the round
+    }
+    J.rounds_ref += 1;
+    R.jobs.push_back(id);
+    if (J.next_item == J.items.size() && J.next_fb == J.fb.size()) R.finishing.push_back(id);
+  }
+  int64_t nitems = 0;
+  for (auto& v : by_cls) nitems += (int64_t)v.size();
+  int rc = ensure_round_buf(S, B, std::max<int64_t>(nitems, 1), std::max<int64_t>((int64_t)fb.size(), 1));
+  if (rc) return rc;
+  std::vector<std::pair<int64_t, int64_t>> ranges(kClasses);
+  int64_t q = 0;
+  for (int c = 0; c < kClasses; ++c) {
+    ranges[c].first = q;
+    for (auto& pr : by_cls[c]) {
+      B.h_items[q] = pr.first;
+      B.h_item_job[q] = pr.second;
+      ++q;
+    }
+    ranges[c].second = q;
+  }
+  CK(hipEventRecord(B.ev_start, ctx->stream));
+  if (nitems > 0) {
+    CK(hipMemcpyAsync(B.d_items, B.h_items, sizeof(Item3) * nitems, hipMemcpyHostToDevice,
+                      ctx->stream));
+    CK(hipMemcpyAsync(B.d_item_job, B.h_item_job, sizeof(int) * nitems, hipMemcpyHostToDevice,
+                      ctx->stream));
+  }
+  hipLaunchKernelGGL(zero_u32s, dim3(1), dim3(64), 0, ctx->stream, B.d_counters, kClasses + 1);
+  CK(hipGetLastError());
+  // the classes run side by side on the side streams, after everything the
+  // context's stream holds (encodes of new jobs, this round's copies)
+  CK(hipEventRecord(B.ev_go, ctx->stream));
+  bool used[4] = {false, false, false, false};
+  int nside = 0;
+  for (int c = 0; c < kClasses; ++c) {
+    if (ranges[c].second == ranges[c].first) continue;
+    const int si = nside++ % 4;
+    hipStream_t st = S->side[si];
+    if (!used[si]) {
+      CK(hipStreamWaitEvent(st, B.ev_go, 0));
+      used[si] = true;
+    }
+    Score5Args a;
+    a.scodes = (const int8_t*)S->d_arena;
+    a.soff = S->d_soff;
+    a.order = S->d_order;
+    a.items = B.d_items;
+    a.item_job = B.d_item_job;
+    a.job_out = S->d_job_out;
+    a.job_ld = S->d_job_ld;
+    a.first = ranges[c].first;
+    a.last = ranges[c].second;
+    a.counter = B.d_counters + c;
+    a.out_mat = (int32_t*)S->d_arena;
+    a.nseq = 0;
+    a.subst = ctx->d_S;
+    a.go = ctx->go;
+    a.ge = ctx->ge;
+    a.carry = S->d_carry;
+    a.carry_cap = 16384;
+    const int64_t nit = ranges[c].second - ranges[c].first;
+    if (c == 0) {
+      launch_score6(a, (int)std::min<int64_t>(S->grid6, nit), st);
+    } else {
+      const int W = (c - 1) / 3 + 1, K = (c - 1) % 3 + 3;
+      const int64_t g = std::min<int64_t>((int64_t)ctx->num_cus * score5_groups_per_cu(K, W), nit);
+      launch_score5(K, W, a, (int)std::max<int64_t>(g, 1), st);
+    }
+    CK(hipGetLastError());
+  }
+  if (!fb.empty()) {
+    // the int32 kernel on positions, then into the jobs' matrices
+    std::copy(fb.begin(), fb.end(), B.h_fb);
+    std::copy(fb_job.begin(), fb_job.end(), B.h_fb_job);
+    CK(hipMemcpyAsync(B.d_fb, B.h_fb, sizeof(int2) * fb.size(), hipMemcpyHostToDevice, ctx->stream));
+    CK(hipMemcpyAsync(B.d_fb_job, B.h_fb_job, sizeof(int) * fb.size(), hipMemcpyHostToDevice,
+                      ctx->stream));
+    ScoreArgs a;
+    a.seqs = (const int8_t*)S->d_arena;
+    a.off = S->d_soff;
+    a.pairs = B.d_fb;
+    a.first = 0;
+    a.last = (int64_t)fb.size();
+    a.out_pair = B.d_fb_val;
+    a.out_mat = nullptr;
+    a.nseq = 0;
+    a.bufs = S->d_bufs;
+    a.bufcap = S->bufs_cap_cols;
+    a.subst = ctx->d_S;
+    a.go = ctx->go;
+    a.ge = ctx->ge;
+    a.transpose_ok = ctx->symmetric ? 1 : 0;
+    launch_score_pairs(16, a, (int)std::min<int64_t>(S->grid32, (int64_t)fb.size()), ctx->stream);
+    CK(hipGetLastError());
+    const int n = (int)fb.size();
+    hipLaunchKernelGGL(scatter_pairs, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, B.d_fb_val,
+                       B.d_fb, B.d_fb_job, S->d_job_out, S->d_job_ld, S->d_order,
+                       (int32_t*)S->d_arena, n);
+    CK(hipGetLastError());
+  }
+  // the round ends with its side streams; the next round's copies and
+  // encodes on the context's stream do not wait for it (other buffer)
+  for (int si = 0; si < 4; ++si) {
+    if (!used[si]) continue;
+    CK(hipEventRecord(B.ev_side[si], S->side[si]));
+    R.evs.push_back(B.ev_side[si]);
+  }
+  CK(hipEventRecord(B.ev_end, ctx->stream));
+  R.evs.push_back(B.ev_end);
+  R.cells = cells;
+  R.t_launch = now_s();
+  S->inflight.push_back(R);
+  *launched = 1;
+  return MST_OK;
+}
+
+// Retires the oldest in-flight round if it has finished (or wait: block).
+int retire(mst_server* S, bool wait, int* ndone) {
+  mst_ctx* ctx = S->ctx;
+  if (S->inflight.empty()) return MST_OK;
+  Round& R = S->inflight.front();
+  for (hipEvent_t ev : R.evs) {
+    hipError_t e = wait ? hipEventSynchronize(ev) : hipEventQuery(ev);
+    if (e == hipErrorNotReady) return MST_OK;
+    if (e != hipSuccess) return ctx->fail(MST_E_HIP, "round", hipGetErrorString(e));
+  }
+  const double t = now_s();
+  // throughput estimate from the round's device time
+  float ms = 0;
+  for (hipEvent_t ev : R.evs) {
+    float m = 0;
+    if (hipEventElapsedTime(&m, S->rb[R.buf].ev_start, ev) == hipSuccess) ms = std::max(ms, m);
+  }
+  if (ms > 0 &&
+      R.cells > (int64_t)(0.25 * S->round_cells))
+    S->rate = 0.8 * S->rate + 0.2 * std::min(2e13, std::max(1e11, (double)R.cells / (ms * 1e-3)));
+  S->t_retire = t;
+  for (int64_t id : R.jobs) {
+    Job& J = S->jobs[id];
+    J.rounds_ref -= 1;
+  }
+  for (int64_t id : R.finishing) {
+    Job& J = S->jobs[id];
+    if (J.state == MST_JOB_RUNNING && !J.cancel) {
+      J.state = MST_JOB_DONE;
+      J.t_done = t;
+      S->done_count += 1;
+      S->cells_done += J.cells_total;
+      S->completed.push_back(id);
+      if (ndone) *ndone += 1;
+    }
+  }
+  S->inflight.pop_front();
+  // drop finished or cancelled jobs from the running list
+  std::vector<int64_t> keep;
+  for (int64_t id : S->running) {
+    Job& J = S->jobs[id];
+    if (J.state == MST_JOB_RUNNING && !J.cancel) {
+      keep.push_back(id);
+    } else if (J.cancel && J.rounds_ref == 0) {
+      release_job(S, J);
+      J.state = MST_JOB_CANCELLED;
+    } else if (J.cancel) {
+      keep.push_back(id);  // released once its rounds retire
+    }
+  }
+  S->running.swap(keep);
+  return MST_OK;
+}
+
+}  // namespace
+
+extern "C" int mst_server_create(mst_ctx* ctx, int64_t budget_bytes, mst_server** out) {
+  if (!ctx || !out || budget_bytes < (1 << 20)) return MST_E_ARG;
+  if (!pk16_usable(ctx)) return ctx->fail(MST_E_ARG, "mst_server_create", "the served form needs a symmetric matrix with S+go+ge in 0..255");
+  CK(hipSetDevice(ctx->device));
+  mst_server* S = new mst_server();
+  S->ctx = ctx;
+  S->budget = budget_bytes;
+  const int64_t max_pos = 1 << 21, max_slots = 1 << 14;
+  auto fail = [&](const char* what, hipError_t e) {
+    mst_server_destroy(S);
+    return ctx->fail(MST_E_NOMEM, what, hipGetErrorString(e));
+  };
+  hipError_t e;
+  if ((e = hipMalloc(&S->d_arena, budget_bytes)) != hipSuccess) return fail("arena", e);
+  if ((e = hipMalloc(&S->d_soff, sizeof(int64_t) * max_pos)) != hipSuccess) return fail("soff", e);
+  if ((e = hipMalloc(&S->d_order, sizeof(int32_t) * max_pos)) != hipSuccess) return fail("order", e);
+  if ((e = hipMalloc(&S->d_job_out, sizeof(int64_t) * max_slots)) != hipSuccess) return fail("jobs", e);
+  if ((e = hipMalloc(&S->d_job_ld, sizeof(int) * max_slots)) != hipSuccess) return fail("jobs", e);
+  S->grid6 = ctx->num_cus * score6_groups_per_cu();
+  if ((e = hipMalloc(&S->d_carry, sizeof(int2) * (size_t)S->grid6 * 16384)) != hipSuccess)
+    return fail("carry", e);
+  S->grid32 = ctx->num_cus * 4;
+  S->bufs_cap_cols = 1 << 15;  // int32 kernel: pairs up to 32768 residues
+  if ((e = hipMalloc(&S->d_bufs, sizeof(int2) * 2 * (size_t)S->bufs_cap_cols * S->grid32)) != hipSuccess)
+    return fail("bufs", e);
+  for (auto& st : S->side)
+    if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess) return fail("stream", e);
+  S->arena.init(budget_bytes, 256);
+  S->pos.init(max_pos, 1);
+  S->slots.init(max_slots, 1);
+  *out = S;
+  return MST_OK;
+}
+
+extern "C" int mst_server_destroy(mst_server* S) {
+  if (!S) return MST_E_ARG;
+  mst_ctx* ctx = S->ctx;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  for (hipStream_t st : S->side)
+    if (st) hipStreamSynchronize(st);
+  for (RoundBuf& B : S->rb) {
+    hipEvent_t evs[] = {B.ev_start, B.ev_end, B.ev_go, B.ev_side[0], B.ev_side[1], B.ev_side[2],
+                        B.ev_side[3]};
+    for (hipEvent_t e : evs)
+      if (e) hipEventDestroy(e);
+    void* ps[] = {B.d_items, B.d_item_job, B.d_fb, B.d_fb_job, B.d_fb_val, B.d_counters};
+    for (void* p : ps)
+      if (p) hipFree(p);
+    void* hs[] = {B.h_items, B.h_item_job, B.h_fb, B.h_fb_job};
+    for (void* p : hs)
+      if (p) hipHostFree(p);
+  }
+  void* ps[] = {S->d_arena, S->d_soff, S->d_order, S->d_job_out, S->d_job_ld, S->d_carry, S->d_bufs};
+  for (void* p : ps)
+    if (p) hipFree(p);
+  for (hipStream_t st : S->side)
+    if (st) hipStreamDestroy(st);
+  delete S;
+  return MST_OK;
+}
+
+extern "C" int mst_server_submit(mst_server* S, const char* ascii, const int64_t* offsets, int n,
+                                 int64_t* job) {
+  if (!S || !job || n < 1 || !offsets || (!ascii && offsets[n] > offsets[0])) return MST_E_ARG;
+  mst_ctx* ctx = S->ctx;
+  for (int i = 0; i < n; ++i)
+    if (offsets[i + 1] < offsets[i]) return ctx->fail(MST_E_ARG, "mst_server_submit", "offsets decrease");
+  Job J;
+  J.id = S->next_id++;
+  J.n = n;
+  J.off.assign(offsets, offsets + n + 1);
+  J.ascii.assign(ascii ? ascii + offsets[0] : nullptr, ascii ? ascii + offsets[n] : nullptr);
+  J.t_submit = now_s();
+  if (job_footprint(J) > S->budget || n + 1 > S->pos.cap) {
+    J.state = MST_JOB_REJECTED;
+    J.t_done = J.t_submit;
+    J.ascii.clear();
+  } else {
+    S->waiting.push_back(J.id);
+  }
+  *job = J.id;
+  S->jobs.emplace(J.id, std::move(J));
+  return MST_OK;
+}
+
+extern "C" int mst_server_step(mst_server* S, int* ncompleted) {
+  if (!S) return MST_E_ARG;
+  mst_ctx* ctx = S->ctx;
+  CK(hipSetDevice(ctx->device));
+  int done = 0, rc;
+  // retire every finished round
+  for (;;) {
+    const size_t before = S->inflight.size();
+    if ((rc = retire(S, false, &done))) return rc;
+    if (S->inflight.size() == before) break;
+  }
+  // admit in arrival order while the head fits
+  while (!S->waiting.empty()) {
+    Job& J = S->jobs[S->waiting.front()];
+    if (J.cancel) {
+      J.state = MST_JOB_CANCELLED;
+      S->waiting.pop_front();
+      continue;
+    }
+    rc = admit(S, J);
+    if (rc == 1) break;
+    if (rc) return rc;
+    S->running.push_back(J.id);
+    S->waiting.pop_front();
+  }
+  // keep up to three rounds queued on the device
+  while ((int)S->inflight.size() < mst_server::kBufs) {
+    int launched = 0;
+    if ((rc = launch_round(S, &launched))) return rc;
+    if (!launched) break;
+  }
+  if (ncompleted) *ncompleted = done;
+  return MST_OK;
+}
+
+extern "C" int mst_server_drain(mst_server* S) {
+  if (!S) return MST_E_ARG;
+  for (;;) {
+    int rc = mst_server_step(S, nullptr);
+    if (rc) return rc;
+    if (S->inflight.empty()) {
+      if (S->waiting.empty()) return MST_OK;
+      // the head of the queue does not fit although nothing runs: cannot happen
+      // for an accepted job (its footprint is within the budget), so the arena
+      // is held by finished jobs whose results were not collected
+      return S->ctx->fail(MST_E_STATE, "mst_server_drain",
+                          "waiting jobs do not fit: collect or release finished jobs");
+    }
+    int d = 0;
+    if ((rc = retire(S, true, &d))) return rc;
+  }
+}
+
+extern "C" int mst_server_job(mst_server* S, int64_t job, int* state, int* n, double* t_submit,
+                              double* t_admit, double* t_done) {
+  if (!S) return MST_E_ARG;
+  auto it = S->jobs.find(job);
+  if (it == S->jobs.end()) return S->ctx->fail(MST_E_ARG, "mst_server_job", "unknown job");
+  const Job& J = it->second;
+  if (state) *state = J.state;
+  if (n) *n = J.n;
+  if (t_submit) *t_submit = J.t_submit;
+  if (t_admit) *t_admit = J.t_admit;
+  if (t_done) *t_done = J.t_done;
+  return MST_OK;
+}
+
+extern "C" int mst_server_result(mst_server* S, int64_t job, int32_t* scores_nxn) {
+  if (!S) return MST_E_ARG;
+  mst_ctx* ctx = S->ctx;
+  auto it = S->jobs.find(job);
+  if (it == S->jobs.end()) return ctx->fail(MST_E_ARG, "mst_server_result", "unknown job");
+  Job& J = it->second;
+  if (J.state != MST_JOB_DONE) return ctx->fail(MST_E_STATE, "mst_server_result", "job not done");
+  CK(hipSetDevice(ctx->device));
+  if (scores_nxn)
+    CK(hipMemcpy(scores_nxn, S->d_arena + J.arena_off, sizeof(int32_t) * (size_t)J.n * J.n,
+                 hipMemcpyDeviceToHost));
+  release_job(S, J);
+  S->jobs.erase(it);
+  return MST_OK;
+}
+
+extern "C" int mst_server_cancel(mst_server* S, int64_t job) {
+  if (!S) return MST_E_ARG;
+  auto it = S->jobs.find(job);
+  if (it == S->jobs.end()) return S->ctx->fail(MST_E_ARG, "mst_server_cancel", "unknown job");
+  Job& J = it->second;
+  if (J.state == MST_JOB_WAITING || J.state == MST_JOB_RUNNING) J.cancel = true;
+  if (J.state == MST_JOB_WAITING) {
+    J.state = MST_JOB_CANCELLED;
+    S->waiting.erase(std::find(S->waiting.begin(), S->waiting.end(), job));
+  } else if (J.state == MST_JOB_RUNNING && J.rounds_ref == 0) {
+    release_job(S, J);
+    J.state = MST_JOB_CANCELLED;
+    S->running.erase(std::find(S->running.begin(), S->running.end(), job));
+  }
+  return MST_OK;
+}
+
+extern "C" int mst_server_stats(mst_server* S, int* waiting, int* running, int* inflight,
+                                int64_t* arena_used, double* rate) {
+  if (!S) return MST_E_ARG;
+  if (waiting) *waiting = (int)S->waiting.size();
+  if (running) *running = (int)S->running.size();
+  if (inflight) *inflight = (int)S->inflight.size();
+  if (arena_used) *arena_used = S->arena.used;
+  if (rate) *rate = S->rate;
+  return MST_OK;
+}
+
+extern "C" int mst_server_set(mst_server* S, const char* key, double value) {
+  if (!S || !key) return MST_E_ARG;
+  if (!strcmp(key, "round_cells") && value >= 1e6 && value <= 1e14) {
+    S->round_cells = value;
+    return MST_OK;
+  }
+  if (!strcmp(key, "chunk") && value >= 1 && value <= 4096) {
+    S->chunk = (int)value;
+    return MST_OK;
+  }
+  if (!strcmp(key, "slack") && value >= 1 && value <= 100) {
+    S->slack = value;
+    return MST_OK;
+  }
+  if (!strcmp(key, "rate") && value > 0) {
+    S->rate = value;
+    return MST_OK;
+  }
+  return S->ctx->fail(MST_E_ARG, "mst_server_set", "unknown key or value out of range");
+}
+
+extern "C" int mst_server_completed(mst_server* S, int64_t* ids, int cap, int* count) {
+  if (!S || !count || (cap > 0 && !ids)) return MST_E_ARG;
+  const int n = (int)std::min<size_t>(S->completed.size(), (size_t)std::max(cap, 0));
+  for (int i = 0; i < n; ++i) ids[i] = S->completed[i];
+  S->completed.erase(S->completed.begin(), S->completed.begin() + n);
+  *count = n;
+  return MST_OK;
+}

@@ -54,6 +54,7 @@ struct mst_ctx {
   int go = 10, ge = 2;
   int8_t S[576];
   bool symmetric = true;
+  int smax = 0;  // largest entry of S
   int8_t* d_S = nullptr;
   std::string err;
   std::map<int, SeqSet> sets;
