@@ -23,9 +23,39 @@ static const char kLetters[] = "ARNDCQEGHILKMFPSTWYVBZX*";
 // Runs the forward pass and traceback of `jobs` (already sized) and returns
 // the paths on the host. Jobs are processed in batches whose flag blocks fit
 // the context's traceback budget.
+static int run_align_batches(mst_ctx* ctx, int src, const int8_t* d_seqs, const uint8_t* d_tab,
+                             std::vector<AlignJob>& jobs, std::vector<int32_t>& scores,
+                             std::vector<std::vector<uint8_t>>& paths);
+
+// Jobs of one 512-row strip run one per wave (align_kernel); longer ones one
+// per workgroup with a wave per strip (align_pipe_kernel). The jobs are put in
+// that order for the batches and the results put back in the caller's order.
 int run_align_jobs(mst_ctx* ctx, int src, const int8_t* d_seqs, const uint8_t* d_tab,
                    std::vector<AlignJob>& jobs, std::vector<int32_t>& scores,
                    std::vector<std::vector<uint8_t>>& paths) {
+  const size_t nj = jobs.size();
+  std::vector<size_t> perm(nj);
+  for (size_t k = 0; k < nj; ++k) perm[k] = k;
+  std::stable_partition(perm.begin(), perm.end(), [&](size_t k) { return align_strips(jobs[k].m) == 1; });
+  std::vector<AlignJob> sorted(nj);
+  for (size_t k = 0; k < nj; ++k) sorted[k] = jobs[perm[k]];
+  std::vector<int32_t> sc;
+  std::vector<std::vector<uint8_t>> pa;
+  int rc = run_align_batches(ctx, src, d_seqs, d_tab, sorted, sc, pa);
+  if (rc) return rc;
+  scores.assign(nj, 0);
+  paths.assign(nj, {});
+  for (size_t k = 0; k < nj; ++k) {
+    jobs[perm[k]] = sorted[k];
+    scores[perm[k]] = sc[k];
+    paths[perm[k]].swap(pa[k]);
+  }
+  return MST_OK;
+}
+
+static int run_align_batches(mst_ctx* ctx, int src, const int8_t* d_seqs, const uint8_t* d_tab,
+                             std::vector<AlignJob>& jobs, std::vector<int32_t>& scores,
+                             std::vector<std::vector<uint8_t>>& paths) {
   const int64_t nj = (int64_t)jobs.size();
   scores.assign(nj, 0);
   paths.assign(nj, {});
@@ -55,9 +85,18 @@ int run_align_jobs(mst_ctx* ctx, int src, const int8_t* d_seqs, const uint8_t* d
     if (rc) return rc;
     rc = ctx->ensure((void**)&ctx->d_jscore, &ctx->jscore_cap, sizeof(int32_t) * 2 * nb);
     if (rc) return rc;
-    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ctx->num_cus * 16, nb));
+    // jobs [k0, ks) are single-strip, [ks, k1) longer (the list is partitioned)
+    int64_t ks = k0;
+    while (ks < k1 && align_strips(jobs[ks].m) == 1) ++ks;
+    const int64_t n1 = ks - k0, n2 = k1 - ks;
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ctx->num_cus * 16, n1));
+    const int grid2 = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ctx->num_cus * 2, n2));
     const int64_t bufcap = maxn;
-    rc = ctx->ensure((void**)&ctx->d_bufs, &ctx->bufs_cap, sizeof(int2) * 2 * bufcap * (size_t)grid);
+    // the two launches run one after the other and share the buffers
+    rc = ctx->ensure((void**)&ctx->d_bufs, &ctx->bufs_cap,
+                     sizeof(int2) * bufcap * std::max<size_t>(2 * (size_t)grid, (size_t)grid2));
+    if (rc) return rc;
+    rc = ctx->ensure((void**)&ctx->d_acounter, &ctx->acounter_cap, sizeof(unsigned int));
     if (rc) return rc;
     CK(hipMemcpyAsync(ctx->d_jobs, jobs.data() + k0, sizeof(AlignJob) * nb, hipMemcpyHostToDevice,
                       ctx->stream));
@@ -74,8 +113,22 @@ int run_align_jobs(mst_ctx* ctx, int src, const int8_t* d_seqs, const uint8_t* d
     a.bufcap = bufcap;
     a.go = ctx->go;
     a.ge = ctx->ge;
-    launch_align(src, a, grid, ctx->stream);
-    CK(hipGetLastError());
+    a.counter = ctx->d_acounter;
+    if (n1 > 0) {
+      a.first = 0;
+      a.last = n1;
+      launch_align(src, a, grid, ctx->stream);
+      CK(hipGetLastError());
+    }
+    if (n2 > 0) {
+      CK(hipMemsetAsync(ctx->d_acounter, 0, sizeof(unsigned int), ctx->stream));
+      a.first = n1;
+      a.last = nb;
+      launch_align_pipe(src, 4, a, grid2, ctx->stream);
+      CK(hipGetLastError());
+    }
+    a.first = 0;
+    a.last = nb;
     TraceArgs t;
     t.jobs = (const AlignJob*)ctx->d_jobs;
     t.first = 0;

@@ -33,8 +33,9 @@ struct MsaScratch {
   int32_t *sx = nullptr, *sy = nullptr, *res = nullptr;  // res[0] score, res[1] length
   AlignJob* job = nullptr;
   int2* bufs = nullptr;
+  unsigned int* counter = nullptr;  // pipelined flag kernel's work queue head
   size_t ccx = 0, ccy = 0, cP = 0, ctab = 0, cfl = 0, cops = 0, csx = 0, csy = 0, cres = 0, cjob = 0,
-         cbuf = 0;
+         cbuf = 0, ccnt = 0;
 };
 
 }  // namespace
@@ -53,6 +54,7 @@ static int ensure_all(mst_ctx* ctx, MsaScratch& w, int n, int m) {
   if ((rc = ctx->ensure((void**)&w.res, &w.cres, sizeof(int32_t) * 2))) return rc;
   if ((rc = ctx->ensure((void**)&w.job, &w.cjob, sizeof(AlignJob)))) return rc;
   if ((rc = ctx->ensure((void**)&w.bufs, &w.cbuf, sizeof(int2) * 2 * (size_t)n))) return rc;
+  if ((rc = ctx->ensure((void**)&w.counter, &w.ccnt, sizeof(unsigned int)))) return rc;
   return MST_OK;
 }
 
@@ -76,7 +78,8 @@ extern "C" int mst_msa(mst_ctx* ctx, int id, const mst_join* joins, int* msa_len
   auto cleanup = [&]() {
     for (auto& x : g)
       if (x.rows) hipFree(x.rows);
-    void* ps[] = {w.cx, w.cy, w.P, w.tab, w.flags, w.ops, w.sx, w.sy, w.res, w.job, w.bufs};
+    void* ps[] = {w.cx, w.cy, w.P, w.tab, w.flags, w.ops, w.sx, w.sy, w.res, w.job, w.bufs,
+                  w.counter};
     for (void* p : ps)
       if (p) hipFree(p);
   };
@@ -144,7 +147,16 @@ extern "C" int mst_msa(mst_ctx* ctx, int id, const mst_join* joins, int* msa_len
       a.bufcap = n;
       a.go = ctx->go;
       a.ge = ctx->ge;
-      launch_align(SRC_TAB, a, 1, ctx->stream);
+      a.counter = w.counter;
+      if (align_strips(m) > 1) {  // a wave per strip, 8 strips in flight
+        if (hipMemsetAsync(w.counter, 0, sizeof(unsigned int), ctx->stream) != hipSuccess) {
+          rc = ctx->fail(MST_E_HIP, "hipMemsetAsync");
+          break;
+        }
+        launch_align_pipe(SRC_TAB, 8, a, 1, ctx->stream);
+      } else {
+        launch_align(SRC_TAB, a, 1, ctx->stream);
+      }
       TraceArgs t;
       t.jobs = w.job;
       t.first = 0;

@@ -106,5 +106,7 @@ struct mst_ctx {
 
   ~mst_ctx();
   int fail(int code, const char* what, const char* detail = nullptr);
+  unsigned int* d_acounter = nullptr;  // work queue head of the pipelined flag kernel
+  size_t acounter_cap = 0;
   int ensure(void** p, size_t* cap, size_t bytes);
 };

@@ -26,6 +26,7 @@ struct AlignArgs {
   int2* bufs;
   int64_t bufcap;
   int go, ge;
+  unsigned int* counter;  // pipelined form: work queue head (zero before the launch)
 };
 
 struct TraceArgs {
@@ -37,6 +38,13 @@ struct TraceArgs {
 };
 
 void launch_align(int src, const AlignArgs& a, int grid, hipStream_t st);
+// Pipelined form for jobs of more than one 512-row strip: one job per
+// workgroup of W waves (W = 4, or 8 with SRC_TAB); wave j computes strips j,
+// j+W, ...; a strip reads the row above from the wave of the previous strip
+// through an LDS ring, or, across the wrap from wave W-1 to wave 0, through a
+// row buffer of bufcap columns per workgroup in bufs.
+void launch_align_pipe(int src, int W, const AlignArgs& a, int grid, hipStream_t st);
+int64_t align_strips(int m);  // 512-row strips of an m-row job
 void launch_traceback(const TraceArgs& a, hipStream_t st);
 int64_t flag_words(int n, int m);  // flag block size of an n x m job
 

@@ -194,6 +194,280 @@ __device__ void arun_strip(ALane& L, const int8_t* X, int n, const uint8_t* prof
   }
 }
 
+// ---- pipelined form: the strips of one job on the waves of a workgroup ----
+
+constexpr int ARING = 128;  // columns held by an LDS ring between two waves
+
+__device__ __forceinline__ int aflag_load(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void aflag_store(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+// every lane of the wave spins on one LDS word; the wave it waits for is in
+// the same workgroup, hence resident, and does not wait on this one
+__device__ __forceinline__ void await_at_least(const int* p, int v) {
+  while (aflag_load(p) < v) __builtin_amdgcn_s_sleep(1);
+}
+
+// F and H of the row above a strip, per column: element (base + c) & mask of p
+// (an LDS ring: mask ARING-1; the per-workgroup row buffer: mask -1).
+struct ALink {
+  int2* p;
+  int base, mask;
+};
+
+template <int SRC, bool PRED, bool FIRST, bool STORE>
+__device__ __forceinline__ void astep_link(ALane& L, int t, int x, uint2 tb, int2 bv,
+                                           const uint8_t* prof_lane, int n, int G,
+                                           const ALink& out, uint32_t* fl_strip) {
+  const int lane = lane_id();
+  int fin, hup;
+  if (FIRST) {
+    fin = shr1_const(L.Fout, MST_NEG_INF);
+    hup = shr1_const(L.Hout, 0);
+  } else {
+    fin = shr1(bv.x, L.Fout);
+    hup = shr1(bv.y, L.Hout);
+  }
+  const int c = t - lane;
+  const bool act = !PRED || (c >= 0 && c < n);
+  if (act) {
+    uint32_t w0, w1;
+    if (SRC == SRC_SEQ) {
+      uint2 v = *(const uint2*)(prof_lane + x * (MST_WAVE * AK));
+      w0 = v.x;
+      w1 = v.y;
+    } else {
+      w0 = tb.x;
+      w1 = tb.y;
+    }
+    int f = fin, hu = hup;
+    const uint32_t word = cell_column_flags(w0, w1, L, f, hu, G);
+    L.Fout = f;
+    L.Hout = hu;
+    fl_strip[(int64_t)t * MST_WAVE + lane] = word;
+    if (STORE && lane == MST_WAVE - 1) out.p[(out.base + c) & out.mask] = make_int2(f, hu);
+  }
+  L.hdiag = hup;
+}
+
+template <int SRC, bool FIRST>
+__device__ __forceinline__ void aload_link(int t0, const int8_t* __restrict__ X, int n,
+                                           const ALink& in, const uint2* tab_strip, int* xs,
+                                           int2* bs, uint2* ts) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int j = 0; j < AU; ++j) {
+    const int t = t0 + j;
+    int c = t - lane;
+    c = c < 0 ? 0 : (c >= n ? n - 1 : c);
+    if (SRC == SRC_SEQ) {
+      xs[j] = X[c];
+      ts[j] = make_uint2(0, 0);
+    } else {
+      xs[j] = 0;
+      int tt = t < n + MST_WAVE - 1 ? t : n + MST_WAVE - 2;
+      ts[j] = tab_strip[(int64_t)tt * MST_WAVE + lane];
+    }
+    if (!FIRST) {
+      int tt = t >= n ? n - 1 : t;
+      bs[j] = in.p[(in.base + tt) & in.mask];
+    } else {
+      bs[j] = make_int2(0, 0);
+    }
+  }
+}
+
+// One strip swept by one wave, reading the row above through `in` and writing
+// its last row through `out`. Counters count columns in global units: gin +
+// column for `in` (its producer's units), gout + column for `out`.
+//   prog_in: columns written to `in`; cons_out: columns this wave has read
+//   from it; prog_out / cons_in: the same for `out`. span: columns `out`
+//   holds before a slot is reused (0: never within a sweep); pub: blocks
+//   between two releases of prog_out.
+template <int SRC, bool FIRST, bool STORE>
+__device__ void arun_strip_pipe(ALane& L, const int8_t* X, int n, const uint8_t* prof_lane,
+                                const uint2* tab_strip, const ALink& in, int gin,
+                                const int* prog_in, int* cons_out, const ALink& out, int gout,
+                                int* prog_out, const int* cons_in, int span, int pub, int G,
+                                uint32_t* fl_strip) {
+  const int lane = lane_id();
+  const int T = n + MST_WAVE - 1;
+  int xa[AU], xb[AU];
+  int2 ba[AU], bb[AU];
+  uint2 ta[AU], tbb[AU];
+  if (!FIRST) await_at_least(prog_in, gin + min(AU, n));
+  aload_link<SRC, FIRST>(0, X, n, in, tab_strip, xa, ba, ta);
+  int blk = 0;
+  for (int t0 = 0; t0 < T; t0 += AU, ++blk) {
+    if (!FIRST) await_at_least(prog_in, gin + min(t0 + 2 * AU, n));
+    aload_link<SRC, FIRST>(t0 + AU, X, n, in, tab_strip, xb, bb, tbb);
+    if (!FIRST && lane == 0) aflag_store(cons_out, gin + min(t0 + 2 * AU, n));
+    if (STORE && span > 0) {
+      // this block writes columns t0-63 .. t0-56, whose slots held the columns
+      // span earlier (in global units); the reader must be past them
+      const int need = gout + t0 - (MST_WAVE - 1) + (AU - 1) - span + 1;
+      if (need > 0) await_at_least(cons_in, min(need, gout + n));
+    }
+    const bool full = (t0 >= MST_WAVE - 1) && (t0 + AU <= n);
+    if (full) {
+#pragma unroll
+      for (int j = 0; j < AU; ++j)
+        astep_link<SRC, false, FIRST, STORE>(L, t0 + j, xa[j], ta[j], ba[j], prof_lane, n, G, out,
+                                             fl_strip);
+    } else {
+#pragma unroll
+      for (int j = 0; j < AU; ++j)
+        if (t0 + j < T)
+          astep_link<SRC, true, FIRST, STORE>(L, t0 + j, xa[j], ta[j], ba[j], prof_lane, n, G, out,
+                                              fl_strip);
+    }
+    if (STORE && (blk % pub) == pub - 1) {
+      int done = t0 + AU - (MST_WAVE - 1);
+      done = done < 0 ? 0 : (done > n ? n : done);
+      if (lane == MST_WAVE - 1) aflag_store(prog_out, gout + done);
+    }
+#pragma unroll
+    for (int j = 0; j < AU; ++j) {
+      xa[j] = xb[j];
+      ba[j] = bb[j];
+      ta[j] = tbb[j];
+    }
+  }
+  if (STORE && lane == MST_WAVE - 1) aflag_store(prog_out, gout + n);
+}
+
+// A workgroup of W waves takes one job at a time; wave j computes strips j,
+// j+W, j+2W, ... (sweep q = strip / W). Strip s reads the row above from
+// strip s-1: wave j-1 in the same sweep through an LDS ring, or, for wave 0,
+// wave W-1's previous sweep through the workgroup's row buffer (`carry`).
+// Every wait ends: a wave waits only for data of an earlier strip or for a
+// later strip to have read a ring slot, and wave W-1 never waits for the
+// carry to be read (wave 0 reads its column c before the data of column c of
+// the same sweep can reach wave W-1).
+template <int SRC, int W>
+__global__ void __launch_bounds__(MST_WAVE* W) align_pipe_kernel(AlignArgs a) {
+  constexpr int SLOT = (SRC == SRC_SEQ) ? MST_LETTERS * MST_WAVE * AK : 16;
+  __shared__ int8_t S_lds[MST_LETTERS * MST_LETTERS];
+  __shared__ __attribute__((aligned(16))) uint8_t prof[W * SLOT];
+  __shared__ int2 ring[(W - 1) * ARING];
+  __shared__ int s_prog[W + 1], s_cons[W + 1];  // [W]: the carry between sweeps
+  __shared__ int s_item;
+  const int lane = lane_id();
+  const int j = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave index (uniform)
+  if (SRC == SRC_SEQ) {
+    for (int i = threadIdx.x; i < MST_LETTERS * MST_LETTERS; i += MST_WAVE * W) S_lds[i] = a.subst[i];
+  }
+  const int G = a.go - a.ge;
+  uint8_t* slot = prof + j * SLOT;
+  const uint8_t* prof_lane = slot + lane * AK;
+  int2* carry = a.bufs + (int64_t)blockIdx.x * a.bufcap;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = (int)atomicAdd(a.counter, 1u);
+    if (threadIdx.x < W + 1) {
+      s_prog[threadIdx.x] = 0;
+      s_cons[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    const int64_t it = (int64_t)s_item + a.first;
+    if (it >= a.last) break;
+    const AlignJob job = a.jobs[it];
+    const int n = job.n, m = job.m;
+    const int nstrips = (m + ASH - 1) / ASH;
+    const int Tn = n + MST_WAVE - 1;
+    const int8_t* X = a.seqs + job.x_off;
+    const int8_t* Y = a.seqs + job.y_off;
+    uint32_t* fl = a.flags + job.flag_off;
+    for (int q = 0;; ++q) {
+      const int s = j + q * W;
+      if (s >= nstrips) break;
+      const int r0 = s * ASH;
+      if (SRC == SRC_SEQ) {
+        // this wave's previous sweep is over and no other wave reads its slot
+        __builtin_amdgcn_wave_barrier();
+        build_profile8(slot, S_lds, Y, m, r0, a.go + a.ge);
+        __builtin_amdgcn_wave_barrier();
+      }
+      ALane L;
+#pragma unroll
+      for (int i = 0; i < AK; ++i) {
+        L.Hm[i] = 0;
+        L.E[i] = MST_NEG_INF;
+      }
+      L.Fout = MST_NEG_INF;
+      L.Hout = 0;
+      L.hdiag = 0;
+      const int g = q * n;
+      ALink in, out;
+      int gin;
+      const int* prog_in;
+      int* cons_out;
+      if (j > 0) {
+        in = ALink{ring + (j - 1) * ARING, g, ARING - 1};
+        gin = g;
+        prog_in = s_prog + j - 1;
+        cons_out = s_cons + j;
+      } else {
+        in = ALink{carry, 0, -1};
+        gin = g - n;
+        prog_in = s_prog + W;
+        cons_out = s_cons + W;
+      }
+      int* prog_out;
+      const int* cons_in;
+      int span, pub;
+      if (j < W - 1) {
+        out = ALink{ring + j * ARING, g, ARING - 1};
+        prog_out = s_prog + j;
+        cons_in = s_cons + j + 1;
+        span = ARING;
+        pub = 1;
+      } else {
+        out = ALink{carry, 0, -1};
+        prog_out = s_prog + W;
+        cons_in = nullptr;
+        span = 0;
+        pub = 16;
+      }
+      uint32_t* fl_strip = fl + (int64_t)s * Tn * MST_WAVE;
+      const uint2* tab_strip =
+          (SRC == SRC_TAB) ? (const uint2*)(a.tab + job.tab_off) + (int64_t)s * Tn * MST_WAVE : nullptr;
+      const bool first = (s == 0), last = (s == nstrips - 1);
+      if (first) {
+        if (last)
+          arun_strip_pipe<SRC, true, false>(L, X, n, prof_lane, tab_strip, in, gin, prog_in,
+                                            cons_out, out, g, prog_out, cons_in, span, pub, G,
+                                            fl_strip);
+        else
+          arun_strip_pipe<SRC, true, true>(L, X, n, prof_lane, tab_strip, in, gin, prog_in,
+                                           cons_out, out, g, prog_out, cons_in, span, pub, G,
+                                           fl_strip);
+      } else {
+        if (last)
+          arun_strip_pipe<SRC, false, false>(L, X, n, prof_lane, tab_strip, in, gin, prog_in,
+                                             cons_out, out, g, prog_out, cons_in, span, pub, G,
+                                             fl_strip);
+        else
+          arun_strip_pipe<SRC, false, true>(L, X, n, prof_lane, tab_strip, in, gin, prog_in,
+                                            cons_out, out, g, prog_out, cons_in, span, pub, G,
+                                            fl_strip);
+      }
+      if (last) {
+        const int rl = (m - 1) - r0;
+        const int owner = rl / AK, idx = rl % AK;
+        int v = L.Hm[0];
+#pragma unroll
+        for (int i = 1; i < AK; ++i)
+          if (idx == i) v = L.Hm[i];
+        v = __shfl(v, owner);
+        if (lane == 0) a.scores[it] = v - a.ge * (m + n);
+      }
+    }
+    __syncthreads();  // rings, counters and the row buffer are reused by the next job
+  }
+}
+
 template <int SRC>
 __global__ void __launch_bounds__(MST_WAVE) align_kernel(AlignArgs a) {
   __shared__ int8_t S_lds[MST_LETTERS * MST_LETTERS];
@@ -322,6 +596,17 @@ void launch_align(int src, const AlignArgs& a, int grid, hipStream_t st) {
   else
     hipLaunchKernelGGL(align_kernel<SRC_TAB>, dim3(grid), dim3(MST_WAVE), 0, st, a);
 }
+
+void launch_align_pipe(int src, int W, const AlignArgs& a, int grid, hipStream_t st) {
+  if (src == SRC_SEQ)
+    hipLaunchKernelGGL((align_pipe_kernel<SRC_SEQ, 4>), dim3(grid), dim3(MST_WAVE * 4), 0, st, a);
+  else if (W >= 8)
+    hipLaunchKernelGGL((align_pipe_kernel<SRC_TAB, 8>), dim3(grid), dim3(MST_WAVE * 8), 0, st, a);
+  else
+    hipLaunchKernelGGL((align_pipe_kernel<SRC_TAB, 4>), dim3(grid), dim3(MST_WAVE * 4), 0, st, a);
+}
+
+int64_t align_strips(int m) { return m <= 0 ? 1 : (m + ASH - 1) / ASH; }
 
 void launch_traceback(const TraceArgs& a, hipStream_t st) {
   const int64_t n = a.last - a.first;

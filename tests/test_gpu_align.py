@@ -43,6 +43,16 @@ def test_strip_boundaries(ctx):
     _check(ctx, seqs, pi.ravel(), pj.ravel())
 
 
+def test_long_pairs(ctx):
+    # 2,000-6,000 rows: 4 to 12 strips of 512 rows
+    rng = np.random.default_rng(12)
+    lens = [2049, 3000, 4500, 6000]
+    seqs = ["".join(rng.choice(list(O.STANDARD20), L)) for L in lens]
+    pi = np.array([0, 1, 2, 3, 3, 1])
+    pj = np.array([1, 0, 3, 2, 0, 3])
+    _check(ctx, seqs, pi, pj)
+
+
 def test_config0_pairs(ctx):
     seqs = M.synth(0, 0)
     rng = np.random.default_rng(8)
