@@ -274,7 +274,7 @@ def main():
         "device_ms_plain": round(float(np.mean(devs_p)), 3) if devs_p else None,
         "roofline": roof,
         "batched_launches": launches,
-        "tunables": tunes,
+        "tunables": dict(ctx.tuning, **tunes),
         "served": None if (args.no_serve or args.batched_only) else served_bench(
             M, ctx, args.serve_jobs,
             sets=[(kv.split("=", 1)[0], float(kv.split("=", 1)[1])) for kv in args.serve_set]),

@@ -42,6 +42,9 @@ int mst_destroy(mst_ctx* ctx);
 const char* mst_last_error(const mst_ctx* ctx);
 /* Number of SIMD-resident waves the batched form launches (informational). */
 int mst_device_info(mst_ctx* ctx, int* num_cus, int64_t* hbm_bytes);
+/* The device's architecture name (gcnArchName, e.g. "gfx950:sramecc+:xnack-"),
+ * NUL-terminated within cap bytes; it keys the persisted tuning tables. */
+int mst_device_arch(mst_ctx* ctx, char* buf, int cap);
 
 /* Replaces AlignmentScore::loadDB/parse (AlignmentScore.cu:143-185) and
  * AlignmentMatch::prepare (AlignmentMatch.cu:116-131): n sequences as ASCII in

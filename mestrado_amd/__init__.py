@@ -36,6 +36,7 @@ def lib():
         L.mst_last_error.argtypes = [P]
         L.mst_last_error.restype = C.c_char_p
         L.mst_device_info.argtypes = [P, C.POINTER(C.c_int), C.POINTER(C.c_int64)]
+        L.mst_device_arch.argtypes = [P, C.c_char_p, C.c_int]
         L.mst_seqset_upload.argtypes = [P, P, P, C.c_int, C.POINTER(C.c_int)]
         L.mst_seqset_free.argtypes = [P, C.c_int]
         L.mst_seqset_codes.argtypes = [P, C.c_int, P, C.c_int64]
@@ -129,7 +130,9 @@ class Join(C.Structure):
 class Context:
     """One device context (mst_create/mst_destroy)."""
 
-    def __init__(self, gap_open=10, gap_ext=2, subst=None, device=0):
+    def __init__(self, gap_open=10, gap_ext=2, subst=None, device=0, tuned=True):
+        """tuned: apply the persisted tuning table of the device
+        (mestrado_amd/tuning/<arch>.json, written by mestrado_amd.autotune)."""
         L = lib()
         self._L = L
         h = C.c_void_p()
@@ -141,6 +144,15 @@ class Context:
         if rc != 0:
             raise MestradoError("mst_create failed (%d)" % rc)
         self.h = h
+        self.tuning = {}
+        if tuned:
+            from . import autotune
+            self.tuning = autotune.apply(self)
+
+    def arch(self):
+        buf = C.create_string_buffer(64)
+        self._ck(self._L.mst_device_arch(self.h, buf, 64), "mst_device_arch")
+        return buf.value.decode()
 
     def _ck(self, rc, what):
         if rc != 0:

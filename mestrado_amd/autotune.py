@@ -1,0 +1,150 @@
+"""Autotuner of the batched all-pairs form.
+
+Searches the tunables of include/mestrado.h (mst_tune_set: "chunk" column
+sequences per work item, "shared3" routing of rows 129..320, "wg5"
+workgroups per CU of the streamed kernel) on a workload, timing each
+candidate with the resident all-pairs pass (device time, plan and graph built
+outside the timed region), and persists the best per device architecture and
+length bucket in mestrado_amd/tuning/<arch>.json:
+
+    {"arch": "gfx950", "buckets": {"<=256": {"chunk": 16, ...,
+      "ms": 12.3, "workload": "..."}, ...}}
+
+A bucket is named by the median sequence length of the set it was tuned on
+(<=256, <=1024, >1024). Context(tuned=True) applies the table's entry for
+the bucket `default` names; apply_for(ctx, lengths) picks the bucket of a set.
+
+    python -m mestrado_amd.autotune --config 2        # tune on the 4,000-protein set
+"""
+import argparse
+import itertools
+import json
+import os
+import sys
+
+import numpy as np
+
+TUNING_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning")
+KEYS = ("chunk", "shared3", "wg5")
+BUCKETS = ((256, "<=256"), (1024, "<=1024"), (None, ">1024"))
+
+
+def arch_key(arch):
+    """gfx950:sramecc+:xnack- -> gfx950"""
+    return arch.split(":")[0] if arch else "unknown"
+
+
+def bucket_of(lengths):
+    med = float(np.median(np.asarray(lengths))) if len(lengths) else 0.0
+    for hi, name in BUCKETS:
+        if hi is None or med <= hi:
+            return name
+    return BUCKETS[-1][1]
+
+
+def table_path(arch):
+    return os.path.join(TUNING_DIR, arch_key(arch) + ".json")
+
+
+def load(arch):
+    p = table_path(arch)
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f)
+
+
+def _set(ctx, params):
+    for k in KEYS:
+        if k in params:
+            ctx.tune_set(k, int(params[k]))
+
+
+def apply(ctx, bucket=None):
+    """Applies the persisted entry (the table's default bucket, or `bucket`);
+    returns the values applied ({} without a table)."""
+    t = load(ctx.arch())
+    if not t:
+        return {}
+    name = bucket or t.get("default")
+    entry = t.get("buckets", {}).get(name)
+    if not entry:
+        return {}
+    params = {k: entry[k] for k in KEYS if k in entry}
+    _set(ctx, params)
+    return dict(params, bucket=name)
+
+
+def apply_for(ctx, lengths):
+    return apply(ctx, bucket_of(lengths))
+
+
+def candidates():
+    for shared3, chunk, wg5 in itertools.product((0, 1), (8, 16, 32, 64), (0,)):
+        yield {"shared3": shared3, "chunk": chunk, "wg5": wg5}
+
+
+def tune(ctx, seqs, reps=2, log=None):
+    """Times every candidate on `seqs`; returns (best params, results)."""
+    import mestrado_amd as M
+    st = ctx.upload(seqs)
+    results = []
+    try:
+        for params in candidates():
+            _set(ctx, params)
+            st.all_pairs_resident(M.FORM_BATCHED)  # plan, graph and a first pass
+            ms = min(st.all_pairs_resident(M.FORM_BATCHED) for _ in range(reps))
+            results.append(dict(params, ms=round(float(ms), 3)))
+            if log:
+                log("tune %s: %.3f ms" % (params, ms))
+    finally:
+        st.free()
+    best = min(results, key=lambda r: r["ms"])
+    return best, results
+
+
+def save(arch, bucket, best, workload, default=False):
+    os.makedirs(TUNING_DIR, exist_ok=True)
+    t = load(arch) or {"arch": arch_key(arch), "buckets": {}}
+    entry = {k: best[k] for k in KEYS}
+    entry["ms"] = best["ms"]
+    entry["workload"] = workload
+    t["buckets"][bucket] = entry
+    if default or "default" not in t:
+        t["default"] = bucket
+    with open(table_path(arch), "w") as f:
+        json.dump(t, f, indent=1, sort_keys=True)
+        f.write("\n")
+    return table_path(arch)
+
+
+def main():
+    import mestrado_amd as M
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--reps", type=int, default=2)
+    ap.add_argument("--no-save", action="store_true")
+    ap.add_argument("--default", action="store_true", help="make this bucket the default")
+    ap.add_argument("--dir", default=None, help="directory of the tables (default: the package's)")
+    args = ap.parse_args()
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    global TUNING_DIR
+    if args.dir:
+        TUNING_DIR = args.dir
+    ctx = M.Context(tuned=False)
+    seqs = M.synth(args.config, args.seed)
+    lengths = [len(s) for s in seqs]
+    bucket = bucket_of(lengths)
+    best, results = tune(ctx, seqs, args.reps, log)
+    workload = "synthetic configuration %d, seed %d, %d sequences" % (args.config, args.seed, len(seqs))
+    out = {"arch": ctx.arch(), "bucket": bucket, "best": best, "results": results}
+    if not args.no_save:
+        out["saved"] = save(ctx.arch(), bucket, best, workload, args.default)
+    print(json.dumps(out))
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

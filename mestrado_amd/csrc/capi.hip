@@ -106,6 +106,7 @@ extern "C" int mst_create(int gap_open, int gap_ext, const int8_t* subst, int de
   hipDeviceProp_t prop;
   CK(hipGetDeviceProperties(&prop, device));
   ctx->num_cus = prop.multiProcessorCount;
+  strncpy(ctx->arch, prop.gcnArchName, sizeof ctx->arch - 1);
   ctx->hbm_bytes = (int64_t)prop.totalGlobalMem;
   CK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   CK(hipEventCreate(&ctx->ev0));
@@ -151,6 +152,13 @@ extern "C" int mst_device_info(mst_ctx* ctx, int* num_cus, int64_t* hbm) {
   if (!ctx) return MST_E_ARG;
   if (num_cus) *num_cus = ctx->num_cus;
   if (hbm) *hbm = ctx->hbm_bytes;
+  return MST_OK;
+}
+
+extern "C" int mst_device_arch(mst_ctx* ctx, char* buf, int cap) {
+  if (!ctx || !buf || cap < 1) return MST_E_ARG;
+  strncpy(buf, ctx->arch, cap - 1);
+  buf[cap - 1] = 0;
   return MST_OK;
 }
 

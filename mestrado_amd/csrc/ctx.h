@@ -48,6 +48,7 @@ struct PairPlan {
 struct mst_ctx {
   int device = 0;
   int num_cus = 0;
+  char arch[64] = {0};  // gcnArchName of the device, e.g. gfx950:sramecc+:xnack-
   int64_t hbm_bytes = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
