@@ -94,17 +94,35 @@ int mst_last_launch_stats(mst_ctx* ctx, int* rows_per_lane, int64_t* pairs, int6
  * batched plans. Unknown keys or values out of range return MST_E_ARG. */
 int mst_tune_set(mst_ctx* ctx, const char* key, int value);
 int mst_tune_get(mst_ctx* ctx, const char* key, int* value);
+/* One join of the guide tree (TreeNode::join, TreeNode.cpp:30-58). */
+typedef struct {
+  int node;             /* id of the new node (n0, n0+1, ...) */
+  int left, right;      /* joined node ids, left < right */
+  float d;              /* D(left, right) */
+  double Ti, Tj;        /* row sums T of left and right at the join */
+  int n;                /* active taxa before the join */
+  double q;             /* the minimal q */
+  double len_left;      /* branch lengths as TreeNode::join computes them */
+  double len_right;
+} mst_join;
+
 /* Served form (no counterpart in the reference, whose runs are one job each):
- * a request loop with continuous batching over one context. Jobs are sets of
- * sequences whose all-pairs score matrix is wanted; mst_server_step retires
- * finished rounds, admits waiting jobs in arrival order while their device
- * footprint (N*N*4 B of scores plus twice the residues) fits the budget, and
- * launches the next round: a quantum of work items of the admitted jobs,
+ * a request loop with continuous batching over one context. A job is a set of
+ * sequences and what is wanted of it: the all-pairs score matrix, plus the NJ
+ * tree of that matrix (what mst_nj_scores gives), plus the progressive
+ * alignment along that tree (what mst_msa gives), i.e. wnj's whole run.
+ * mst_server_step retires finished device work, admits waiting jobs in
+ * arrival order while their device footprint fits the budget (scores N*N*4 B
+ * and twice the residues; with a tree, the fp32 matrix N*N*4 B and the NJ
+ * state; with an alignment, the leaf rows and 4*N*(maxlen+64) B of group
+ * rows), and launches: rounds of score work items of the admitted jobs,
  * earliest deadline first, packed into shared launches of the batched form's
- * kernels. Results stay on the device until mst_server_result copies them
- * out and frees the job (finished jobs hold their memory until then). A job
- * larger than the budget is rejected at submission. Jobs are independent: a
- * job's matrix is identical alone and in any mix. Not thread-safe. */
+ * kernels; one NJ launch for the jobs whose scores are done (a workgroup per
+ * job); one merge batch for the ready profile merges of all jobs in their
+ * alignment stage. Results stay on the device until copied out; finished
+ * jobs hold their memory until mst_server_result frees them. A job larger
+ * than the budget is rejected at submission. Jobs are independent: a job's
+ * results are identical alone and in any mix. Not thread-safe. */
 typedef struct mst_server mst_server;
 enum { MST_JOB_WAITING = 0, MST_JOB_RUNNING = 1, MST_JOB_DONE = 2, MST_JOB_CANCELLED = 3,
        MST_JOB_REJECTED = 4 };
@@ -113,6 +131,11 @@ int mst_server_destroy(mst_server* srv);
 /* The job's sequences are ascii[offsets[i] .. offsets[i+1]) (copied). */
 int mst_server_submit(mst_server* srv, const char* ascii, const int64_t* offsets, int n,
                       int64_t* job);
+/* What a job computes: MST_SERVE_SCORES (as mst_server_submit), _TREE (and
+ * the NJ tree), _MSA (and the alignment along it). */
+enum { MST_SERVE_SCORES = 1, MST_SERVE_TREE = 3, MST_SERVE_MSA = 7 };
+int mst_server_submit_job(mst_server* srv, const char* ascii, const int64_t* offsets, int n,
+                          int what, int64_t* job);
 /* One scheduling pass (never blocks on the device); *ncompleted (may be NULL)
  * receives the number of jobs that finished in it. */
 int mst_server_step(mst_server* srv, int* ncompleted);
@@ -121,8 +144,15 @@ int mst_server_drain(mst_server* srv);
 /* State, size and host steady-clock times (seconds) of a job. */
 int mst_server_job(mst_server* srv, int64_t job, int* state, int* n, double* t_submit,
                    double* t_admit, double* t_done);
-/* Copies a done job's n x n scores out (may be NULL) and forgets the job. */
+/* Copies a done job's n x n scores out (may be NULL) and forgets the job:
+ * take its tree and alignment first. */
 int mst_server_result(mst_server* srv, int64_t job, int32_t* scores_nxn);
+/* The n-1 joins of a done job with a tree (as mst_nj_scores fills them). */
+int mst_server_tree(mst_server* srv, int64_t job, mst_join* joins);
+/* A done job's alignment: *msa_len columns; rows (may be NULL) receives its
+ * n gapped rows in input order, row r at rows + r*stride, NUL-terminated
+ * (stride >= *msa_len + 1), as mst_msa_rows. */
+int mst_server_msa(mst_server* srv, int64_t job, char* rows, int64_t stride, int* msa_len);
 int mst_server_cancel(mst_server* srv, int64_t job);
 /* Up to cap ids of jobs that finished since the previous call, oldest first;
  * *count receives how many were written. */
@@ -133,7 +163,8 @@ int mst_server_stats(mst_server* srv, int* waiting, int* running, int* inflight,
  * "chunk" (column sequences per item, default 8), "slack" (a job's deadline
  * is its submission plus slack times its estimated run time, default 2;
  * jobs go earliest deadline first), "rate" (initial throughput estimate in
- * cells/s, for those estimates). */
+ * cells/s, for those estimates), "nj_ratio" (NJ rebuild ratio, default 0.5),
+ * "merge_bytes" (scratch of one merge batch, default 1 GiB). */
 int mst_server_set(mst_server* srv, const char* key, double value);
 /* Measurement helper (no counterpart in the reference): integer VALU rate of
  * the context's device in T ops/s, int32 (packed = 0) or packed int16
@@ -153,18 +184,6 @@ int mst_valu_peak(mst_ctx* ctx, int packed, double* tops);
 int mst_align_pairs(mst_ctx* ctx, int set_id, const int32_t* pi, const int32_t* pj,
                     int64_t npairs, int32_t* scores, char* gapped_x, char* gapped_y,
                     int64_t stride);
-
-/* One join of the guide tree (TreeNode::join, TreeNode.cpp:30-58). */
-typedef struct {
-  int node;             /* id of the new node (n0, n0+1, ...) */
-  int left, right;      /* joined node ids, left < right */
-  float d;              /* D(left, right) */
-  double Ti, Tj;        /* row sums T of left and right at the join */
-  int n;                /* active taxa before the join */
-  double q;             /* the minimal q */
-  double len_left;      /* branch lengths as TreeNode::join computes them */
-  double len_right;
-} mst_join;
 
 /* Replaces SpeciesBlock::buildTree (SpeciesBlock.cpp:844-1343) on one device:
  * neighbour joining of the n x n matrix D (row-major fp32, symmetric, the

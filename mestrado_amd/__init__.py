@@ -50,6 +50,9 @@ def lib():
         L.mst_server_create.argtypes = [P, C.c_int64, C.POINTER(P)]
         L.mst_server_destroy.argtypes = [P]
         L.mst_server_submit.argtypes = [P, P, P, C.c_int, C.POINTER(C.c_int64)]
+        L.mst_server_submit_job.argtypes = [P, P, P, C.c_int, C.c_int, C.POINTER(C.c_int64)]
+        L.mst_server_tree.argtypes = [P, C.c_int64, P]
+        L.mst_server_msa.argtypes = [P, C.c_int64, P, C.c_int64, C.POINTER(C.c_int)]
         L.mst_server_step.argtypes = [P, C.POINTER(C.c_int)]
         L.mst_server_drain.argtypes = [P]
         L.mst_server_job.argtypes = [P, C.c_int64, C.POINTER(C.c_int), C.POINTER(C.c_int),
@@ -300,11 +303,13 @@ class SeqSet:
 
 
 JOB_WAITING, JOB_RUNNING, JOB_DONE, JOB_CANCELLED, JOB_REJECTED = range(5)
+SERVE_SCORES, SERVE_TREE, SERVE_MSA = 1, 3, 7
 
 
 class Server:
     """Served form (mst_server_* in include/mestrado.h): continuous batching of
-    all-pairs jobs on one context."""
+    jobs (all-pairs scores, and their NJ tree and alignment when asked) on one
+    context."""
 
     def __init__(self, ctx, budget_bytes=8 << 30):
         self.ctx = ctx
@@ -324,12 +329,32 @@ class Server:
     def __exit__(self, *a):
         self.close()
 
-    def submit(self, seqs):
+    def submit(self, seqs, what=SERVE_SCORES):
+        """what: SERVE_SCORES, SERVE_TREE (and the NJ tree) or SERVE_MSA (and
+        the alignment along it)."""
         buf, off = pack(seqs)
         j = C.c_int64(0)
-        self.ctx._ck(self._L.mst_server_submit(self.h, _p(buf), _p(off), len(seqs), C.byref(j)),
-                     "mst_server_submit")
+        self.ctx._ck(self._L.mst_server_submit_job(self.h, _p(buf), _p(off), len(seqs), what,
+                                                   C.byref(j)), "mst_server_submit_job")
         return j.value
+
+    def tree(self, jid):
+        """The n-1 joins of a done job submitted with a tree."""
+        n = self.job(jid)["n"]
+        joins = (Join * max(1, n - 1))()
+        self.ctx._ck(self._L.mst_server_tree(self.h, jid, joins), "mst_server_tree")
+        return list(joins)[: n - 1]
+
+    def msa(self, jid):
+        """Gapped rows (input order) of a done job submitted with an alignment."""
+        n = self.job(jid)["n"]
+        L = C.c_int(0)
+        self.ctx._ck(self._L.mst_server_msa(self.h, jid, None, 0, C.byref(L)), "mst_server_msa")
+        stride = L.value + 1
+        buf = C.create_string_buffer(n * stride)
+        self.ctx._ck(self._L.mst_server_msa(self.h, jid, buf, stride, C.byref(L)), "mst_server_msa")
+        raw = buf.raw
+        return [raw[r * stride: r * stride + L.value].decode() for r in range(n)]
 
     def step(self):
         n = C.c_int(0)

@@ -164,8 +164,8 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
     const bool packed = kind != 0;
     // rows (ya, yb) against columns s[first..last]; yb < 0: ya alone. The
     // streamed kernels name rows by position (pa, pb), the others by id.
-    auto add_cols = [&](int first, int last, int ya_, int yb_) {
-      const int pa = 2 * k, pb = yb_ >= 0 ? 2 * k + 1 : -1;
+    auto add_cols = [&](int first, int last, int pa, int pb) {
+      const int ya_ = order[pa], yb_ = pb >= 0 ? order[pb] : -1;
       std::vector<int> elig;
       for (int t = first; t <= last; ++t) {
         const int x = order[t];
@@ -228,10 +228,18 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
       }
     };
     if (single) {
-      add_cols(2 * k, N - 1, ya, -1);
+      add_cols(2 * k, N - 1, 2 * k, -1);
+    } else if (len(ya) == 0) {
+      // an empty row sequence has no last row to emit its scores from: its
+      // pairs (-ge * n each) take the int32 kernel, yb runs alone
+      for (int t = 2 * k; t < N; ++t) {
+        fb_i.push_back(order[t]);
+        fb_j.push_back(ya);
+      }
+      add_cols(2 * k + 1, N - 1, 2 * k + 1, -1);
     } else {
-      add_cols(2 * k, 2 * k, ya, -1);          // (ya, ya)
-      add_cols(2 * k + 1, N - 1, ya, yb);      // every x at or after yb, both rows
+      add_cols(2 * k, 2 * k, 2 * k, -1);          // (ya, ya)
+      add_cols(2 * k + 1, N - 1, 2 * k, 2 * k + 1);  // every x at or after yb, both rows
     }
   }
   auto by_class = [](auto& v) {

@@ -14,11 +14,18 @@ using namespace mst;
     if (e_ != hipSuccess) return ctx->fail(MST_E_HIP, #call, hipGetErrorString(e_)); \
   } while (0)
 
-namespace {
-
 // TreeNode::join (TreeNode.cpp:30-58): lengths in fp64, a negative side is
 // clamped to zero and its excess moved to the other side.
-void branch_lengths(mst_join& j) {
+void mst::nj_to_join(const NjJoin& in, mst_join* out) {
+  mst_join& j = *out;
+  j.node = in.node;
+  j.left = in.left;
+  j.right = in.right;
+  j.d = in.d;
+  j.Ti = in.Ti;
+  j.Tj = in.Tj;
+  j.n = in.n;
+  j.q = in.q;
   double L, R, half = 0.5;
   double minD = (double)j.d;
   if (j.n == 2) {
@@ -37,13 +44,6 @@ void branch_lengths(mst_join& j) {
   j.len_left = L;
   j.len_right = R;
 }
-
-__global__ void scores_to_float(const int32_t* __restrict__ s, float* __restrict__ d, int64_t n) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) d[i] = (float)s[i];
-}
-
-}  // namespace
 
 // Runs NJ on the device matrix ctx->d_nj (n x n, filled by the caller).
 static int run_nj(mst_ctx* ctx, int n, float ratio, mst_join* joins) {
@@ -73,18 +73,7 @@ static int run_nj(mst_ctx* ctx, int n, float ratio, mst_join* joins) {
     CK(hipMemcpyAsync(hj.data(), ctx->d_nj_joins, sizeof(NjJoin) * (n - 1), hipMemcpyDeviceToHost,
                       ctx->stream));
   CK(hipStreamSynchronize(ctx->stream));
-  for (int k = 0; k + 1 < n; ++k) {
-    mst_join& j = joins[k];
-    j.node = hj[k].node;
-    j.left = hj[k].left;
-    j.right = hj[k].right;
-    j.d = hj[k].d;
-    j.Ti = hj[k].Ti;
-    j.Tj = hj[k].Tj;
-    j.n = hj[k].n;
-    j.q = hj[k].q;
-    branch_lengths(j);
-  }
+  for (int k = 0; k + 1 < n; ++k) nj_to_join(hj[k], &joins[k]);
   return MST_OK;
 }
 
@@ -109,8 +98,7 @@ extern "C" int mst_nj_scores(mst_ctx* ctx, int id, float ratio, mst_join* joins)
   int rc = ctx->ensure((void**)&ctx->d_nj, &ctx->nj_cap, sizeof(float) * (size_t)n * n);
   if (rc) return rc;
   int64_t nn = (int64_t)n * n;
-  hipLaunchKernelGGL(scores_to_float, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, ctx->stream,
-                     ctx->d_mat, ctx->d_nj, nn);
+  launch_scores_to_float(ctx->d_mat, ctx->d_nj, nn, ctx->stream);
   CK(hipGetLastError());
   return run_nj(ctx, n, ratio, joins);
 }

@@ -1,21 +1,29 @@
-// Served form of the all-pairs scores: a request loop with continuous
-// batching. Jobs (sets of sequences) are submitted at any time; each call of
-// mst_server_step retires finished rounds, admits waiting jobs whose device
-// footprint fits the budget (FIFO), and launches the next round: a quantum of
-// work items drawn from the admitted jobs, earliest deadline first, packed
-// into shared launches of the streamed packed kernels (nw_score5.hip,
-// nw_score6.hip) whatever job they belong to.
+// Served form: a request loop with continuous batching. Jobs (sets of
+// sequences, and whether their NJ tree and progressive alignment are wanted
+// besides the all-pairs scores) are submitted at any time; each call of
+// mst_server_step retires finished device work, admits waiting jobs whose
+// device footprint fits the budget (FIFO), and launches:
+//  * score rounds: a quantum of work items drawn from the admitted jobs,
+//    earliest deadline first, packed into shared launches of the streamed
+//    packed kernels (nw_score5.hip, nw_score6.hip) whatever job they belong to;
+//  * one NJ launch (nj.hip, a workgroup per job) for the jobs whose scores
+//    are complete;
+//  * one merge batch (msa.hip) of the profile merges whose children exist,
+//    over every job in its alignment stage, earliest deadline first.
 //
 // Device state: one arena of the budget's size holds, per admitted job, its
 // N x N int32 score matrix, its residue codes in length order and the ASCII
-// they are encoded from; a position table (soff: code offset, order: index in
-// the job) names every admitted sequence; a job table gives each job's matrix
-// offset and leading dimension, so a kernel writes a pair's score into its own
-// job's matrix. Rounds are enqueued on the context's stream, at most three in
-// flight. Inside a round the launches of different kernel classes run side by
-// side on four side streams (default priority) after the round's copies on
-// the context's stream; a round is over when every stream it used has passed
-// its end event, and the next round's copies do not wait for it.
+// they are encoded from, and (tree, alignment) the fp32 NJ matrix, NJ state,
+// leaf rows and a region for group rows; a position table (soff: code
+// offset, order: index in the job) names every admitted sequence; a job
+// table gives each job's matrix offset and leading dimension, so a kernel
+// writes a pair's score into its own job's matrix. Score rounds are enqueued
+// on the context's stream, at most three in flight; inside a round the
+// launches of different kernel classes run side by side on four side streams
+// (default priority) after the round's copies on the context's stream; a
+// round is over when every stream it used has passed its end event, and the
+// next round's copies do not wait for it. NJ and merge batches have a stream
+// each, one batch of each in flight.
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -27,8 +35,11 @@
 #include "../../include/mestrado.h"
 #include "ctx.h"
 #include "encode.h"
+#include "merge.h"
+#include "nj.h"
 #include "nw_score.h"
 #include "nw_score5.h"
+#include "pool.h"
 
 using namespace mst;
 
@@ -66,65 +77,37 @@ double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// First-fit allocator over [0, cap) in units of `align`.
-struct Pool {
-  int64_t cap = 0, align = 1, used = 0;
-  std::map<int64_t, int64_t> free_;  // offset -> size
-  void init(int64_t c, int64_t a) {
-    cap = c;
-    align = a;
-    used = 0;
-    free_.clear();
-    if (c > 0) free_[0] = c;
-  }
-  int64_t round(int64_t n) const { return (n + align - 1) / align * align; }
-  int64_t alloc(int64_t n) {
-    n = round(std::max<int64_t>(n, 1));
-    for (auto it = free_.begin(); it != free_.end(); ++it) {
-      if (it->second >= n) {
-        const int64_t off = it->first, rest = it->second - n;
-        free_.erase(it);
-        if (rest > 0) free_[off + n] = rest;
-        used += n;
-        return off;
-      }
-    }
-    return -1;
-  }
-  void release(int64_t off, int64_t n) {
-    n = round(std::max<int64_t>(n, 1));
-    used -= n;
-    auto it = free_.emplace(off, n).first;
-    auto nx = std::next(it);
-    if (nx != free_.end() && it->first + it->second == nx->first) {
-      it->second += nx->second;
-      free_.erase(nx);
-    }
-    if (it != free_.begin()) {
-      auto pv = std::prev(it);
-      if (pv->first + pv->second == it->first) {
-        pv->second += it->second;
-        free_.erase(it);
-      }
-    }
-  }
-};
-
 // Classes of the streamed kernels: 0 = score6, 1 + (W-1)*3 + (K-3) = score5<K, W>.
 constexpr int kClasses = 13;
 int class_of(int kind, int K, int W) { return kind == 6 ? 0 : 1 + (W - 1) * 3 + (K - 3); }
 
+// Stages of a running job.
+enum { kStageScores = 0, kStageNj = 1, kStageMsa = 2, kStageDone = 3 };
+
+// A group of the progressive alignment: k rows of len columns at arena
+// offset off (a block of cap bytes, in the job's region or the arena).
+struct Group {
+  int64_t off = -1, cap = 0;
+  int k = 0, len = 0;
+  bool ready = false, leaf = false, in_arena = false;
+};
+
 struct Job {
   int64_t id = 0;
   int n = 0;
+  int what = MST_SERVE_SCORES;
   int state = MST_JOB_WAITING;
+  int stage = kStageScores;
   std::vector<char> ascii;        // input letters
   std::vector<int64_t> off;       // input offsets (n + 1)
   double t_submit = 0, t_admit = 0, t_done = 0, deadline = 0;
   // device placement
-  int64_t arena_off = -1, arena_bytes = 0;  // one block: scores, codes, ascii, offsets
+  int64_t arena_off = -1, arena_bytes = 0;  // one block: scores, codes, ascii, offsets, NJ, MSA
+  int64_t o_codes = 0, o_D = 0, o_T = 0, o_act = 0, o_joins = 0, o_leaf = 0, o_msa = 0, msa_cap = 0;
   int64_t pos0 = -1;                        // positions pos0 .. pos0 + n (n + 1)
   int slot = -1;
+  std::vector<int64_t> roff;                // code offsets in length order (n + 1)
+  std::vector<int32_t> where;               // input index -> length-order position
   // plan: streamed items (positions) and int32 pairs (positions)
   std::vector<Item3> items;
   std::vector<int> item_cls;
@@ -132,8 +115,15 @@ struct Job {
   std::vector<int2> fb;
   size_t next_item = 0, next_fb = 0;
   int64_t cells_total = 0, cells_left = 0;
-  int rounds_ref = 0;  // in-flight rounds holding work of this job
+  int refs = 0;  // in-flight rounds and batches holding work of this job
   bool cancel = false;
+  // tree and alignment
+  std::vector<mst_join> tree;
+  std::vector<Group> groups;     // 2n - 1 nodes
+  std::vector<char> merge_state; // per join: 0 waiting, 1 in flight, 2 done
+  int merges_done = 0;
+  bool leaves_done = false;
+  Pool msa_pool;                 // over [o_msa, o_msa + msa_cap)
 };
 
 struct RoundBuf {
@@ -196,15 +186,81 @@ struct mst_server {
   int chunk = 8;
   int64_t done_count = 0, cells_done = 0;
   std::vector<int64_t> completed;  // finished since the last mst_server_completed
+  // NJ: jobs whose scores are done, the batch in flight
+  float nj_ratio = 0.5f;
+  hipStream_t nj_st = nullptr;
+  hipEvent_t nj_ev = nullptr;
+  std::deque<int64_t> nj_queue;
+  std::vector<int64_t> nj_batch;
+  std::vector<int64_t> nj_stage_off;  // joins of nj_batch[i] at h_nj + nj_stage_off[i]
+  NjArgs* d_nj_args = nullptr;
+  NjArgs* h_nj_args = nullptr;
+  NjJoin* h_nj = nullptr;
+  int64_t nj_args_cap = 0, nj_args_host_cap = 0, nj_cap = 0;
+  // merges: the batch in flight
+  double merge_bytes = (double)(1 << 30);
+  hipStream_t msa_st = nullptr;
+  hipEvent_t msa_ev = nullptr;
+  bool msa_busy = false;
+  mst::MergeEngine eng;
+  std::vector<std::pair<int64_t, int>> msa_batch;  // (job, join) of each merge
+  std::vector<int64_t> msa_leaf_jobs;              // jobs whose leaves the batch writes
 };
 
 namespace {
 
-int64_t job_footprint(const Job& J) {
-  const int64_t L = J.off[J.n] - J.off[0];
-  auto r = [](int64_t x) { return (x + 255) / 256 * 256; };
-  return r((int64_t)J.n * J.n * 4) + r(L) + r(L) + r((int64_t)(J.n + 1) * 8);
+int64_t r256(int64_t x) { return (x + 255) / 256 * 256; }
+
+// Group rows of a job's alignment held in its own region: room for the
+// groups alive at once (each sequence is in one, at most ~maxlen columns wide
+// with gaps) and the merges of a batch being written.
+int64_t msa_region(const Job& J) {
+  int64_t maxlen = 0;
+  for (int i = 0; i < J.n; ++i) maxlen = std::max<int64_t>(maxlen, J.off[i + 1] - J.off[i]);
+  return r256(4 * (int64_t)J.n * (maxlen + 64));
 }
+
+// Bytes of a job's block and the offsets of its parts (relative to the block).
+struct Layout {
+  int64_t scores, codes, ascii, roff, D, T, act, joins, leaf, msa, msa_cap, total;
+};
+
+Layout job_layout(const Job& J) {
+  const int64_t N = J.n, L = J.off[J.n] - J.off[0];
+  Layout l;
+  int64_t o = 0;
+  l.scores = o;
+  o += r256(N * N * 4);
+  l.codes = o;
+  o += r256(L);
+  l.ascii = o;
+  o += r256(L);
+  l.roff = o;
+  o += r256((N + 1) * 8);
+  l.D = l.T = l.act = l.joins = l.leaf = l.msa = o;
+  l.msa_cap = 0;
+  if ((J.what & MST_SERVE_TREE) == MST_SERVE_TREE) {
+    l.D = o;
+    o += r256(N * N * 4);
+    l.T = o;
+    o += r256(N * 8);
+    l.act = o;
+    o += r256(N * 16);
+    l.joins = o;
+    o += r256(N * (int64_t)sizeof(NjJoin));
+  }
+  if ((J.what & MST_SERVE_MSA) == MST_SERVE_MSA) {
+    l.leaf = o;
+    o += r256(std::max<int64_t>(L, 1));
+    l.msa = o;
+    l.msa_cap = msa_region(J);
+    o += l.msa_cap;
+  }
+  l.total = o;
+  return l;
+}
+
+int64_t job_footprint(const Job& J) { return job_layout(J).total; }
 
 // Streamed plan of one job whose sequences sit at positions pos0 .. pos0+n-1
 // in length order (lens ascending): the row pairs (2k, 2k+1) against every
@@ -266,6 +322,11 @@ void plan_job(const mst_ctx* ctx, Job& J, const std::vector<int>& lens, int chun
     const int64_t R = kind == 6 ? (int64_t)(m + 255) / 256 * 256 : (int64_t)W * 64 * K;
     if (single) {
       add(kind, K, W, R, pa, -1, pa, N - 1);
+    } else if (lens[pa] == 0) {
+      // an empty row sequence has no last row to emit its scores from: its
+      // pairs take the int32 kernel, pb runs alone
+      for (int t = pa; t < N; ++t) J.fb.push_back(make_int2(p0 + t, p0 + pa));
+      add(kind, K, W, R, pb, -1, pb, N - 1);
     } else {
       add(kind, K, W, R, pa, -1, pa, pa);
       add(kind, K, W, R, pa, pb, pb, N - 1);
@@ -278,6 +339,10 @@ void plan_job(const mst_ctx* ctx, Job& J, const std::vector<int>& lens, int chun
 }
 
 void release_job(mst_server* S, Job& J) {
+  for (Group& G : J.groups)
+    if (G.in_arena && G.off >= 0) S->arena.release(G.off, G.cap);
+  J.groups.clear();
+  J.groups.shrink_to_fit();
   if (J.arena_off >= 0) S->arena.release(J.arena_off, J.arena_bytes);
   if (J.pos0 >= 0) S->pos.release(J.pos0, J.n + 1);
   if (J.slot >= 0) S->slots.release(J.slot, 1);
@@ -321,9 +386,18 @@ int admit(mst_server* S, Job& J) {
   auto len = [&](int i) { return (int)(J.off[i + 1] - J.off[i]); };
   std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return len(x) < len(y); });
   const int64_t L = J.off[N] - J.off[0];
-  auto r = [](int64_t x) { return (x + 255) / 256 * 256; };
-  const int64_t o_scores = a, o_codes = a + r((int64_t)N * N * 4), o_ascii = o_codes + r(L),
-                o_roff = o_ascii + r(L);
+  const Layout lay = job_layout(J);
+  const int64_t o_scores = a + lay.scores, o_codes = a + lay.codes, o_ascii = a + lay.ascii,
+                o_roff = a + lay.roff;
+  J.o_codes = o_codes;
+  J.o_D = a + lay.D;
+  J.o_T = a + lay.T;
+  J.o_act = a + lay.act;
+  J.o_joins = a + lay.joins;
+  J.o_leaf = a + lay.leaf;
+  J.o_msa = a + lay.msa;
+  J.msa_cap = lay.msa_cap;
+  J.msa_pool.init(lay.msa_cap, 256);
   std::vector<char> asc(std::max<int64_t>(L, 1));
   std::vector<int64_t> roff(N + 1, 0), soff(N + 1, 0);
   std::vector<int32_t> order(N + 1, 0);
@@ -348,7 +422,11 @@ int admit(mst_server* S, Job& J) {
                 (int8_t*)(S->d_arena + o_codes), ctx->stream);
   CK(hipGetLastError());
   plan_job(ctx, J, lens, S->chunk);
+  J.roff.assign(roff.begin(), roff.end());
+  J.where.assign(N, 0);
+  for (int i = 0; i < N; ++i) J.where[order[i]] = i;
   J.state = MST_JOB_RUNNING;
+  J.stage = kStageScores;
   J.t_admit = now_s();
   J.deadline = J.t_submit + S->slack * ((double)J.cells_total / S->rate + 0.0005);
   J.ascii.clear();
@@ -436,7 +514,7 @@ int launch_round(mst_server* S, int* launched) {
       fb_job.push_back(J.slot);
       cells += 1;  // the int32 kernel's share is small; it is timed with the round
     }
-    J.rounds_ref += 1;
+    J.refs += 1;
     R.jobs.push_back(id);
     if (J.next_item == J.items.size() && J.next_fb == J.fb.size()) R.finishing.push_back(id);
   }
@@ -551,6 +629,64 @@ int launch_round(mst_server* S, int* launched) {
   return MST_OK;
 }
 
+void job_done(mst_server* S, Job& J, double t, int* ndone) {
+  J.state = MST_JOB_DONE;
+  J.stage = kStageDone;
+  J.t_done = t;
+  S->done_count += 1;
+  S->cells_done += J.cells_total;
+  S->completed.push_back(J.id);
+  if (ndone) *ndone += 1;
+}
+
+// Alignment stage: leaves (input order) in the job's leaf block, no group
+// rows yet; the leaves are written by the next merge batch.
+void start_msa(Job& J) {
+  const int N = J.n;
+  J.stage = kStageMsa;
+  J.groups.assign(2 * (size_t)N - 1, Group());
+  for (int i = 0; i < N; ++i) {
+    Group& G = J.groups[i];
+    G.off = J.o_leaf + (J.off[i] - J.off[0]);
+    G.k = 1;
+    G.len = (int)(J.off[i + 1] - J.off[i]);
+    G.leaf = true;
+  }
+  J.merge_state.assign(N > 1 ? N - 1 : 0, 0);
+  J.merges_done = 0;
+  J.leaves_done = false;
+}
+
+// The job's score rounds are over: on to NJ, the alignment or done.
+void scores_done(mst_server* S, Job& J, double t, int* ndone) {
+  if ((J.what & MST_SERVE_TREE) == MST_SERVE_TREE && J.n >= 2) {
+    J.stage = kStageNj;
+    S->nj_queue.push_back(J.id);
+  } else if ((J.what & MST_SERVE_MSA) == MST_SERVE_MSA) {
+    start_msa(J);  // one sequence: its leaf is the alignment
+  } else {
+    job_done(S, J, t, ndone);
+  }
+}
+
+// Drops finished or cancelled jobs from the running list; a cancelled job is
+// released once no round or batch in flight holds it.
+void prune_running(mst_server* S) {
+  std::vector<int64_t> keep;
+  for (int64_t id : S->running) {
+    Job& J = S->jobs[id];
+    if (J.state == MST_JOB_RUNNING && !J.cancel) {
+      keep.push_back(id);
+    } else if (J.cancel && J.refs == 0) {
+      release_job(S, J);
+      J.state = MST_JOB_CANCELLED;
+    } else if (J.cancel) {
+      keep.push_back(id);
+    }
+  }
+  S->running.swap(keep);
+}
+
 // Retires the oldest in-flight round if it has finished (or wait: block).
 int retire(mst_server* S, bool wait, int* ndone) {
   mst_ctx* ctx = S->ctx;
@@ -574,34 +710,244 @@ int retire(mst_server* S, bool wait, int* ndone) {
   S->t_retire = t;
   for (int64_t id : R.jobs) {
     Job& J = S->jobs[id];
-    J.rounds_ref -= 1;
+    J.refs -= 1;
   }
   for (int64_t id : R.finishing) {
     Job& J = S->jobs[id];
-    if (J.state == MST_JOB_RUNNING && !J.cancel) {
-      J.state = MST_JOB_DONE;
-      J.t_done = t;
-      S->done_count += 1;
-      S->cells_done += J.cells_total;
-      S->completed.push_back(id);
-      if (ndone) *ndone += 1;
-    }
+    if (J.state == MST_JOB_RUNNING && !J.cancel) scores_done(S, J, t, ndone);
   }
   S->inflight.pop_front();
-  // drop finished or cancelled jobs from the running list
-  std::vector<int64_t> keep;
+  prune_running(S);
+  return MST_OK;
+}
+
+template <typename T>
+int grow_buf(mst_ctx* ctx, T** p, int64_t* cap, int64_t count, bool pinned) {
+  if (count <= *cap) return MST_OK;
+  const int64_t want = std::max<int64_t>(count, *cap * 2);
+  if (*p) CK(pinned ? hipHostFree(*p) : hipFree(*p));
+  *p = nullptr;
+  *cap = 0;
+  if (pinned)
+    CK(hipHostMalloc((void**)p, sizeof(T) * want, hipHostMallocDefault));
+  else
+    CK(hipMalloc((void**)p, sizeof(T) * want));
+  *cap = want;
+  return MST_OK;
+}
+
+// One NJ launch for the jobs whose scores are done (none in flight): per job
+// the fp32 matrix and NJ state in its block, then a workgroup each, then the
+// joins to pinned staging.
+int launch_nj_jobs(mst_server* S) {
+  mst_ctx* ctx = S->ctx;
+  if (!S->nj_batch.empty() || S->nj_queue.empty()) return MST_OK;
+  std::vector<int64_t> ids;
+  int64_t total = 0;
+  while (!S->nj_queue.empty() && ids.size() < 4096) {
+    const int64_t id = S->nj_queue.front();
+    S->nj_queue.pop_front();
+    Job& J = S->jobs[id];
+    if (J.cancel || J.state != MST_JOB_RUNNING) continue;
+    ids.push_back(id);
+    total += J.n - 1;
+  }
+  if (ids.empty()) return MST_OK;
+  const int cnt = (int)ids.size();
+  int rc;
+  if ((rc = grow_buf(ctx, &S->d_nj_args, &S->nj_args_cap, cnt, false))) return rc;
+  if ((rc = grow_buf(ctx, &S->h_nj_args, &S->nj_args_host_cap, cnt, true))) return rc;
+  if ((rc = grow_buf(ctx, &S->h_nj, &S->nj_cap, std::max<int64_t>(total, 1), true))) return rc;
+  S->nj_stage_off.clear();
+  int64_t o = 0;
+  for (int k = 0; k < cnt; ++k) {
+    Job& J = S->jobs[ids[k]];
+    const int n = J.n;
+    float* D = (float*)(S->d_arena + J.o_D);
+    int* act = (int*)(S->d_arena + J.o_act);
+    launch_scores_to_float((const int32_t*)(S->d_arena + J.arena_off), D, (int64_t)n * n, S->nj_st);
+    launch_nj_init(act, act + 2 * n, n, S->nj_st);
+    NjArgs& a = S->h_nj_args[k];
+    a.D = D;
+    a.ld = n;
+    a.n0 = n;
+    a.T = (double*)(S->d_arena + J.o_T);
+    a.act = act;
+    a.ids = act + 2 * n;
+    a.joins = (NjJoin*)(S->d_arena + J.o_joins);
+    a.ratio = S->nj_ratio;
+    S->nj_stage_off.push_back(o);
+    o += n - 1;
+    J.refs += 1;
+  }
+  CK(hipGetLastError());
+  CK(hipMemcpyAsync(S->d_nj_args, S->h_nj_args, sizeof(NjArgs) * cnt, hipMemcpyHostToDevice, S->nj_st));
+  CK(launch_nj_batch(S->d_nj_args, cnt, S->nj_st));
+  for (int k = 0; k < cnt; ++k) {
+    Job& J = S->jobs[ids[k]];
+    CK(hipMemcpyAsync(S->h_nj + S->nj_stage_off[k], S->d_arena + J.o_joins, sizeof(NjJoin) * (J.n - 1),
+                      hipMemcpyDeviceToHost, S->nj_st));
+  }
+  CK(hipEventRecord(S->nj_ev, S->nj_st));
+  S->nj_batch.swap(ids);
+  return MST_OK;
+}
+
+int retire_nj(mst_server* S, bool wait, int* ndone) {
+  mst_ctx* ctx = S->ctx;
+  if (S->nj_batch.empty()) return MST_OK;
+  hipError_t e = wait ? hipEventSynchronize(S->nj_ev) : hipEventQuery(S->nj_ev);
+  if (e == hipErrorNotReady) return MST_OK;
+  if (e != hipSuccess) return ctx->fail(MST_E_HIP, "NJ batch", hipGetErrorString(e));
+  const double t = now_s();
+  for (size_t k = 0; k < S->nj_batch.size(); ++k) {
+    Job& J = S->jobs[S->nj_batch[k]];
+    J.refs -= 1;
+    if (J.cancel) continue;
+    J.tree.resize(J.n - 1);
+    for (int q = 0; q + 1 < J.n; ++q) nj_to_join(S->h_nj[S->nj_stage_off[k] + q], &J.tree[q]);
+    if ((J.what & MST_SERVE_MSA) == MST_SERVE_MSA)
+      start_msa(J);
+    else
+      job_done(S, J, t, ndone);
+  }
+  S->nj_batch.clear();
+  prune_running(S);
+  return MST_OK;
+}
+
+// One merge batch (none in flight): the leaves of jobs new to the alignment
+// stage, then every merge whose children exist, jobs earliest deadline
+// first, while the batch's scratch stays within merge_bytes. A merge's rows
+// go to its job's region, else the arena; a merge with no room waits.
+int launch_merges(mst_server* S) {
+  mst_ctx* ctx = S->ctx;
+  if (S->msa_busy) return MST_OK;
+  std::vector<int64_t> order;
   for (int64_t id : S->running) {
     Job& J = S->jobs[id];
-    if (J.state == MST_JOB_RUNNING && !J.cancel) {
-      keep.push_back(id);
-    } else if (J.cancel && J.rounds_ref == 0) {
-      release_job(S, J);
-      J.state = MST_JOB_CANCELLED;
-    } else if (J.cancel) {
-      keep.push_back(id);  // released once its rounds retire
-    }
+    if (J.state == MST_JOB_RUNNING && !J.cancel && J.stage == kStageMsa) order.push_back(id);
   }
-  S->running.swap(keep);
+  if (order.empty()) return MST_OK;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int64_t x, int64_t y) { return S->jobs[x].deadline < S->jobs[y].deadline; });
+  std::vector<LeafDesc> leaves;
+  std::vector<MergeSpec> spec;
+  std::vector<int64_t> touched;
+  S->msa_batch.clear();
+  const int64_t budget = (int64_t)S->merge_bytes;
+  int64_t bytes = 0;
+  bool full = false;
+  for (int64_t id : order) {
+    Job& J = S->jobs[id];
+    bool touch = false;
+    if (!J.leaves_done) {
+      for (int i = 0; i < J.n; ++i) {
+        Group& G = J.groups[i];
+        LeafDesc l;
+        l.src = J.roff[J.where[i]] + J.o_codes;
+        l.dst = S->d_arena + G.off;
+        l.len = G.len;
+        l.pad = 0;
+        leaves.push_back(l);
+        G.ready = true;  // written ahead of every merge on the merge stream
+      }
+      J.leaves_done = true;
+      touch = true;
+    }
+    for (int k = 0; k + 1 < J.n && !full; ++k) {
+      if (J.merge_state[k] != 0) continue;
+      const mst_join& jn = J.tree[k];
+      Group& X = J.groups[jn.left];
+      Group& Y = J.groups[jn.right];
+      if (!X.ready || !Y.ready) continue;
+      const int64_t b = merge_scratch_bytes(X.len, Y.len);
+      if (!spec.empty() && bytes + b > budget) {
+        full = true;
+        break;
+      }
+      Group& Z = J.groups[jn.node];
+      const int64_t cap = std::max<int64_t>((int64_t)(X.k + Y.k) * ((int64_t)X.len + Y.len), 1);
+      int64_t off = J.msa_pool.alloc(cap);
+      bool in_arena = false;
+      if (off >= 0) {
+        off += J.o_msa;
+      } else {
+        off = S->arena.alloc(cap);
+        in_arena = true;
+        if (off < 0) continue;  // no room now: waits for memory
+      }
+      Z.off = off;
+      Z.cap = cap;
+      Z.in_arena = in_arena;
+      Z.k = X.k + Y.k;
+      bytes += b;
+      J.merge_state[k] = 1;
+      spec.push_back(MergeSpec{S->d_arena + X.off, X.k, X.len, S->d_arena + Y.off, Y.k, Y.len,
+                               S->d_arena + Z.off});
+      S->msa_batch.push_back(std::make_pair(id, k));
+      touch = true;
+    }
+    if (touch) {
+      touched.push_back(id);
+      J.refs += 1;
+    }
+    if (full) break;
+  }
+  if (touched.empty()) return MST_OK;
+  CK(S->eng.leaves((const int8_t*)S->d_arena, leaves.data(), (int)leaves.size(), S->msa_st));
+  CK(S->eng.enqueue(spec.data(), (int)spec.size(), ctx->d_S, ctx->go, ctx->ge, ctx->num_cus, S->msa_st));
+  CK(hipEventRecord(S->msa_ev, S->msa_st));
+  S->msa_leaf_jobs.swap(touched);
+  S->msa_busy = true;
+  return MST_OK;
+}
+
+void release_group(mst_server* S, Job& J, Group& G) {
+  if (G.leaf || G.off < 0) return;
+  if (G.in_arena)
+    S->arena.release(G.off, G.cap);
+  else
+    J.msa_pool.release(G.off - J.o_msa, G.cap);
+  G.off = -1;
+  G.cap = 0;
+}
+
+int retire_merges(mst_server* S, bool wait, int* ndone) {
+  mst_ctx* ctx = S->ctx;
+  if (!S->msa_busy) return MST_OK;
+  hipError_t e = wait ? hipEventSynchronize(S->msa_ev) : hipEventQuery(S->msa_ev);
+  if (e == hipErrorNotReady) return MST_OK;
+  if (e != hipSuccess) return ctx->fail(MST_E_HIP, "merge batch", hipGetErrorString(e));
+  const double t = now_s();
+  const int32_t* lens = S->eng.lens();
+  for (size_t b = 0; b < S->msa_batch.size(); ++b) {
+    Job& J = S->jobs[S->msa_batch[b].first];
+    const int k = S->msa_batch[b].second;
+    const mst_join& jn = J.tree[k];
+    Group& Z = J.groups[jn.node];
+    Z.len = lens[b];
+    const int64_t keep = std::max<int64_t>((int64_t)Z.k * Z.len, 1);
+    if (Z.in_arena)
+      S->arena.shrink(Z.off, Z.cap, keep);
+    else
+      J.msa_pool.shrink(Z.off - J.o_msa, Z.cap, keep);
+    Z.cap = keep;
+    Z.ready = true;
+    J.merge_state[k] = 2;
+    J.merges_done += 1;
+    release_group(S, J, J.groups[jn.left]);
+    release_group(S, J, J.groups[jn.right]);
+  }
+  for (int64_t id : S->msa_leaf_jobs) {
+    Job& J = S->jobs[id];
+    J.refs -= 1;
+    if (!J.cancel && J.state == MST_JOB_RUNNING && J.merges_done == J.n - 1) job_done(S, J, t, ndone);
+  }
+  S->msa_batch.clear();
+  S->msa_leaf_jobs.clear();
+  S->msa_busy = false;
+  prune_running(S);
   return MST_OK;
 }
 
@@ -634,6 +980,10 @@ extern "C" int mst_server_create(mst_ctx* ctx, int64_t budget_bytes, mst_server*
     return fail("bufs", e);
   for (auto& st : S->side)
     if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess) return fail("stream", e);
+  if ((e = hipStreamCreateWithFlags(&S->nj_st, hipStreamNonBlocking)) != hipSuccess) return fail("stream", e);
+  if ((e = hipStreamCreateWithFlags(&S->msa_st, hipStreamNonBlocking)) != hipSuccess) return fail("stream", e);
+  if ((e = hipEventCreateWithFlags(&S->nj_ev, hipEventDisableTiming)) != hipSuccess) return fail("event", e);
+  if ((e = hipEventCreateWithFlags(&S->msa_ev, hipEventDisableTiming)) != hipSuccess) return fail("event", e);
   S->arena.init(budget_bytes, 256);
   S->pos.init(max_pos, 1);
   S->slots.init(max_slots, 1);
@@ -648,6 +998,13 @@ extern "C" int mst_server_destroy(mst_server* S) {
   hipStreamSynchronize(ctx->stream);
   for (hipStream_t st : S->side)
     if (st) hipStreamSynchronize(st);
+  for (hipStream_t st : {S->nj_st, S->msa_st})
+    if (st) hipStreamSynchronize(st);
+  for (hipEvent_t ev : {S->nj_ev, S->msa_ev})
+    if (ev) hipEventDestroy(ev);
+  if (S->d_nj_args) hipFree(S->d_nj_args);
+  if (S->h_nj_args) hipHostFree(S->h_nj_args);
+  if (S->h_nj) hipHostFree(S->h_nj);
   for (RoundBuf& B : S->rb) {
     hipEvent_t evs[] = {B.ev_start, B.ev_end, B.ev_go, B.ev_side[0], B.ev_side[1], B.ev_side[2],
                         B.ev_side[3]};
@@ -665,19 +1022,30 @@ extern "C" int mst_server_destroy(mst_server* S) {
     if (p) hipFree(p);
   for (hipStream_t st : S->side)
     if (st) hipStreamDestroy(st);
+  for (hipStream_t st : {S->nj_st, S->msa_st})
+    if (st) hipStreamDestroy(st);
   delete S;
   return MST_OK;
 }
 
 extern "C" int mst_server_submit(mst_server* S, const char* ascii, const int64_t* offsets, int n,
                                  int64_t* job) {
+  return mst_server_submit_job(S, ascii, offsets, n, MST_SERVE_SCORES, job);
+}
+
+extern "C" int mst_server_submit_job(mst_server* S, const char* ascii, const int64_t* offsets, int n,
+                                     int what, int64_t* job) {
   if (!S || !job || n < 1 || !offsets || (!ascii && offsets[n] > offsets[0])) return MST_E_ARG;
   mst_ctx* ctx = S->ctx;
+  if (what != MST_SERVE_SCORES && what != MST_SERVE_TREE && what != MST_SERVE_MSA)
+    return ctx->fail(MST_E_ARG, "mst_server_submit_job", "what: MST_SERVE_SCORES, _TREE or _MSA");
+  if (n > 46340) return ctx->fail(MST_E_ARG, "mst_server_submit_job", "too many sequences");
   for (int i = 0; i < n; ++i)
     if (offsets[i + 1] < offsets[i]) return ctx->fail(MST_E_ARG, "mst_server_submit", "offsets decrease");
   Job J;
   J.id = S->next_id++;
   J.n = n;
+  J.what = what;
   J.off.assign(offsets, offsets + n + 1);
   J.ascii.assign(ascii ? ascii + offsets[0] : nullptr, ascii ? ascii + offsets[n] : nullptr);
   J.t_submit = now_s();
@@ -698,12 +1066,14 @@ extern "C" int mst_server_step(mst_server* S, int* ncompleted) {
   mst_ctx* ctx = S->ctx;
   CK(hipSetDevice(ctx->device));
   int done = 0, rc;
-  // retire every finished round
+  // retire every finished round and batch
   for (;;) {
     const size_t before = S->inflight.size();
     if ((rc = retire(S, false, &done))) return rc;
     if (S->inflight.size() == before) break;
   }
+  if ((rc = retire_nj(S, false, &done))) return rc;
+  if ((rc = retire_merges(S, false, &done))) return rc;
   // admit in arrival order while the head fits
   while (!S->waiting.empty()) {
     Job& J = S->jobs[S->waiting.front()];
@@ -724,6 +1094,8 @@ extern "C" int mst_server_step(mst_server* S, int* ncompleted) {
     if ((rc = launch_round(S, &launched))) return rc;
     if (!launched) break;
   }
+  if ((rc = launch_nj_jobs(S))) return rc;
+  if ((rc = launch_merges(S))) return rc;
   if (ncompleted) *ncompleted = done;
   return MST_OK;
 }
@@ -733,16 +1105,22 @@ extern "C" int mst_server_drain(mst_server* S) {
   for (;;) {
     int rc = mst_server_step(S, nullptr);
     if (rc) return rc;
-    if (S->inflight.empty()) {
-      if (S->waiting.empty()) return MST_OK;
-      // the head of the queue does not fit although nothing runs: cannot happen
-      // for an accepted job (its footprint is within the budget), so the arena
-      // is held by finished jobs whose results were not collected
+    int d = 0;
+    if (!S->inflight.empty()) {
+      if ((rc = retire(S, true, &d))) return rc;
+    } else if (!S->nj_batch.empty()) {
+      if ((rc = retire_nj(S, true, &d))) return rc;
+    } else if (S->msa_busy) {
+      if ((rc = retire_merges(S, true, &d))) return rc;
+    } else if (S->waiting.empty() && S->running.empty()) {
+      return MST_OK;
+    } else {
+      // nothing on the device, yet jobs wait: an accepted job's footprint is
+      // within the budget, so the arena is held by finished jobs whose
+      // results were not collected
       return S->ctx->fail(MST_E_STATE, "mst_server_drain",
                           "waiting jobs do not fit: collect or release finished jobs");
     }
-    int d = 0;
-    if ((rc = retire(S, true, &d))) return rc;
   }
 }
 
@@ -776,6 +1154,58 @@ extern "C" int mst_server_result(mst_server* S, int64_t job, int32_t* scores_nxn
   return MST_OK;
 }
 
+extern "C" int mst_server_tree(mst_server* S, int64_t job, mst_join* joins) {
+  if (!S || !joins) return MST_E_ARG;
+  mst_ctx* ctx = S->ctx;
+  auto it = S->jobs.find(job);
+  if (it == S->jobs.end()) return ctx->fail(MST_E_ARG, "mst_server_tree", "unknown job");
+  const Job& J = it->second;
+  if (J.state != MST_JOB_DONE) return ctx->fail(MST_E_STATE, "mst_server_tree", "job not done");
+  if ((J.what & MST_SERVE_TREE) != MST_SERVE_TREE)
+    return ctx->fail(MST_E_STATE, "mst_server_tree", "the job was submitted without a tree");
+  std::copy(J.tree.begin(), J.tree.end(), joins);
+  return MST_OK;
+}
+
+extern "C" int mst_server_msa(mst_server* S, int64_t job, char* rows, int64_t stride, int* msa_len) {
+  static const char kLetters[] = "ARNDCQEGHILKMFPSTWYVBZX*-";
+  if (!S || !msa_len) return MST_E_ARG;
+  mst_ctx* ctx = S->ctx;
+  auto it = S->jobs.find(job);
+  if (it == S->jobs.end()) return ctx->fail(MST_E_ARG, "mst_server_msa", "unknown job");
+  const Job& J = it->second;
+  if (J.state != MST_JOB_DONE) return ctx->fail(MST_E_STATE, "mst_server_msa", "job not done");
+  if ((J.what & MST_SERVE_MSA) != MST_SERVE_MSA)
+    return ctx->fail(MST_E_STATE, "mst_server_msa", "the job was submitted without an alignment");
+  const Group& R = J.groups[2 * (size_t)J.n - 2];
+  *msa_len = R.len;
+  if (!rows) return MST_OK;
+  if (stride < (int64_t)R.len + 1) return ctx->fail(MST_E_ARG, "mst_server_msa", "stride too small");
+  // rows of the root: the members of its left subtree, then of its right
+  std::vector<int> members;
+  std::vector<int> stack{2 * J.n - 2};
+  while (!stack.empty()) {
+    const int v = stack.back();
+    stack.pop_back();
+    if (v < J.n) {
+      members.push_back(v);
+    } else {
+      stack.push_back(J.tree[v - J.n].right);
+      stack.push_back(J.tree[v - J.n].left);
+    }
+  }
+  CK(hipSetDevice(ctx->device));
+  std::vector<uint8_t> h((size_t)R.k * R.len);
+  if (!h.empty()) CK(hipMemcpy(h.data(), S->d_arena + R.off, h.size(), hipMemcpyDeviceToHost));
+  for (int r = 0; r < R.k; ++r) {
+    char* o = rows + (int64_t)members[r] * stride;
+    const uint8_t* src = &h[(size_t)r * R.len];
+    for (int t = 0; t < R.len; ++t) o[t] = kLetters[src[t] < 25 ? src[t] : 24];
+    o[R.len] = 0;
+  }
+  return MST_OK;
+}
+
 extern "C" int mst_server_cancel(mst_server* S, int64_t job) {
   if (!S) return MST_E_ARG;
   auto it = S->jobs.find(job);
@@ -785,7 +1215,7 @@ extern "C" int mst_server_cancel(mst_server* S, int64_t job) {
   if (J.state == MST_JOB_WAITING) {
     J.state = MST_JOB_CANCELLED;
     S->waiting.erase(std::find(S->waiting.begin(), S->waiting.end(), job));
-  } else if (J.state == MST_JOB_RUNNING && J.rounds_ref == 0) {
+  } else if (J.state == MST_JOB_RUNNING && J.refs == 0) {
     release_job(S, J);
     J.state = MST_JOB_CANCELLED;
     S->running.erase(std::find(S->running.begin(), S->running.end(), job));
@@ -820,6 +1250,14 @@ extern "C" int mst_server_set(mst_server* S, const char* key, double value) {
   }
   if (!strcmp(key, "rate") && value > 0) {
     S->rate = value;
+    return MST_OK;
+  }
+  if (!strcmp(key, "nj_ratio") && value >= 0 && value <= 1) {
+    S->nj_ratio = (float)value;
+    return MST_OK;
+  }
+  if (!strcmp(key, "merge_bytes") && value >= (1 << 20) && value <= 1e12) {
+    S->merge_bytes = value;
     return MST_OK;
   }
   return S->ctx->fail(MST_E_ARG, "mst_server_set", "unknown key or value out of range");

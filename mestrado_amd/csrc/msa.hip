@@ -16,16 +16,52 @@
 // Group rows are kept in forward column order as codes 0..23 and 24 (gap);
 // the DP runs on reversed columns like the pairwise path, so the traceback
 // path comes out in forward order.
+//
+// Merges run in batches (MergeEngine): a batch is any set of merges whose
+// inputs exist (a level of one guide tree, or ready merges of many jobs).
+// Each stage is one launch over the whole batch, its threads flattened over
+// the batch's columns, rows or table entries; a thread finds its merge by a
+// binary search over the batch's prefix sums. The flag kernels and the
+// traceback take the batch's merges as one job list.
+#include <algorithm>
+#include <vector>
+
 #include "common.h"
-#include "msa.h"
+#include "merge.h"
 
 namespace mst {
 
-// counts[col][25] of a group's k rows (row-major k x len).
-__global__ void profile_counts_kernel(const uint8_t* __restrict__ rows, int k, int len,
-                                      int32_t* __restrict__ counts) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= len) return;
+namespace {
+
+constexpr int kStripRows = 512;  // rows of one flag-kernel strip
+
+__host__ __device__ inline int64_t r256(int64_t x) { return (x + 255) / 256 * 256; }
+
+// largest i in [0, count) with key(i) <= x (keys ascending in i)
+template <typename Key>
+__device__ int find_merge(int count, int64_t x, Key key) {
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (key(mid) <= x)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// Prefix sums the flattened kernels search, one entry per merge, stored after
+// the batch's descriptors.
+struct Prefix {
+  int64_t col0;   // counts: columns of X then Y of nontrivial merges
+  int64_t yrow0;  // P rows (nY of nontrivial merges)
+  int64_t item0;  // table entries, one per wave (strip, step)
+};
+
+// counts[c][25] of column c of a group's k rows (row-major k x len)
+__device__ __forceinline__ void count_column(const uint8_t* __restrict__ rows, int k, int len, int c,
+                                             int32_t* __restrict__ counts) {
   int32_t cnt[25];
 #pragma unroll
   for (int a = 0; a < 25; ++a) cnt[a] = 0;
@@ -38,11 +74,9 @@ __global__ void profile_counts_kernel(const uint8_t* __restrict__ rows, int k, i
   for (int a = 0; a < 25; ++a) counts[(int64_t)c * 25 + a] = cnt[a];
 }
 
-// P_r(a) for every Y column (forward order): P[col][24].
-__global__ void profile_p_kernel(const int32_t* __restrict__ countsY, int m, int kY,
-                                 const int8_t* __restrict__ S, float* __restrict__ P) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= m) return;
+// P_j(a) of Y column j (forward order): P[j][24]
+__device__ __forceinline__ void p_row(const int32_t* __restrict__ countsY, int kY,
+                                      const int8_t* __restrict__ S, int j, float* __restrict__ P) {
   float fy[24];
 #pragma unroll
   for (int b = 0; b < 24; ++b) fy[b] = (float)countsY[(int64_t)j * 25 + b] / (float)kY;
@@ -57,20 +91,13 @@ __global__ void profile_p_kernel(const int32_t* __restrict__ countsY, int m, int
   }
 }
 
-// Byte table of S''(r,c) = rint(s(r,c)) + go + ge in the skewed layout of the
-// flag kernel: entry ((strip*(n+63) + t)*64 + lane) holds the 8 rows of the
-// lane (DP row = strip*512 + lane*8 + i) at DP column c = t - lane. DP row r is
+// Entry (strip s, step t, lane) of the byte table of S''(r,c) = rint(s(r,c))
+// + go + ge in the skewed layout of the flag kernel: the 8 rows of the lane
+// (DP row = s*512 + lane*8 + i) at DP column c = t - lane. DP row r is
 // forward Y column m-1-r and DP column c is forward X column n-1-c.
-__global__ void profile_table_kernel(const int32_t* __restrict__ countsX, int n, int kX,
-                                     const float* __restrict__ P, int m, int go_plus_ge,
-                                     uint2* __restrict__ tab) {
-  const int lane = threadIdx.x & 63;
-  const int64_t item = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const int Tn = n + MST_WAVE - 1;
-  const int nstrips = (m + 511) / 512;
-  if (item >= (int64_t)nstrips * Tn) return;
-  const int s = (int)(item / Tn);
-  const int t = (int)(item - (int64_t)s * Tn);
+__device__ __forceinline__ uint2 table_entry(const int32_t* __restrict__ countsX, int n, int kX,
+                                             const float* __restrict__ P, int m, int go_plus_ge,
+                                             int s, int t, int lane) {
   const int c = t - lane;
   uint32_t w[2] = {0, 0};
   if (c >= 0 && c < n) {
@@ -80,7 +107,7 @@ __global__ void profile_table_kernel(const int32_t* __restrict__ countsX, int n,
     for (int a = 0; a < 24; ++a) fx[a] = (float)countsX[(int64_t)xc * 25 + a] / (float)kX;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-      const int r = s * 512 + lane * 8 + i;
+      const int r = s * kStripRows + lane * 8 + i;
       int v = 0;
       if (r < m) {
         const float* p = P + (int64_t)(m - 1 - r) * 24;
@@ -95,84 +122,313 @@ __global__ void profile_table_kernel(const int32_t* __restrict__ countsX, int n,
       w[i >> 2] |= ((uint32_t)v & 0xFFu) << (8 * (i & 3));
     }
   }
-  tab[((int64_t)s * Tn + t) * MST_WAVE + lane] = make_uint2(w[0], w[1]);
+  return make_uint2(w[0], w[1]);
 }
 
-// Source column of every output column: sx[t] = X column or -1 (gap), same
-// for Y; one thread walks the path (paths are at most a few 10^4 long).
-__global__ void path_columns_kernel(const uint8_t* __restrict__ ops, const int32_t* __restrict__ len,
-                                    int32_t* __restrict__ sx, int32_t* __restrict__ sy) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const int L = *len;
-  int x = 0, y = 0;
-  for (int t = 0; t < L; ++t) {
-    const int op = ops[t];
-    sx[t] = (op != 2) ? x++ : -1;  // 2 = UP: gap in X
-    sy[t] = (op != 1) ? y++ : -1;  // 1 = LEFT: gap in Y
-  }
+__global__ void mb_counts_kernel(const MergeDesc* __restrict__ d, const Prefix* __restrict__ pre,
+                                 int count, int64_t total, uint8_t* __restrict__ scratch) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= total) return;
+  const int i = find_merge(count, x, [&](int k) { return pre[k].col0; });
+  const MergeDesc m = d[i];
+  const int64_t local = x - pre[i].col0;
+  if (local < m.nX)
+    count_column(m.rx, m.kX, m.nX, (int)local, (int32_t*)(scratch + m.cx));
+  else
+    count_column(m.ry, m.kY, m.nY, (int)(local - m.nX), (int32_t*)(scratch + m.cy));
 }
 
-// New rows: X's kX rows then Y's kY rows, L columns.
-__global__ void merge_rows_kernel(const uint8_t* __restrict__ rx, int kX, int nx,
-                                  const uint8_t* __restrict__ ry, int kY, int ny,
-                                  const int32_t* __restrict__ sx, const int32_t* __restrict__ sy,
-                                  int L, uint8_t* __restrict__ out) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int row = blockIdx.y;
-  if (t >= L) return;
-  uint8_t v;
-  if (row < kX) {
-    const int c = sx[t];
-    v = c < 0 ? (uint8_t)24 : rx[(int64_t)row * nx + c];
+__global__ void mb_p_kernel(const MergeDesc* __restrict__ d, const Prefix* __restrict__ pre,
+                            int count, int64_t total, const int8_t* __restrict__ S,
+                            uint8_t* __restrict__ scratch) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= total) return;
+  const int i = find_merge(count, x, [&](int k) { return pre[k].yrow0; });
+  const MergeDesc m = d[i];
+  p_row((const int32_t*)(scratch + m.cy), m.kY, S, (int)(x - pre[i].yrow0), (float*)(scratch + m.p));
+}
+
+__global__ void mb_table_kernel(const MergeDesc* __restrict__ d, const Prefix* __restrict__ pre,
+                                int count, int64_t total, int go_plus_ge,
+                                uint8_t* __restrict__ scratch) {
+  const int lane = threadIdx.x & 63;
+  const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (w >= total) return;
+  const int i = find_merge(count, w, [&](int k) { return pre[k].item0; });
+  const MergeDesc m = d[i];
+  const int64_t item = w - pre[i].item0;
+  const int Tn = m.nX + MST_WAVE - 1;
+  const int s = (int)(item / Tn);
+  const int t = (int)(item - (int64_t)s * Tn);
+  uint2* tab = (uint2*)(scratch + m.tab);
+  tab[((int64_t)s * Tn + t) * MST_WAVE + lane] =
+      table_entry((const int32_t*)(scratch + m.cx), m.nX, m.kX, (const float*)(scratch + m.p), m.nY,
+                  go_plus_ge, s, t, lane);
+}
+
+// Source column of every output column, one thread per merge: sx[t] = X
+// column or -1 (gap in X), same for Y; lens[i] = path length.
+__global__ void mb_path_kernel(const MergeDesc* __restrict__ d, int count,
+                               const int32_t* __restrict__ tlens, uint8_t* __restrict__ scratch,
+                               int32_t* __restrict__ lens) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const MergeDesc m = d[i];
+  int32_t* sx = (int32_t*)(scratch + m.sxy);
+  int32_t* sy = sx + (m.nX + m.nY);
+  int L;
+  if (m.job < 0) {  // one side empty: the other side's columns, all gaps opposite
+    L = m.nX + m.nY;
+    for (int t = 0; t < L; ++t) {
+      sx[t] = m.nX > 0 ? t : -1;
+      sy[t] = m.nX > 0 ? -1 : t;
+    }
   } else {
-    const int c = sy[t];
-    v = c < 0 ? (uint8_t)24 : ry[(int64_t)(row - kX) * ny + c];
+    L = tlens[m.job];
+    const uint8_t* ops = scratch + m.ops;
+    int x = 0, y = 0;
+    for (int t = 0; t < L; ++t) {
+      const int op = ops[t];
+      sx[t] = (op != OP_UP) ? x++ : -1;
+      sy[t] = (op != OP_LEFT) ? y++ : -1;
+    }
   }
-  out[(int64_t)row * L + t] = v;
+  lens[i] = L;
 }
 
-// Leaf rows: forward codes of sequence s from the reversed set codes.
-__global__ void leaf_rows_kernel(const int8_t* __restrict__ codes, int64_t off, int len,
-                                 uint8_t* __restrict__ out) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < len) out[t] = (uint8_t)codes[off + len - 1 - t];
+// One workgroup per output row: X's kX rows then Y's kY rows, L columns.
+__global__ void mb_rows_kernel(const MergeDesc* __restrict__ d, int count,
+                               const uint8_t* __restrict__ scratch, const int32_t* __restrict__ lens) {
+  const int64_t r = blockIdx.x;
+  const int i = find_merge(count, r, [&](int k) { return d[k].row0; });
+  const MergeDesc m = d[i];
+  const int lr = (int)(r - m.row0);
+  const int L = lens[i];
+  const int32_t* sx = (const int32_t*)(scratch + m.sxy);
+  const int32_t* map = lr < m.kX ? sx : sx + (m.nX + m.nY);
+  const uint8_t* src = lr < m.kX ? m.rx + (int64_t)lr * m.nX : m.ry + (int64_t)(lr - m.kX) * m.nY;
+  uint8_t* out = m.out + (int64_t)lr * L;
+  for (int t = threadIdx.x; t < L; t += blockDim.x) {
+    const int c = map[t];
+    out[t] = c < 0 ? (uint8_t)24 : src[c];
+  }
 }
 
-void launch_profile_counts(const uint8_t* rows, int k, int len, int32_t* counts, hipStream_t st) {
-  if (len > 0)
-    hipLaunchKernelGGL(profile_counts_kernel, dim3((len + 255) / 256), dim3(256), 0, st, rows, k, len,
-                       counts);
+// One workgroup per leaf: forward codes from the reversed code block.
+__global__ void mb_leaf_kernel(const int8_t* __restrict__ codes, const LeafDesc* __restrict__ l) {
+  const LeafDesc q = l[blockIdx.x];
+  for (int t = threadIdx.x; t < q.len; t += blockDim.x) q.dst[t] = (uint8_t)codes[q.src + q.len - 1 - t];
 }
 
-void launch_profile_p(const int32_t* countsY, int m, int kY, const int8_t* S, float* P,
-                      hipStream_t st) {
-  if (m > 0)
-    hipLaunchKernelGGL(profile_p_kernel, dim3((m + 255) / 256), dim3(256), 0, st, countsY, m, kY, S, P);
+__global__ void mb_zero_kernel(unsigned int* p, int n) {
+  const int i = threadIdx.x;
+  if (i < n) p[i] = 0u;
 }
 
-void launch_profile_table(const int32_t* countsX, int n, int kX, const float* P, int m,
-                          int go_plus_ge, uint2* tab, hipStream_t st) {
-  const int64_t items = (int64_t)((m + 511) / 512) * (n + MST_WAVE - 1);
-  const int per_block = 4;  // waves per block
-  hipLaunchKernelGGL(profile_table_kernel, dim3((unsigned)((items + per_block - 1) / per_block)),
-                     dim3(64 * per_block), 0, st, countsX, n, kX, P, m, go_plus_ge, tab);
+int strips_of(int m) { return (m + kStripRows - 1) / kStripRows; }
+
+}  // namespace
+
+int64_t merge_scratch_bytes(int nX, int nY) {
+  const int64_t cols = (int64_t)nX + nY;
+  int64_t b = r256(8 * cols);  // column maps
+  if (nX > 0 && nY > 0) {
+    const int64_t words = flag_words(nX, nY);
+    b += r256(100 * (int64_t)nX) + r256(100 * (int64_t)nY) + r256(96 * (int64_t)nY) + r256(8 * words) +
+         r256(4 * words) + r256(cols);
+  }
+  return b;
 }
 
-void launch_path_columns(const uint8_t* ops, const int32_t* len, int32_t* sx, int32_t* sy,
-                         hipStream_t st) {
-  hipLaunchKernelGGL(path_columns_kernel, dim3(1), dim3(64), 0, st, ops, len, sx, sy);
+hipError_t MergeEngine::grow(void** p, int64_t* cap, int64_t bytes, bool pinned) {
+  if (bytes <= *cap) return hipSuccess;
+  const int64_t want = std::max<int64_t>(bytes, *cap * 3 / 2);
+  hipError_t e;
+  if (*p) {
+    e = pinned ? hipHostFree(*p) : hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    if (e != hipSuccess) return e;
+  }
+  e = pinned ? hipHostMalloc(p, want, hipHostMallocDefault) : hipMalloc(p, want);
+  if (e != hipSuccess) return e;
+  *cap = want;
+  return hipSuccess;
 }
 
-void launch_merge_rows(const uint8_t* rx, int kX, int nx, const uint8_t* ry, int kY, int ny,
-                       const int32_t* sx, const int32_t* sy, int L, uint8_t* out, hipStream_t st) {
-  if (L > 0)
-    hipLaunchKernelGGL(merge_rows_kernel, dim3((L + 255) / 256, kX + kY), dim3(256), 0, st, rx, kX,
-                       nx, ry, kY, ny, sx, sy, L, out);
+MergeEngine::~MergeEngine() {
+  void* dev[] = {d_scratch_, d_desc_, d_jobs_, d_res_, d_lens_, d_bufs_, d_counter_, d_leaf_};
+  for (void* p : dev)
+    if (p) hipFree(p);
+  void* host[] = {h_desc_, h_jobs_, h_lens_, h_leaf_};
+  for (void* p : host)
+    if (p) hipHostFree(p);
 }
 
-void launch_leaf_rows(const int8_t* codes, int64_t off, int len, uint8_t* out, hipStream_t st) {
-  if (len > 0)
-    hipLaunchKernelGGL(leaf_rows_kernel, dim3((len + 255) / 256), dim3(256), 0, st, codes, off, len, out);
+hipError_t MergeEngine::leaves(const int8_t* codes, const LeafDesc* l, int count, hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  hipError_t e;
+  const int64_t bytes = sizeof(LeafDesc) * (int64_t)count;
+  if ((e = grow((void**)&d_leaf_, &leaf_cap_, bytes, false)) != hipSuccess) return e;
+  if ((e = grow((void**)&h_leaf_, &hleaf_cap_, bytes, true)) != hipSuccess) return e;
+  std::copy(l, l + count, h_leaf_);
+  if ((e = hipMemcpyAsync(d_leaf_, h_leaf_, bytes, hipMemcpyHostToDevice, st)) != hipSuccess) return e;
+  hipLaunchKernelGGL(mb_leaf_kernel, dim3(count), dim3(256), 0, st, codes, d_leaf_);
+  return hipGetLastError();
+}
+
+hipError_t MergeEngine::enqueue(const MergeSpec* m, int count, const int8_t* d_S, int go, int ge,
+                                int num_cus, hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  // flag jobs: one strip, 2..4 strips (pipeline of 4 waves), more (8 waves)
+  std::vector<int> cls(count, -1);
+  int64_t nb[3] = {0, 0, 0};
+  for (int i = 0; i < count; ++i) {
+    if (m[i].nX > 0 && m[i].nY > 0) {
+      const int s = strips_of(m[i].nY);
+      cls[i] = s == 1 ? 0 : (s <= 4 ? 1 : 2);
+      nb[cls[i]] += 1;
+    }
+  }
+  const int64_t njobs = nb[0] + nb[1] + nb[2];
+  int64_t next[3] = {0, nb[0], nb[0] + nb[1]};
+  // layout of the scratch block and the prefix sums
+  const int64_t desc_bytes = (int64_t)count * (sizeof(MergeDesc) + sizeof(Prefix));
+  hipError_t e;
+  if ((e = grow((void**)&h_desc_, &hdesc_cap_, desc_bytes, true)) != hipSuccess) return e;
+  if ((e = grow((void**)&h_jobs_, &hjobs_cap_, sizeof(AlignJob) * std::max<int64_t>(njobs, 1), true)) !=
+      hipSuccess)
+    return e;
+  MergeDesc* hd = h_desc_;
+  Prefix* hp = (Prefix*)(h_desc_ + count);
+  int64_t off = 0, col0 = 0, yrow0 = 0, item0 = 0, row0 = 0;
+  int maxn = 1;
+  for (int i = 0; i < count; ++i) {
+    MergeDesc& q = hd[i];
+    q.rx = m[i].rx;
+    q.ry = m[i].ry;
+    q.out = m[i].out;
+    q.kX = m[i].kX;
+    q.nX = m[i].nX;
+    q.kY = m[i].kY;
+    q.nY = m[i].nY;
+    q.pad = 0;
+    q.row0 = row0;
+    row0 += q.kX + q.kY;
+    hp[i].col0 = col0;
+    hp[i].yrow0 = yrow0;
+    hp[i].item0 = item0;
+    q.cx = q.cy = q.p = q.tab = q.ops = 0;
+    q.sxy = off;
+    off += r256(8 * ((int64_t)q.nX + q.nY));
+    if (cls[i] < 0) {
+      q.job = -1;
+      continue;
+    }
+    const int64_t words = flag_words(q.nX, q.nY);
+    q.cx = off;
+    off += r256(100 * (int64_t)q.nX);
+    q.cy = off;
+    off += r256(100 * (int64_t)q.nY);
+    q.p = off;
+    off += r256(96 * (int64_t)q.nY);
+    q.tab = off;
+    off += r256(8 * words);
+    const int64_t fl = off;
+    off += r256(4 * words);
+    q.ops = off;
+    off += r256((int64_t)q.nX + q.nY);
+    col0 += (int64_t)q.nX + q.nY;
+    yrow0 += q.nY;
+    item0 += (int64_t)strips_of(q.nY) * (q.nX + MST_WAVE - 1);
+    q.job = (int)next[cls[i]]++;
+    AlignJob& j = h_jobs_[q.job];
+    j.n = q.nX;
+    j.m = q.nY;
+    j.x_off = j.y_off = 0;
+    j.tab_off = q.tab;
+    j.flag_off = fl / 4;
+    j.ops_off = q.ops;
+    maxn = std::max(maxn, q.nX);
+  }
+  const int64_t grid1 = std::max<int64_t>(1, std::min<int64_t>((int64_t)num_cus * 16, nb[0]));
+  const int64_t grid2 = std::max<int64_t>(1, std::min<int64_t>((int64_t)num_cus * 2, nb[1]));
+  const int64_t grid3 = std::max<int64_t>(1, std::min<int64_t>((int64_t)num_cus, nb[2]));
+  const int64_t bufcap = maxn;
+  const int64_t bufs = bufcap * std::max(std::max(2 * grid1, grid2), grid3);
+  if ((e = grow((void**)&d_scratch_, &scratch_cap_, std::max<int64_t>(off, 256), false)) != hipSuccess)
+    return e;
+  if ((e = grow((void**)&d_desc_, &desc_cap_, desc_bytes, false)) != hipSuccess) return e;
+  if ((e = grow((void**)&d_jobs_, &jobs_cap_, sizeof(AlignJob) * std::max<int64_t>(njobs, 1), false)) !=
+      hipSuccess)
+    return e;
+  if ((e = grow((void**)&d_res_, &res_cap_, sizeof(int32_t) * 2 * std::max<int64_t>(njobs, 1), false)) !=
+      hipSuccess)
+    return e;
+  if ((e = grow((void**)&d_lens_, &lens_cap_, sizeof(int32_t) * count, false)) != hipSuccess) return e;
+  if ((e = grow((void**)&h_lens_, &hlens_cap_, sizeof(int32_t) * count, true)) != hipSuccess) return e;
+  if ((e = grow((void**)&d_bufs_, &bufs_cap_, sizeof(int2) * bufs, false)) != hipSuccess) return e;
+  if ((e = grow((void**)&d_counter_, &counter_cap_, sizeof(unsigned int) * 2, false)) != hipSuccess)
+    return e;
+  const MergeDesc* dd = d_desc_;
+  const Prefix* dp = (const Prefix*)(d_desc_ + count);
+  if ((e = hipMemcpyAsync(d_desc_, h_desc_, desc_bytes, hipMemcpyHostToDevice, st)) != hipSuccess) return e;
+  if (njobs > 0) {
+    if ((e = hipMemcpyAsync(d_jobs_, h_jobs_, sizeof(AlignJob) * njobs, hipMemcpyHostToDevice, st)) !=
+        hipSuccess)
+      return e;
+    hipLaunchKernelGGL(mb_zero_kernel, dim3(1), dim3(64), 0, st, d_counter_, 2);
+    hipLaunchKernelGGL(mb_counts_kernel, dim3((unsigned)((col0 + 255) / 256)), dim3(256), 0, st, dd, dp,
+                       count, col0, d_scratch_);
+    hipLaunchKernelGGL(mb_p_kernel, dim3((unsigned)((yrow0 + 255) / 256)), dim3(256), 0, st, dd, dp, count,
+                       yrow0, d_S, d_scratch_);
+    hipLaunchKernelGGL(mb_table_kernel, dim3((unsigned)((item0 + 3) / 4)), dim3(256), 0, st, dd, dp, count,
+                       item0, go + ge, d_scratch_);
+    AlignArgs a;
+    a.jobs = d_jobs_;
+    a.seqs = nullptr;
+    a.tab = d_scratch_;
+    a.subst = d_S;
+    a.flags = (uint32_t*)d_scratch_;
+    a.scores = d_res_;
+    a.bufs = d_bufs_;
+    a.bufcap = bufcap;
+    a.go = go;
+    a.ge = ge;
+    if (nb[0] > 0) {
+      a.first = 0;
+      a.last = nb[0];
+      a.counter = d_counter_;
+      launch_align(SRC_TAB, a, (int)grid1, st);
+    }
+    if (nb[1] > 0) {
+      a.first = nb[0];
+      a.last = nb[0] + nb[1];
+      a.counter = d_counter_;
+      launch_align_pipe(SRC_TAB, 4, a, (int)grid2, st);
+    }
+    if (nb[2] > 0) {
+      a.first = nb[0] + nb[1];
+      a.last = njobs;
+      a.counter = d_counter_ + 1;
+      launch_align_pipe(SRC_TAB, 8, a, (int)grid3, st);
+    }
+    TraceArgs t;
+    t.jobs = d_jobs_;
+    t.first = 0;
+    t.last = njobs;
+    t.flags = (const uint32_t*)d_scratch_;
+    t.ops = d_scratch_;
+    t.lens = d_res_ + njobs;
+    launch_traceback(t, st);
+  }
+  hipLaunchKernelGGL(mb_path_kernel, dim3((count + 63) / 64), dim3(64), 0, st, dd, count,
+                     (const int32_t*)(d_res_ + njobs), d_scratch_, d_lens_);
+  if (row0 > 0)
+    hipLaunchKernelGGL(mb_rows_kernel, dim3((unsigned)row0), dim3(256), 0, st, dd, count,
+                       (const uint8_t*)d_scratch_, (const int32_t*)d_lens_);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  return hipMemcpyAsync(h_lens_, d_lens_, sizeof(int32_t) * count, hipMemcpyDeviceToHost, st);
 }
 
 }  // namespace mst

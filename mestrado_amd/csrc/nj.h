@@ -2,6 +2,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/mestrado.h"
+
 namespace mst {
 
 struct NjJoin {
@@ -24,5 +26,13 @@ struct NjArgs {
 };
 
 hipError_t launch_nj(const NjArgs& a, hipStream_t st);
+// One workgroup per matrix: d_args[0 .. count) in device memory.
+hipError_t launch_nj_batch(const NjArgs* d_args, int count, hipStream_t st);
+// act/ids of NjArgs before the first step
+void launch_nj_init(int* act, int* ids, int n, hipStream_t st);
+// D = (float)scores, n elements (the NJ input of a score matrix)
+void launch_scores_to_float(const int32_t* s, float* d, int64_t n, hipStream_t st);
+// Join record of the C ABI with TreeNode::join's branch lengths.
+void nj_to_join(const NjJoin& in, mst_join* out);
 
 }  // namespace mst

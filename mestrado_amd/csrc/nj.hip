@@ -46,7 +46,8 @@ __device__ __forceinline__ bool better(const Best& a, const Best& b) {
   return a.I > b.I;
 }
 
-__global__ void __launch_bounds__(NJ_THREADS) nj_kernel(NjArgs a) {
+// The whole join loop of one matrix, run by one workgroup of NJ_THREADS.
+__device__ __forceinline__ void nj_run(const NjArgs a) {
   __shared__ Best red[NJ_THREADS / 64];
   __shared__ Best win;
   __shared__ int s_m, s_R, s_steps, s_next;
@@ -209,8 +210,44 @@ __global__ void __launch_bounds__(NJ_THREADS) nj_kernel(NjArgs a) {
   }
 }
 
+__global__ void __launch_bounds__(NJ_THREADS) nj_kernel(NjArgs a) { nj_run(a); }
+
+// Independent matrices (jobs of the served form), one workgroup each.
+__global__ void __launch_bounds__(NJ_THREADS) nj_batch_kernel(const NjArgs* __restrict__ args) {
+  nj_run(args[blockIdx.x]);
+}
+
 hipError_t launch_nj(const NjArgs& a, hipStream_t st) {
   hipLaunchKernelGGL(nj_kernel, dim3(1), dim3(NJ_THREADS), 0, st, a);
+  return hipGetLastError();
+}
+
+__global__ void scores_to_float_kernel(const int32_t* __restrict__ s, float* __restrict__ d, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = (float)s[i];
+}
+
+// act and ids (2n each): 0..n-1 in the first half, zeros in the second
+__global__ void nj_init_kernel(int* __restrict__ act, int* __restrict__ ids, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 2 * n) {
+    act[i] = i < n ? i : 0;
+    ids[i] = i < n ? i : 0;
+  }
+}
+
+void launch_nj_init(int* act, int* ids, int n, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(nj_init_kernel, dim3((2 * n + 255) / 256), dim3(256), 0, st, act, ids, n);
+}
+
+void launch_scores_to_float(const int32_t* s, float* d, int64_t n, hipStream_t st) {
+  if (n > 0)
+    hipLaunchKernelGGL(scores_to_float_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, s, d, n);
+}
+
+hipError_t launch_nj_batch(const NjArgs* d_args, int count, hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(nj_batch_kernel, dim3(count), dim3(NJ_THREADS), 0, st, d_args);
   return hipGetLastError();
 }
 

@@ -45,6 +45,20 @@ def test_edge_lengths(ctx, form):
     assert len(bad) == 0, [(lens[i], lens[j], g[i, j], o[i, j]) for i, j in bad[:10]]
 
 
+@pytest.mark.parametrize("form", [M.FORM_PLAIN, M.FORM_BATCHED])
+def test_empty_sequences(ctx, form):
+    # empty sequences next to short and long ones: an empty row sequence
+    # paired with a non-empty one, empty columns, empty self pairs
+    rng = np.random.default_rng(3)
+    lens = [0, 5, 0, 300, 1, 0, 700, 64, 0, 1200, 2, 0]
+    seqs = [_rand(rng, L) for L in lens]
+    st = ctx.upload(seqs)
+    g = st.all_pairs(form)
+    o, _ = O.all_pairs(seqs, S)
+    assert (g == o).all()
+    st.free()
+
+
 def test_batched_packed_class_edges(ctx):
     # row lengths at the edges of the packed classes (K = 3, 4, 5 rows per lane)
     # against column sequences up to 1100, next to rows that take the int32 kernel

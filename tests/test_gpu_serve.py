@@ -1,6 +1,7 @@
 """Served form (continuous batching, mst_server_*): every job's matrix is the
-oracle's, alone and in a mix; admission under a small budget, rejection,
-cancellation and misuse."""
+oracle's, alone and in a mix; jobs with a tree and an alignment give the
+one-job path's (and the oracle's) tree and rows, alone and in a mix;
+admission under a small budget, rejection, cancellation and misuse."""
 import numpy as np
 import pytest
 
@@ -93,3 +94,101 @@ def test_cancel_and_misuse(ctx):
             srv.job(a)  # forgotten after its result was taken
         with pytest.raises(M.MestradoError):
             srv.set("no-such-key", 1)
+
+
+JOIN_FIELDS = ("node", "left", "right", "d", "Ti", "Tj", "n", "q", "len_left", "len_right")
+
+
+def _tree_tuple(joins):
+    return [tuple(getattr(j, f) for f in JOIN_FIELDS) for j in joins]
+
+
+def _offline(ctx, seqs):
+    """Scores, tree and alignment of the one-job path (all_pairs, nj, msa)."""
+    st = ctx.upload(seqs)
+    D = st.all_pairs(M.FORM_BATCHED)
+    joins = st.nj() if len(seqs) > 1 else []
+    rows = st.msa(joins)
+    st.free()
+    return D, joins, rows
+
+
+def _msa_jobs(seed):
+    rng = np.random.default_rng(seed)
+    spec = [(1, 50, 80), (2, 30, 60), (3, 10, 40), (12, 100, 300), (40, 50, 200), (7, 500, 900),
+            (6, 0, 3), (9, 600, 1300), (25, 20, 120)]
+    return [_job(rng, n, lo, hi) for n, lo, hi in spec]
+
+
+def _check_job(srv, jid, seqs, want, what):
+    D, joins, rows = want
+    assert srv.job(jid)["state"] == M.JOB_DONE
+    if what == M.SERVE_MSA:
+        assert srv.msa(jid) == rows
+    if what in (M.SERVE_TREE, M.SERVE_MSA) and len(seqs) > 1:
+        assert _tree_tuple(srv.tree(jid)) == _tree_tuple(joins)
+    assert (srv.result(jid) == D).all()
+
+
+def test_msa_jobs_alone_and_in_mix(ctx):
+    """Scores, NJ tree and alignment of every job equal the one-job path's,
+    alone and with jobs of every stage sharing rounds, NJ launches and merge
+    batches."""
+    jobs = _msa_jobs(5)
+    want = [_offline(ctx, j) for j in jobs]
+    with M.Server(ctx, 1 << 30) as srv:
+        for j, w in zip(jobs, want):
+            jid = srv.submit(j, M.SERVE_MSA)
+            srv.drain()
+            _check_job(srv, jid, j, w, M.SERVE_MSA)
+        srv.set("round_cells", 2e6)
+        srv.set("merge_bytes", 1 << 20)  # small merge batches: levels of many jobs interleave
+        ids, whats = [], []
+        for k, j in enumerate(jobs + jobs[::-1]):
+            what = (M.SERVE_MSA, M.SERVE_TREE, M.SERVE_SCORES)[k % 3]
+            ids.append(srv.submit(j, what))
+            whats.append(what)
+            if k % 2:
+                srv.step()
+        srv.drain()
+        for jid, j, w, what in zip(ids, jobs + jobs[::-1], want + want[::-1], whats):
+            _check_job(srv, jid, j, w, what)
+        st = srv.stats()
+        assert st["waiting"] == 0 and st["running"] == 0 and st["arena_used"] == 0
+
+
+def test_msa_job_matches_oracle(ctx):
+    """A served alignment is the oracle's progressive alignment along the
+    served tree, and the served tree is the oracle's NJ of the oracle's
+    scores."""
+    rng = np.random.default_rng(9)
+    seqs = _job(rng, 14, 40, 160)
+    D, _ = O.all_pairs(seqs, S)
+    ref_joins, _ = O.nj_fullscan(D.astype(np.float32))
+    with M.Server(ctx, 1 << 28) as srv:
+        jid = srv.submit(seqs, M.SERVE_MSA)
+        srv.drain()
+        joins = srv.tree(jid)
+        assert [(j.node, j.left, j.right) for j in joins] == [(j.node, j.left, j.right) for j in ref_joins]
+        assert srv.msa(jid) == O.progressive_msa(seqs, joins, S)
+        assert (srv.result(jid) == D).all()
+
+
+def test_msa_job_misuse(ctx):
+    rng = np.random.default_rng(4)
+    with M.Server(ctx, 1 << 26) as srv:
+        jid = srv.submit(_job(rng, 5, 20, 40), M.SERVE_SCORES)
+        with pytest.raises(M.MestradoError):
+            srv.tree(jid)  # not done
+        srv.drain()
+        with pytest.raises(M.MestradoError):
+            srv.msa(jid)  # submitted without an alignment
+        with pytest.raises(M.MestradoError):
+            srv.submit(_job(rng, 3, 5, 9), 5)  # not a valid `what`
+        srv.release(jid)
+        a = srv.submit(_job(rng, 30, 200, 400), M.SERVE_MSA)
+        srv.step()
+        srv.cancel(a)
+        srv.drain()
+        assert srv.job(a)["state"] == M.JOB_CANCELLED
+        assert srv.stats()["arena_used"] == 0
