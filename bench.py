@@ -29,6 +29,12 @@ import os
 import sys
 import time
 
+# One hardware queue per stream: the served form runs 13 streams (score
+# rounds, NJ and merge batches); with HIP's default of 4 queues, streams
+# sharing a queue serialise and a merge waits behind unrelated score rounds.
+# Read when the runtime starts, so set before the first HIP call.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -76,12 +82,18 @@ def cpu_baseline(seqs, target_seconds=12.0):
     }
 
 
-def served_bench(M, ctx, njobs=2000, load=0.75, seed=0, sets=()):
+def served_bench(M, ctx, njobs=2000, load=0.75, seed=0, sets=(), what=None, unloaded_every=4):
     """Served form on the job workload: njobs jobs, sizes power-law (exponent
-    1.5) on 8..512 sequences, each a tree-evolved family of lengths 100..2000.
-    Offline: every job submitted at once, drained. Unloaded: one job at a time.
-    Loaded: Poisson arrivals at `load` x the offline job rate; latency runs
-    from a job's scheduled arrival to the end of its last round."""
+    1.5) on 8..512 sequences, each a tree-evolved family of lengths 100..2000;
+    a job is the whole run (all-pairs scores, NJ tree, progressive alignment)
+    unless `what` says otherwise. Offline: every job submitted at once,
+    drained. Unloaded: one job at a time, every `unloaded_every`-th job (a
+    sample over all sizes, to bound the run time). Loaded: Poisson arrivals
+    at `load` x the offline job rate; latency runs from a job's scheduled
+    arrival to the end of its last stage."""
+    if what is None:
+        what = M.SERVE_MSA
+
     def note(msg):
         print("served: " + msg, file=sys.stderr, flush=True)
 
@@ -98,7 +110,7 @@ def served_bench(M, ctx, njobs=2000, load=0.75, seed=0, sets=()):
         srv.set(k, v)
     # offline
     t0 = time.perf_counter()
-    ids = [srv.submit(j) for j in jobs]
+    ids = [srv.submit(j, what) for j in jobs]
     srv.drain()
     t_off = time.perf_counter() - t0
     for jid in ids:
@@ -107,13 +119,13 @@ def served_bench(M, ctx, njobs=2000, load=0.75, seed=0, sets=()):
     note("offline %.2f s, %.3g cells/s estimate" % (t_off, srv.stats()["cells_per_s"]))
     # unloaded
     lat_u = []
-    for j in jobs:
-        jid = srv.submit(j)
+    for j in jobs[::unloaded_every]:
+        jid = srv.submit(j, what)
         srv.drain()
         info = srv.job(jid)
         lat_u.append(info["t_done"] - info["t_submit"])
         srv.release(jid)
-        if len(lat_u) % 500 == 0:
+        if len(lat_u) % 100 == 0:
             note("unloaded %d jobs" % len(lat_u))
     srv.completed(1 << 16)
     # loaded
@@ -121,18 +133,21 @@ def served_bench(M, ctx, njobs=2000, load=0.75, seed=0, sets=()):
     rate_off = njobs / t_off
     arr = np.cumsum(rng.exponential(1.0 / (load * rate_off), njobs))
     perm = rng.permutation(njobs)
-    k, done, lat, first_of = 0, 0, [], {}
+    k, done, lat, first_of, stage_s = 0, 0, [], {}, []
     t0m = time.monotonic()
     t_last = t0m
     while done < njobs:
         now = time.monotonic() - t0m
         while k < njobs and arr[k] <= now:
-            first_of[srv.submit(jobs[perm[k]])] = k
+            first_of[srv.submit(jobs[perm[k]], what)] = k
             k += 1
         srv.step()
         for jid in srv.completed():
             info = srv.job(jid)
             lat.append(info["t_done"] - (t0m + arr[first_of[jid]]))
+            tt = srv.times(jid)
+            stage_s.append((tt[2] - tt[0], (tt[3] - tt[2]) if tt[3] else 0.0,
+                            (tt[4] - (tt[3] or tt[2]))))
             t_last = max(t_last, info["t_done"])
             srv.release(jid)
             done += 1
@@ -141,9 +156,13 @@ def served_bench(M, ctx, njobs=2000, load=0.75, seed=0, sets=()):
     t_load = t_last - t0m
     srv.close()
     ms = lambda v, q: round(float(np.percentile(v, q)) * 1e3, 3)
+    stages = {M.SERVE_SCORES: "all-pairs scores", M.SERVE_TREE: "all-pairs scores + NJ tree",
+              M.SERVE_MSA: "all-pairs scores + NJ tree + progressive alignment"}[what]
     return {
         "workload": "%d jobs, sizes power-law 1.5 on 8..512, lengths 100..2000, %.3g cells (i<j)"
                     % (njobs, total_cells),
+        "job": stages,
+        "unloaded_sample": "every %d-th job (%d jobs)" % (unloaded_every, len(lat_u)),
         "offline_jobs_per_s": round(rate_off, 1),
         "offline_gcups": round(total_cells / t_off / 1e9, 1),
         "load": load,
@@ -153,6 +172,10 @@ def served_bench(M, ctx, njobs=2000, load=0.75, seed=0, sets=()):
         "p50_ms": ms(lat, 50), "p99_ms": ms(lat, 99),
         "unloaded_p50_ms": ms(lat_u, 50), "unloaded_p99_ms": ms(lat_u, 99),
         "p99_over_unloaded_p99": round(float(np.percentile(lat, 99) / np.percentile(lat_u, 99)), 3),
+        "loaded_stage_p50_ms": {name: ms([x[i] for x in stage_s], 50)
+                                for i, name in enumerate(("scores", "tree", "alignment"))},
+        "loaded_stage_p99_ms": {name: ms([x[i] for x in stage_s], 99)
+                                for i, name in enumerate(("scores", "tree", "alignment"))},
         "params": dict(sets),
     }
 
@@ -172,6 +195,9 @@ def main():
     ap.add_argument("--serve-jobs", type=int, default=2000)
     ap.add_argument("--serve-set", action="append", default=[], metavar="KEY=VALUE",
                     help="a parameter of the served form (mst_server_set), repeatable")
+    ap.add_argument("--serve-what", choices=("msa", "tree", "scores"), default="msa",
+                    help="what a served job computes (default: the whole run)")
+    ap.add_argument("--serve-only", action="store_true", help="only the served-form measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the all-pairs path runs on a single MI355X "
@@ -187,6 +213,14 @@ def main():
     self_cells = int((L * L).sum())
 
     ctx = M.Context(device=0)
+    if args.serve_only:
+        served = served_bench(
+            M, ctx, args.serve_jobs,
+            sets=[(kv.split("=", 1)[0], float(kv.split("=", 1)[1])) for kv in args.serve_set],
+            what={"msa": M.SERVE_MSA, "tree": M.SERVE_TREE, "scores": M.SERVE_SCORES}[args.serve_what])
+        print(json.dumps({"served": served}), flush=True)
+        ctx.close()
+        return 0
     tunes = {}
     for kv in args.tune:
         k, v = kv.split("=", 1)
@@ -277,7 +311,8 @@ def main():
         "tunables": dict(ctx.tuning, **tunes),
         "served": None if (args.no_serve or args.batched_only) else served_bench(
             M, ctx, args.serve_jobs,
-            sets=[(kv.split("=", 1)[0], float(kv.split("=", 1)[1])) for kv in args.serve_set]),
+            sets=[(kv.split("=", 1)[0], float(kv.split("=", 1)[1])) for kv in args.serve_set],
+            what={"msa": M.SERVE_MSA, "tree": M.SERVE_TREE, "scores": M.SERVE_SCORES}[args.serve_what]),
         "cpu_baseline": None if args.no_cpu else cpu_baseline(seqs),
     }
     print(json.dumps(line), flush=True)

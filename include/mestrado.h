@@ -117,12 +117,16 @@ typedef struct {
  * state; with an alignment, the leaf rows and 4*N*(maxlen+64) B of group
  * rows), and launches: rounds of score work items of the admitted jobs,
  * earliest deadline first, packed into shared launches of the batched form's
- * kernels; one NJ launch for the jobs whose scores are done (a workgroup per
- * job); one merge batch for the ready profile merges of all jobs in their
- * alignment stage. Results stay on the device until copied out; finished
- * jobs hold their memory until mst_server_result frees them. A job larger
+ * kernels; NJ launches for the jobs whose scores are done (a workgroup per
+ * job); merge batches of the ready profile merges of all jobs in their
+ * alignment stage (up to four NJ launches and four merge batches in
+ * flight). Results stay on the device until copied out; finished jobs hold
+ * their memory until mst_server_result frees them. A job larger
  * than the budget is rejected at submission. Jobs are independent: a job's
- * results are identical alone and in any mix. Not thread-safe. */
+ * results are identical alone and in any mix. Not thread-safe. The server
+ * uses 13 streams; give the process as many hardware queues
+ * (GPU_MAX_HW_QUEUES=16 in the environment before the first HIP call), or
+ * streams sharing a queue serialise. */
 typedef struct mst_server mst_server;
 enum { MST_JOB_WAITING = 0, MST_JOB_RUNNING = 1, MST_JOB_DONE = 2, MST_JOB_CANCELLED = 3,
        MST_JOB_REJECTED = 4 };
@@ -144,6 +148,10 @@ int mst_server_drain(mst_server* srv);
 /* State, size and host steady-clock times (seconds) of a job. */
 int mst_server_job(mst_server* srv, int64_t job, int* state, int* n, double* t_submit,
                    double* t_admit, double* t_done);
+/* Stage times of a job (host steady clock, seconds; 0 for a stage not
+ * reached): times[0] submission, [1] admission, [2] scores done, [3] tree
+ * done, [4] done. */
+int mst_server_job_times(mst_server* srv, int64_t job, double* times);
 /* Copies a done job's n x n scores out (may be NULL) and forgets the job:
  * take its tree and alignment first. */
 int mst_server_result(mst_server* srv, int64_t job, int32_t* scores_nxn);
@@ -163,8 +171,13 @@ int mst_server_stats(mst_server* srv, int* waiting, int* running, int* inflight,
  * "chunk" (column sequences per item, default 8), "slack" (a job's deadline
  * is its submission plus slack times its estimated run time, default 2;
  * jobs go earliest deadline first), "rate" (initial throughput estimate in
- * cells/s, for those estimates), "nj_ratio" (NJ rebuild ratio, default 0.5),
- * "merge_bytes" (scratch of one merge batch, default 1 GiB). */
+ * cells/s, for those estimates), "round_cells_shared" and "score_share"
+ * (while jobs are in their NJ or alignment stage: cells per round, default
+ * 2e10, and the share of the score kernels' grid, default 1; lower values let
+ * those stages' launches find free compute units sooner at some cost in
+ * throughput), "nj_ratio" (NJ rebuild ratio,
+ * default 0.5),
+ * "merge_bytes" (scratch of one merge batch, default 512 MiB). */
 int mst_server_set(mst_server* srv, const char* key, double value);
 /* Measurement helper (no counterpart in the reference): integer VALU rate of
  * the context's device in T ops/s, int32 (packed = 0) or packed int16

@@ -52,6 +52,7 @@ def lib():
         L.mst_server_submit.argtypes = [P, P, P, C.c_int, C.POINTER(C.c_int64)]
         L.mst_server_submit_job.argtypes = [P, P, P, C.c_int, C.c_int, C.POINTER(C.c_int64)]
         L.mst_server_tree.argtypes = [P, C.c_int64, P]
+        L.mst_server_job_times.argtypes = [P, C.c_int64, P]
         L.mst_server_msa.argtypes = [P, C.c_int64, P, C.c_int64, C.POINTER(C.c_int)]
         L.mst_server_step.argtypes = [P, C.POINTER(C.c_int)]
         L.mst_server_drain.argtypes = [P]
@@ -371,6 +372,13 @@ class Server:
                                             C.byref(ta), C.byref(td)), "mst_server_job")
         return {"state": st.value, "n": n.value, "t_submit": ts.value, "t_admit": ta.value,
                 "t_done": td.value}
+
+    def times(self, jid):
+        """Host steady-clock times of a job: submit, admit, scores done, tree
+        done, done (0 for a stage not reached)."""
+        t = np.zeros(5, dtype=np.float64)
+        self.ctx._ck(self._L.mst_server_job_times(self.h, jid, _p(t)), "mst_server_job_times")
+        return t
 
     def result(self, jid):
         n = self.job(jid)["n"]

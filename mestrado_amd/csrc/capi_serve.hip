@@ -6,10 +6,10 @@
 //  * score rounds: a quantum of work items drawn from the admitted jobs,
 //    earliest deadline first, packed into shared launches of the streamed
 //    packed kernels (nw_score5.hip, nw_score6.hip) whatever job they belong to;
-//  * one NJ launch (nj.hip, a workgroup per job) for the jobs whose scores
-//    are complete;
-//  * one merge batch (msa.hip) of the profile merges whose children exist,
-//    over every job in its alignment stage, earliest deadline first.
+//  * NJ launches (nj.hip, a workgroup per job) for the jobs whose scores are
+//    complete;
+//  * merge batches (msa.hip) of the profile merges whose children exist, over
+//    every job in its alignment stage, earliest deadline first.
 //
 // Device state: one arena of the budget's size holds, per admitted job, its
 // N x N int32 score matrix, its residue codes in length order and the ASCII
@@ -22,8 +22,9 @@
 // launches of different kernel classes run side by side on four side streams
 // (default priority) after the round's copies on the context's stream; a
 // round is over when every stream it used has passed its end event, and the
-// next round's copies do not wait for it. NJ and merge batches have a stream
-// each, one batch of each in flight.
+// next round's copies do not wait for it. NJ launches and merge batches run
+// on four streams each, one batch per stream in flight, so a long NJ or a
+// large merge does not hold back the small jobs behind it.
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -101,6 +102,7 @@ struct Job {
   std::vector<char> ascii;        // input letters
   std::vector<int64_t> off;       // input offsets (n + 1)
   double t_submit = 0, t_admit = 0, t_done = 0, deadline = 0;
+  double t_scores = 0, t_tree = 0;  // ends of the score and NJ stages
   // device placement
   int64_t arena_off = -1, arena_bytes = 0;  // one block: scores, codes, ascii, offsets, NJ, MSA
   int64_t o_codes = 0, o_D = 0, o_T = 0, o_act = 0, o_joins = 0, o_leaf = 0, o_msa = 0, msa_cap = 0;
@@ -181,30 +183,41 @@ struct mst_server {
   int64_t next_id = 1;
   double rate = 4e12;          // cells per second, running estimate
   double round_cells = 2e10;   // work per round (about 4 ms of the device)
+  double round_cells_shared = 2e10;  // per round while jobs are in later stages
+  double score_share = 1.0;          // share of the score kernels' grid meanwhile
   double slack = 2.0;          // deadline = submission + slack x estimated run time
   double t_retire = 0;         // when the last round was seen finished
   int chunk = 8;
   int64_t done_count = 0, cells_done = 0;
   std::vector<int64_t> completed;  // finished since the last mst_server_completed
-  // NJ: jobs whose scores are done, the batch in flight
+  // NJ: jobs whose scores are done; batches in flight (one per slot)
   float nj_ratio = 0.5f;
-  hipStream_t nj_st = nullptr;
-  hipEvent_t nj_ev = nullptr;
   std::deque<int64_t> nj_queue;
-  std::vector<int64_t> nj_batch;
-  std::vector<int64_t> nj_stage_off;  // joins of nj_batch[i] at h_nj + nj_stage_off[i]
-  NjArgs* d_nj_args = nullptr;
-  NjArgs* h_nj_args = nullptr;
-  NjJoin* h_nj = nullptr;
-  int64_t nj_args_cap = 0, nj_args_host_cap = 0, nj_cap = 0;
-  // merges: the batch in flight
-  double merge_bytes = (double)(1 << 30);
-  hipStream_t msa_st = nullptr;
-  hipEvent_t msa_ev = nullptr;
-  bool msa_busy = false;
-  mst::MergeEngine eng;
-  std::vector<std::pair<int64_t, int>> msa_batch;  // (job, join) of each merge
-  std::vector<int64_t> msa_leaf_jobs;              // jobs whose leaves the batch writes
+  struct NjSlot {
+    hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr;
+    bool busy = false;
+    std::vector<int64_t> jobs;
+    std::vector<int64_t> off;  // joins of jobs[i] at h_nj + off[i]
+    NjArgs* d_args = nullptr;
+    NjArgs* h_args = nullptr;
+    NjJoin* h_nj = nullptr;
+    int64_t args_cap = 0, hargs_cap = 0, nj_cap = 0;
+  };
+  static constexpr int kSlots = 4;
+  NjSlot nj[kSlots];
+  // merges: batches in flight (one per slot)
+  double merge_bytes = (double)(512 << 20);
+  struct MsaSlot {
+    hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr;
+    bool busy = false;
+    mst::MergeEngine eng;
+    std::vector<std::pair<int64_t, int>> merges;  // (job, join) of each merge
+    std::vector<int64_t> jobs;                    // jobs with work in the batch
+    std::vector<int64_t> leaf_jobs;               // jobs whose leaves the batch writes
+  };
+  MsaSlot ms[kSlots];
 };
 
 namespace {
@@ -492,7 +505,13 @@ int launch_round(mst_server* S, int* launched) {
   if (bi == mst_server::kBufs) return MST_OK;
   // a round buffer is rewritten only after the round that used it retired
   RoundBuf& B = S->rb[bi];
-  const int64_t budget = (int64_t)S->round_cells;
+  // while jobs are in their NJ or alignment stage, rounds are shorter and
+  // leave part of the device free, so those stages' launches do not wait
+  // behind a whole round of persistent score kernels
+  bool later = false;
+  for (int64_t id : S->running) later |= S->jobs[id].stage == kStageNj || S->jobs[id].stage == kStageMsa;
+  const int64_t budget = (int64_t)(later ? std::min(S->round_cells, S->round_cells_shared) : S->round_cells);
+  const double share = later ? S->score_share : 1.0;
   int64_t cells = 0;
   std::vector<std::vector<std::pair<Item3, int>>> by_cls(kClasses);
   std::vector<int2> fb;
@@ -575,10 +594,11 @@ int launch_round(mst_server* S, int* launched) {
     a.carry_cap = 16384;
     const int64_t nit = ranges[c].second - ranges[c].first;
     if (c == 0) {
-      launch_score6(a, (int)std::min<int64_t>(S->grid6, nit), st);
+      launch_score6(a, (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)(S->grid6 * share), nit)), st);
     } else {
       const int W = (c - 1) / 3 + 1, K = (c - 1) % 3 + 3;
-      const int64_t g = std::min<int64_t>((int64_t)ctx->num_cus * score5_groups_per_cu(K, W), nit);
+      const int64_t g =
+          std::min<int64_t>((int64_t)(ctx->num_cus * score5_groups_per_cu(K, W) * share), nit);
       launch_score5(K, W, a, (int)std::max<int64_t>(g, 1), st);
     }
     CK(hipGetLastError());
@@ -659,6 +679,7 @@ void start_msa(Job& J) {
 
 // The job's score rounds are over: on to NJ, the alignment or done.
 void scores_done(mst_server* S, Job& J, double t, int* ndone) {
+  J.t_scores = t;
   if ((J.what & MST_SERVE_TREE) == MST_SERVE_TREE && J.n >= 2) {
     J.stage = kStageNj;
     S->nj_queue.push_back(J.id);
@@ -736,12 +757,19 @@ int grow_buf(mst_ctx* ctx, T** p, int64_t* cap, int64_t count, bool pinned) {
   return MST_OK;
 }
 
-// One NJ launch for the jobs whose scores are done (none in flight): per job
+// One NJ launch for the jobs whose scores are done, on a free slot: per job
 // the fp32 matrix and NJ state in its block, then a workgroup each, then the
-// joins to pinned staging.
+// joins to the slot's pinned staging.
 int launch_nj_jobs(mst_server* S) {
   mst_ctx* ctx = S->ctx;
-  if (!S->nj_batch.empty() || S->nj_queue.empty()) return MST_OK;
+  if (S->nj_queue.empty()) return MST_OK;
+  mst_server::NjSlot* slot = nullptr;
+  for (auto& q : S->nj)
+    if (!q.busy) {
+      slot = &q;
+      break;
+    }
+  if (!slot) return MST_OK;
   std::vector<int64_t> ids;
   int64_t total = 0;
   while (!S->nj_queue.empty() && ids.size() < 4096) {
@@ -755,19 +783,19 @@ int launch_nj_jobs(mst_server* S) {
   if (ids.empty()) return MST_OK;
   const int cnt = (int)ids.size();
   int rc;
-  if ((rc = grow_buf(ctx, &S->d_nj_args, &S->nj_args_cap, cnt, false))) return rc;
-  if ((rc = grow_buf(ctx, &S->h_nj_args, &S->nj_args_host_cap, cnt, true))) return rc;
-  if ((rc = grow_buf(ctx, &S->h_nj, &S->nj_cap, std::max<int64_t>(total, 1), true))) return rc;
-  S->nj_stage_off.clear();
+  if ((rc = grow_buf(ctx, &slot->d_args, &slot->args_cap, cnt, false))) return rc;
+  if ((rc = grow_buf(ctx, &slot->h_args, &slot->hargs_cap, cnt, true))) return rc;
+  if ((rc = grow_buf(ctx, &slot->h_nj, &slot->nj_cap, std::max<int64_t>(total, 1), true))) return rc;
+  slot->off.clear();
   int64_t o = 0;
   for (int k = 0; k < cnt; ++k) {
     Job& J = S->jobs[ids[k]];
     const int n = J.n;
     float* D = (float*)(S->d_arena + J.o_D);
     int* act = (int*)(S->d_arena + J.o_act);
-    launch_scores_to_float((const int32_t*)(S->d_arena + J.arena_off), D, (int64_t)n * n, S->nj_st);
-    launch_nj_init(act, act + 2 * n, n, S->nj_st);
-    NjArgs& a = S->h_nj_args[k];
+    launch_scores_to_float((const int32_t*)(S->d_arena + J.arena_off), D, (int64_t)n * n, slot->st);
+    launch_nj_init(act, act + 2 * n, n, slot->st);
+    NjArgs& a = slot->h_args[k];
     a.D = D;
     a.ld = n;
     a.n0 = n;
@@ -776,130 +804,158 @@ int launch_nj_jobs(mst_server* S) {
     a.ids = act + 2 * n;
     a.joins = (NjJoin*)(S->d_arena + J.o_joins);
     a.ratio = S->nj_ratio;
-    S->nj_stage_off.push_back(o);
+    slot->off.push_back(o);
     o += n - 1;
     J.refs += 1;
   }
   CK(hipGetLastError());
-  CK(hipMemcpyAsync(S->d_nj_args, S->h_nj_args, sizeof(NjArgs) * cnt, hipMemcpyHostToDevice, S->nj_st));
-  CK(launch_nj_batch(S->d_nj_args, cnt, S->nj_st));
+  CK(hipMemcpyAsync(slot->d_args, slot->h_args, sizeof(NjArgs) * cnt, hipMemcpyHostToDevice, slot->st));
+  CK(launch_nj_batch(slot->d_args, cnt, slot->st));
   for (int k = 0; k < cnt; ++k) {
     Job& J = S->jobs[ids[k]];
-    CK(hipMemcpyAsync(S->h_nj + S->nj_stage_off[k], S->d_arena + J.o_joins, sizeof(NjJoin) * (J.n - 1),
-                      hipMemcpyDeviceToHost, S->nj_st));
+    CK(hipMemcpyAsync(slot->h_nj + slot->off[k], S->d_arena + J.o_joins, sizeof(NjJoin) * (J.n - 1),
+                      hipMemcpyDeviceToHost, slot->st));
   }
-  CK(hipEventRecord(S->nj_ev, S->nj_st));
-  S->nj_batch.swap(ids);
+  CK(hipEventRecord(slot->ev, slot->st));
+  slot->jobs.swap(ids);
+  slot->busy = true;
   return MST_OK;
 }
 
+// Retires the finished NJ batches (wait: block on the first busy one).
 int retire_nj(mst_server* S, bool wait, int* ndone) {
   mst_ctx* ctx = S->ctx;
-  if (S->nj_batch.empty()) return MST_OK;
-  hipError_t e = wait ? hipEventSynchronize(S->nj_ev) : hipEventQuery(S->nj_ev);
-  if (e == hipErrorNotReady) return MST_OK;
-  if (e != hipSuccess) return ctx->fail(MST_E_HIP, "NJ batch", hipGetErrorString(e));
-  const double t = now_s();
-  for (size_t k = 0; k < S->nj_batch.size(); ++k) {
-    Job& J = S->jobs[S->nj_batch[k]];
-    J.refs -= 1;
-    if (J.cancel) continue;
-    J.tree.resize(J.n - 1);
-    for (int q = 0; q + 1 < J.n; ++q) nj_to_join(S->h_nj[S->nj_stage_off[k] + q], &J.tree[q]);
-    if ((J.what & MST_SERVE_MSA) == MST_SERVE_MSA)
-      start_msa(J);
-    else
-      job_done(S, J, t, ndone);
+  for (auto& q : S->nj) {
+    if (!q.busy) continue;
+    hipError_t e = wait ? hipEventSynchronize(q.ev) : hipEventQuery(q.ev);
+    if (e == hipErrorNotReady) continue;
+    if (e != hipSuccess) return ctx->fail(MST_E_HIP, "NJ batch", hipGetErrorString(e));
+    wait = false;
+    const double t = now_s();
+    for (size_t k = 0; k < q.jobs.size(); ++k) {
+      Job& J = S->jobs[q.jobs[k]];
+      J.refs -= 1;
+      if (J.cancel) continue;
+      J.tree.resize(J.n - 1);
+      for (int r = 0; r + 1 < J.n; ++r) nj_to_join(q.h_nj[q.off[k] + r], &J.tree[r]);
+      J.t_tree = t;
+      if ((J.what & MST_SERVE_MSA) == MST_SERVE_MSA)
+        start_msa(J);
+      else
+        job_done(S, J, t, ndone);
+    }
+    q.jobs.clear();
+    q.busy = false;
   }
-  S->nj_batch.clear();
   prune_running(S);
   return MST_OK;
 }
 
-// One merge batch (none in flight): the leaves of jobs new to the alignment
-// stage, then every merge whose children exist, jobs earliest deadline
-// first, while the batch's scratch stays within merge_bytes. A merge's rows
-// go to its job's region, else the arena; a merge with no room waits.
+bool nj_busy(const mst_server* S) {
+  for (const auto& q : S->nj)
+    if (q.busy) return true;
+  return false;
+}
+
+bool msa_busy(const mst_server* S) {
+  for (const auto& q : S->ms)
+    if (q.busy) return true;
+  return false;
+}
+
+// Merge batches on the free slots: a batch writes the leaves of jobs new to
+// the alignment stage, then every merge whose children exist, jobs earliest
+// deadline first, while its scratch stays within merge_bytes. A merge's rows
+// go to its job's region, else the arena; a merge with no room waits. Groups
+// a batch writes (leaves included) are ready for other batches once it is
+// retired; within the batch, stream order makes leaves ready for its merges.
 int launch_merges(mst_server* S) {
   mst_ctx* ctx = S->ctx;
-  if (S->msa_busy) return MST_OK;
-  std::vector<int64_t> order;
-  for (int64_t id : S->running) {
-    Job& J = S->jobs[id];
-    if (J.state == MST_JOB_RUNNING && !J.cancel && J.stage == kStageMsa) order.push_back(id);
+  for (auto& slot : S->ms) {
+    if (slot.busy) continue;
+    std::vector<int64_t> order;
+    for (int64_t id : S->running) {
+      Job& J = S->jobs[id];
+      if (J.state == MST_JOB_RUNNING && !J.cancel && J.stage == kStageMsa) order.push_back(id);
+    }
+    if (order.empty()) return MST_OK;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t x, int64_t y) { return S->jobs[x].deadline < S->jobs[y].deadline; });
+    std::vector<LeafDesc> leaves;
+    std::vector<MergeSpec> spec;
+    slot.merges.clear();
+    slot.jobs.clear();
+    slot.leaf_jobs.clear();
+    const int64_t budget = (int64_t)S->merge_bytes;
+    int64_t bytes = 0;
+    bool full = false;
+    for (int64_t id : order) {
+      Job& J = S->jobs[id];
+      bool touch = false;
+      if (!J.leaves_done) {
+        for (int i = 0; i < J.n; ++i) {
+          Group& G = J.groups[i];
+          LeafDesc l;
+          l.src = J.roff[J.where[i]] + J.o_codes;
+          l.dst = S->d_arena + G.off;
+          l.len = G.len;
+          l.pad = 0;
+          leaves.push_back(l);
+          G.ready = true;  // for this batch's merges only; reset below
+        }
+        J.leaves_done = true;
+        slot.leaf_jobs.push_back(id);
+        touch = true;
+      }
+      for (int k = 0; k + 1 < J.n && !full; ++k) {
+        if (J.merge_state[k] != 0) continue;
+        const mst_join& jn = J.tree[k];
+        Group& X = J.groups[jn.left];
+        Group& Y = J.groups[jn.right];
+        if (!X.ready || !Y.ready) continue;
+        const int64_t b = merge_scratch_bytes(X.len, Y.len);
+        if (!spec.empty() && bytes + b > budget) {
+          full = true;
+          break;
+        }
+        Group& Z = J.groups[jn.node];
+        const int64_t cap = std::max<int64_t>((int64_t)(X.k + Y.k) * ((int64_t)X.len + Y.len), 1);
+        int64_t off = J.msa_pool.alloc(cap);
+        bool in_arena = false;
+        if (off >= 0) {
+          off += J.o_msa;
+        } else {
+          off = S->arena.alloc(cap);
+          in_arena = true;
+          if (off < 0) continue;  // no room now: waits for memory
+        }
+        Z.off = off;
+        Z.cap = cap;
+        Z.in_arena = in_arena;
+        Z.k = X.k + Y.k;
+        bytes += b;
+        J.merge_state[k] = 1;
+        spec.push_back(MergeSpec{S->d_arena + X.off, X.k, X.len, S->d_arena + Y.off, Y.k, Y.len,
+                                 S->d_arena + Z.off});
+        slot.merges.push_back(std::make_pair(id, k));
+        touch = true;
+      }
+      if (touch) {
+        slot.jobs.push_back(id);
+        J.refs += 1;
+      }
+      if (full) break;
+    }
+    for (int64_t id : slot.leaf_jobs) {
+      Job& J = S->jobs[id];
+      for (int i = 0; i < J.n; ++i) J.groups[i].ready = false;
+    }
+    if (slot.jobs.empty()) return MST_OK;
+    CK(slot.eng.leaves((const int8_t*)S->d_arena, leaves.data(), (int)leaves.size(), slot.st));
+    CK(slot.eng.enqueue(spec.data(), (int)spec.size(), ctx->d_S, ctx->go, ctx->ge, ctx->num_cus, slot.st));
+    CK(hipEventRecord(slot.ev, slot.st));
+    slot.busy = true;
   }
-  if (order.empty()) return MST_OK;
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int64_t x, int64_t y) { return S->jobs[x].deadline < S->jobs[y].deadline; });
-  std::vector<LeafDesc> leaves;
-  std::vector<MergeSpec> spec;
-  std::vector<int64_t> touched;
-  S->msa_batch.clear();
-  const int64_t budget = (int64_t)S->merge_bytes;
-  int64_t bytes = 0;
-  bool full = false;
-  for (int64_t id : order) {
-    Job& J = S->jobs[id];
-    bool touch = false;
-    if (!J.leaves_done) {
-      for (int i = 0; i < J.n; ++i) {
-        Group& G = J.groups[i];
-        LeafDesc l;
-        l.src = J.roff[J.where[i]] + J.o_codes;
-        l.dst = S->d_arena + G.off;
-        l.len = G.len;
-        l.pad = 0;
-        leaves.push_back(l);
-        G.ready = true;  // written ahead of every merge on the merge stream
-      }
-      J.leaves_done = true;
-      touch = true;
-    }
-    for (int k = 0; k + 1 < J.n && !full; ++k) {
-      if (J.merge_state[k] != 0) continue;
-      const mst_join& jn = J.tree[k];
-      Group& X = J.groups[jn.left];
-      Group& Y = J.groups[jn.right];
-      if (!X.ready || !Y.ready) continue;
-      const int64_t b = merge_scratch_bytes(X.len, Y.len);
-      if (!spec.empty() && bytes + b > budget) {
-        full = true;
-        break;
-      }
-      Group& Z = J.groups[jn.node];
-      const int64_t cap = std::max<int64_t>((int64_t)(X.k + Y.k) * ((int64_t)X.len + Y.len), 1);
-      int64_t off = J.msa_pool.alloc(cap);
-      bool in_arena = false;
-      if (off >= 0) {
-        off += J.o_msa;
-      } else {
-        off = S->arena.alloc(cap);
-        in_arena = true;
-        if (off < 0) continue;  // no room now: waits for memory
-      }
-      Z.off = off;
-      Z.cap = cap;
-      Z.in_arena = in_arena;
-      Z.k = X.k + Y.k;
-      bytes += b;
-      J.merge_state[k] = 1;
-      spec.push_back(MergeSpec{S->d_arena + X.off, X.k, X.len, S->d_arena + Y.off, Y.k, Y.len,
-                               S->d_arena + Z.off});
-      S->msa_batch.push_back(std::make_pair(id, k));
-      touch = true;
-    }
-    if (touch) {
-      touched.push_back(id);
-      J.refs += 1;
-    }
-    if (full) break;
-  }
-  if (touched.empty()) return MST_OK;
-  CK(S->eng.leaves((const int8_t*)S->d_arena, leaves.data(), (int)leaves.size(), S->msa_st));
-  CK(S->eng.enqueue(spec.data(), (int)spec.size(), ctx->d_S, ctx->go, ctx->ge, ctx->num_cus, S->msa_st));
-  CK(hipEventRecord(S->msa_ev, S->msa_st));
-  S->msa_leaf_jobs.swap(touched);
-  S->msa_busy = true;
   return MST_OK;
 }
 
@@ -913,40 +969,56 @@ void release_group(mst_server* S, Job& J, Group& G) {
   G.cap = 0;
 }
 
+// Retires the finished merge batches (wait: block on the first busy one).
 int retire_merges(mst_server* S, bool wait, int* ndone) {
   mst_ctx* ctx = S->ctx;
-  if (!S->msa_busy) return MST_OK;
-  hipError_t e = wait ? hipEventSynchronize(S->msa_ev) : hipEventQuery(S->msa_ev);
-  if (e == hipErrorNotReady) return MST_OK;
-  if (e != hipSuccess) return ctx->fail(MST_E_HIP, "merge batch", hipGetErrorString(e));
-  const double t = now_s();
-  const int32_t* lens = S->eng.lens();
-  for (size_t b = 0; b < S->msa_batch.size(); ++b) {
-    Job& J = S->jobs[S->msa_batch[b].first];
-    const int k = S->msa_batch[b].second;
-    const mst_join& jn = J.tree[k];
-    Group& Z = J.groups[jn.node];
-    Z.len = lens[b];
-    const int64_t keep = std::max<int64_t>((int64_t)Z.k * Z.len, 1);
-    if (Z.in_arena)
-      S->arena.shrink(Z.off, Z.cap, keep);
-    else
-      J.msa_pool.shrink(Z.off - J.o_msa, Z.cap, keep);
-    Z.cap = keep;
-    Z.ready = true;
-    J.merge_state[k] = 2;
-    J.merges_done += 1;
-    release_group(S, J, J.groups[jn.left]);
-    release_group(S, J, J.groups[jn.right]);
+  for (auto& slot : S->ms) {
+    if (!slot.busy) continue;
+    hipError_t e = wait ? hipEventSynchronize(slot.ev) : hipEventQuery(slot.ev);
+    if (e == hipErrorNotReady) continue;
+    if (e != hipSuccess) return ctx->fail(MST_E_HIP, "merge batch", hipGetErrorString(e));
+    wait = false;
+    const double t = now_s();
+    const int32_t* lens = slot.eng.lens();
+    for (size_t b = 0; b < slot.merges.size(); ++b) {
+      Job& J = S->jobs[slot.merges[b].first];
+      const int k = slot.merges[b].second;
+      const mst_join& jn = J.tree[k];
+      Group& Z = J.groups[jn.node];
+      Z.len = lens[b];
+      const int64_t keep = std::max<int64_t>((int64_t)Z.k * Z.len, 1);
+      if (Z.in_arena)
+        S->arena.shrink(Z.off, Z.cap, keep);
+      else
+        J.msa_pool.shrink(Z.off - J.o_msa, Z.cap, keep);
+      Z.cap = keep;
+      Z.ready = true;
+      J.merge_state[k] = 2;
+      J.merges_done += 1;
+      release_group(S, J, J.groups[jn.left]);
+      release_group(S, J, J.groups[jn.right]);
+    }
+    for (int64_t id : slot.leaf_jobs) {
+      Job& J = S->jobs[id];
+      for (int i = 0; i < J.n; ++i) J.groups[i].ready = true;
+    }
+    for (int64_t id : slot.jobs) {
+      Job& J = S->jobs[id];
+      J.refs -= 1;
+      if (!J.cancel && J.state == MST_JOB_RUNNING && J.merges_done == J.n - 1) {
+        // done once every batch that wrote its groups has retired
+        bool pending = false;
+        for (const auto& o : S->ms)
+          if (&o != &slot && o.busy && std::find(o.jobs.begin(), o.jobs.end(), id) != o.jobs.end())
+            pending = true;
+        if (!pending) job_done(S, J, t, ndone);
+      }
+    }
+    slot.merges.clear();
+    slot.jobs.clear();
+    slot.leaf_jobs.clear();
+    slot.busy = false;
   }
-  for (int64_t id : S->msa_leaf_jobs) {
-    Job& J = S->jobs[id];
-    J.refs -= 1;
-    if (!J.cancel && J.state == MST_JOB_RUNNING && J.merges_done == J.n - 1) job_done(S, J, t, ndone);
-  }
-  S->msa_batch.clear();
-  S->msa_leaf_jobs.clear();
-  S->msa_busy = false;
   prune_running(S);
   return MST_OK;
 }
@@ -980,10 +1052,14 @@ extern "C" int mst_server_create(mst_ctx* ctx, int64_t budget_bytes, mst_server*
     return fail("bufs", e);
   for (auto& st : S->side)
     if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess) return fail("stream", e);
-  if ((e = hipStreamCreateWithFlags(&S->nj_st, hipStreamNonBlocking)) != hipSuccess) return fail("stream", e);
-  if ((e = hipStreamCreateWithFlags(&S->msa_st, hipStreamNonBlocking)) != hipSuccess) return fail("stream", e);
-  if ((e = hipEventCreateWithFlags(&S->nj_ev, hipEventDisableTiming)) != hipSuccess) return fail("event", e);
-  if ((e = hipEventCreateWithFlags(&S->msa_ev, hipEventDisableTiming)) != hipSuccess) return fail("event", e);
+  for (auto& q : S->nj) {
+    if ((e = hipStreamCreateWithFlags(&q.st, hipStreamNonBlocking)) != hipSuccess) return fail("stream", e);
+    if ((e = hipEventCreateWithFlags(&q.ev, hipEventDisableTiming)) != hipSuccess) return fail("event", e);
+  }
+  for (auto& q : S->ms) {
+    if ((e = hipStreamCreateWithFlags(&q.st, hipStreamNonBlocking)) != hipSuccess) return fail("stream", e);
+    if ((e = hipEventCreateWithFlags(&q.ev, hipEventDisableTiming)) != hipSuccess) return fail("event", e);
+  }
   S->arena.init(budget_bytes, 256);
   S->pos.init(max_pos, 1);
   S->slots.init(max_slots, 1);
@@ -998,13 +1074,17 @@ extern "C" int mst_server_destroy(mst_server* S) {
   hipStreamSynchronize(ctx->stream);
   for (hipStream_t st : S->side)
     if (st) hipStreamSynchronize(st);
-  for (hipStream_t st : {S->nj_st, S->msa_st})
-    if (st) hipStreamSynchronize(st);
-  for (hipEvent_t ev : {S->nj_ev, S->msa_ev})
-    if (ev) hipEventDestroy(ev);
-  if (S->d_nj_args) hipFree(S->d_nj_args);
-  if (S->h_nj_args) hipHostFree(S->h_nj_args);
-  if (S->h_nj) hipHostFree(S->h_nj);
+  for (auto& q : S->nj) {
+    if (q.st) hipStreamSynchronize(q.st);
+    if (q.ev) hipEventDestroy(q.ev);
+    if (q.d_args) hipFree(q.d_args);
+    if (q.h_args) hipHostFree(q.h_args);
+    if (q.h_nj) hipHostFree(q.h_nj);
+  }
+  for (auto& q : S->ms) {
+    if (q.st) hipStreamSynchronize(q.st);
+    if (q.ev) hipEventDestroy(q.ev);
+  }
   for (RoundBuf& B : S->rb) {
     hipEvent_t evs[] = {B.ev_start, B.ev_end, B.ev_go, B.ev_side[0], B.ev_side[1], B.ev_side[2],
                         B.ev_side[3]};
@@ -1022,8 +1102,10 @@ extern "C" int mst_server_destroy(mst_server* S) {
     if (p) hipFree(p);
   for (hipStream_t st : S->side)
     if (st) hipStreamDestroy(st);
-  for (hipStream_t st : {S->nj_st, S->msa_st})
-    if (st) hipStreamDestroy(st);
+  for (auto& q : S->nj)
+    if (q.st) hipStreamDestroy(q.st);
+  for (auto& q : S->ms)
+    if (q.st) hipStreamDestroy(q.st);
   delete S;
   return MST_OK;
 }
@@ -1108,9 +1190,9 @@ extern "C" int mst_server_drain(mst_server* S) {
     int d = 0;
     if (!S->inflight.empty()) {
       if ((rc = retire(S, true, &d))) return rc;
-    } else if (!S->nj_batch.empty()) {
+    } else if (nj_busy(S)) {
       if ((rc = retire_nj(S, true, &d))) return rc;
-    } else if (S->msa_busy) {
+    } else if (msa_busy(S)) {
       if ((rc = retire_merges(S, true, &d))) return rc;
     } else if (S->waiting.empty() && S->running.empty()) {
       return MST_OK;
@@ -1135,6 +1217,19 @@ extern "C" int mst_server_job(mst_server* S, int64_t job, int* state, int* n, do
   if (t_submit) *t_submit = J.t_submit;
   if (t_admit) *t_admit = J.t_admit;
   if (t_done) *t_done = J.t_done;
+  return MST_OK;
+}
+
+extern "C" int mst_server_job_times(mst_server* S, int64_t job, double* times) {
+  if (!S || !times) return MST_E_ARG;
+  auto it = S->jobs.find(job);
+  if (it == S->jobs.end()) return S->ctx->fail(MST_E_ARG, "mst_server_job_times", "unknown job");
+  const Job& J = it->second;
+  times[0] = J.t_submit;
+  times[1] = J.t_admit;
+  times[2] = J.t_scores;
+  times[3] = J.t_tree;
+  times[4] = J.t_done;
   return MST_OK;
 }
 
@@ -1250,6 +1345,14 @@ extern "C" int mst_server_set(mst_server* S, const char* key, double value) {
   }
   if (!strcmp(key, "rate") && value > 0) {
     S->rate = value;
+    return MST_OK;
+  }
+  if (!strcmp(key, "round_cells_shared") && value >= 1e6 && value <= 1e14) {
+    S->round_cells_shared = value;
+    return MST_OK;
+  }
+  if (!strcmp(key, "score_share") && value > 0 && value <= 1) {
+    S->score_share = value;
     return MST_OK;
   }
   if (!strcmp(key, "nj_ratio") && value >= 0 && value <= 1) {

@@ -166,34 +166,54 @@ __global__ void mb_table_kernel(const MergeDesc* __restrict__ d, const Prefix* _
                   go_plus_ge, s, t, lane);
 }
 
-// Source column of every output column, one thread per merge: sx[t] = X
-// column or -1 (gap in X), same for Y; lens[i] = path length.
-__global__ void mb_path_kernel(const MergeDesc* __restrict__ d, int count,
-                               const int32_t* __restrict__ tlens, uint8_t* __restrict__ scratch,
-                               int32_t* __restrict__ lens) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
+// Source column of every output column: sx[t] = X column or -1 (gap in X),
+// same for Y; lens[i] = path length. One workgroup per merge scans the path
+// in chunks of 256 steps (wave ballots, then the waves' totals).
+constexpr int kPathThreads = 256;
+
+__global__ void __launch_bounds__(kPathThreads) mb_path_kernel(const MergeDesc* __restrict__ d,
+                                                               const int32_t* __restrict__ tlens,
+                                                               uint8_t* __restrict__ scratch,
+                                                               int32_t* __restrict__ lens) {
+  __shared__ int wx[kPathThreads / MST_WAVE], wy[kPathThreads / MST_WAVE];
+  const int i = blockIdx.x;
   const MergeDesc m = d[i];
   int32_t* sx = (int32_t*)(scratch + m.sxy);
   int32_t* sy = sx + (m.nX + m.nY);
-  int L;
-  if (m.job < 0) {  // one side empty: the other side's columns, all gaps opposite
-    L = m.nX + m.nY;
-    for (int t = 0; t < L; ++t) {
-      sx[t] = m.nX > 0 ? t : -1;
-      sy[t] = m.nX > 0 ? -1 : t;
+  const uint8_t* ops = scratch + m.ops;
+  const int L = m.job < 0 ? m.nX + m.nY : tlens[m.job];
+  const int lane = lane_id(), wv = threadIdx.x / MST_WAVE;
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  int bx = 0, by = 0;  // X and Y columns consumed before the chunk
+  for (int t0 = 0; t0 < L; t0 += kPathThreads) {
+    const int t = t0 + (int)threadIdx.x;
+    int op = -1;
+    if (t < L) op = m.job < 0 ? (m.nX > 0 ? OP_LEFT : OP_UP) : ops[t];
+    const bool ax = op >= 0 && op != OP_UP, ay = op >= 0 && op != OP_LEFT;
+    const uint64_t mx = __ballot(ax), my = __ballot(ay);
+    if (lane == 0) {
+      wx[wv] = __popcll(mx);
+      wy[wv] = __popcll(my);
     }
-  } else {
-    L = tlens[m.job];
-    const uint8_t* ops = scratch + m.ops;
-    int x = 0, y = 0;
-    for (int t = 0; t < L; ++t) {
-      const int op = ops[t];
-      sx[t] = (op != OP_UP) ? x++ : -1;
-      sy[t] = (op != OP_LEFT) ? y++ : -1;
+    __syncthreads();
+    int ox = bx, oy = by, tx = 0, ty = 0;
+    for (int w = 0; w < kPathThreads / MST_WAVE; ++w) {
+      if (w < wv) {
+        ox += wx[w];
+        oy += wy[w];
+      }
+      tx += wx[w];
+      ty += wy[w];
     }
+    if (t < L) {
+      sx[t] = ax ? ox + __popcll(mx & below) : -1;
+      sy[t] = ay ? oy + __popcll(my & below) : -1;
+    }
+    bx += tx;
+    by += ty;
+    __syncthreads();  // totals are rewritten by the next chunk
   }
-  lens[i] = L;
+  if (threadIdx.x == 0) lens[i] = L;
 }
 
 // One workgroup per output row: X's kX rows then Y's kY rows, L columns.
@@ -422,7 +442,7 @@ hipError_t MergeEngine::enqueue(const MergeSpec* m, int count, const int8_t* d_S
     t.lens = d_res_ + njobs;
     launch_traceback(t, st);
   }
-  hipLaunchKernelGGL(mb_path_kernel, dim3((count + 63) / 64), dim3(64), 0, st, dd, count,
+  hipLaunchKernelGGL(mb_path_kernel, dim3(count), dim3(kPathThreads), 0, st, dd,
                      (const int32_t*)(d_res_ + njobs), d_scratch_, d_lens_);
   if (row0 > 0)
     hipLaunchKernelGGL(mb_rows_kernel, dim3((unsigned)row0), dim3(256), 0, st, dd, count,

@@ -538,56 +538,96 @@ __global__ void __launch_bounds__(MST_WAVE) align_kernel(AlignArgs a) {
   }
 }
 
-// ---- traceback: one lane per job ----
+// ---- traceback: one wave per job ----
+//
+// The walk (NeedlemanWunsch.cu:409-468) is one dependent chain of cell reads,
+// so its cost is the latency of each read. A wave keeps a tile of the flags
+// in LDS: 64 columns by up to 64 rows (8 lanes' dwords of one strip) whose
+// top-right cell is the walk's position when the tile is loaded; each lane
+// loads one column (8 dwords, independent loads). The walk then reads LDS
+// (every lane reads the same word, a broadcast) until it leaves the tile,
+// typically after 64 steps, and the next tile is loaded. Every lane runs the
+// walk (uniform control flow); lane 0 writes the path.
 
-__device__ __forceinline__ int nib_at(const uint32_t* fl, int n, int x, int y) {
+constexpr int TB_GROUPS = 8;  // 8-row groups of a tile
+
+__device__ __forceinline__ int boundary_nib(int x, int y) {
   if (x < 0 && y < 0) return 0;
   if (y < 0) return 9;  // LEFT + contLeft
-  if (x < 0) return 6;  // UP + contUp
-  const int Tn = n + MST_WAVE - 1;
-  const int s = y / ASH, rl = y - s * ASH;
-  const int lane = rl / AK, i = rl - lane * AK;
-  const int t = x + lane;
-  const uint32_t w = fl[((int64_t)s * Tn + t) * MST_WAVE + lane];
-  return (int)((w >> (4 * i)) & 15u);
+  return 6;             // UP + contUp
 }
 
-__global__ void traceback_kernel(TraceArgs a) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x + a.first;
-  if (j >= a.last) return;
-  const AlignJob job = a.jobs[j];
-  const int n = job.n, m = job.m;
-  const uint32_t* fl = a.flags + job.flag_off;
-  uint8_t* out = a.ops + job.ops_off;
-  int x = n - 1, y = m - 1;
-  int prev = 12;
-  int cur = nib_at(fl, n, x, y);
-  int k = 0;
-  while (cur & 12) {
-    int todo;
-    if ((prev & 12) == 4 && (prev & 2))
-      todo = 4;
-    else if ((prev & 12) == 8 && (prev & 1))
-      todo = 8;
-    else if ((cur & 12) == 4)
-      todo = 4;
-    else if ((cur & 12) == 8)
-      todo = 8;
-    else
-      todo = 12;
-    out[k++] = (uint8_t)(todo == 12 ? OP_DIAG : (todo == 8 ? OP_LEFT : OP_UP));
-    if (todo == 8)
-      --x;
-    else if (todo == 4)
-      --y;
-    else {
-      --x;
-      --y;
+__global__ void __launch_bounds__(MST_WAVE) traceback_kernel(TraceArgs a) {
+  __shared__ uint32_t tile[TB_GROUPS][MST_WAVE];
+  const int lane = lane_id();
+  for (int64_t j = (int64_t)blockIdx.x + a.first; j < a.last; j += gridDim.x) {
+    const AlignJob job = a.jobs[j];
+    const int n = job.n, m = job.m;
+    const int Tn = n + MST_WAVE - 1;
+    const uint32_t* fl = a.flags + job.flag_off;
+    uint8_t* out = a.ops + job.ops_off;
+    // tile: columns [cx0, cx0 + 63], rows [cy0, cy1] (groups from cy0 / 8)
+    int cx0 = 1, cy0 = 1, cy1 = 0;
+    auto nib = [&](int x, int y) -> int {
+      if (x < 0 || y < 0) return boundary_nib(x, y);
+      if (x < cx0 || x > cx0 + MST_WAVE - 1 || y < cy0 || y > cy1) {
+        const int s = y / ASH;
+        const int gs = s * (ASH / AK);  // first group of the strip
+        const int g = y / AK;
+        const int g0 = max(gs, g - (TB_GROUPS - 1));
+        const int c = x - (MST_WAVE - 1) + lane;
+        __syncthreads();  // the previous tile has been read
+#pragma unroll
+        for (int gi = 0; gi < TB_GROUPS; ++gi) {
+          const int G = g0 + gi;
+          uint32_t v = 0;
+          if (G <= g && c >= 0) {
+            const int lg = G - gs;  // lane of the flag kernel holding group G
+            v = fl[((int64_t)s * Tn + c + lg) * MST_WAVE + lg];
+          }
+          tile[gi][lane] = v;
+        }
+        __syncthreads();
+        cx0 = x - (MST_WAVE - 1);
+        cy0 = g0 * AK;
+        cy1 = g * AK + AK - 1;
+      }
+      const int gi = y / AK - cy0 / AK;
+      const uint32_t w = tile[gi][x - cx0];
+      return (int)((w >> (4 * (y % AK))) & 15u);
+    };
+    int x = n - 1, y = m - 1;
+    int prev = 12;
+    int cur = nib(x, y);
+    int k = 0;
+    while (cur & 12) {
+      int todo;
+      if ((prev & 12) == 4 && (prev & 2))
+        todo = 4;
+      else if ((prev & 12) == 8 && (prev & 1))
+        todo = 8;
+      else if ((cur & 12) == 4)
+        todo = 4;
+      else if ((cur & 12) == 8)
+        todo = 8;
+      else
+        todo = 12;
+      if (lane == 0) out[k] = (uint8_t)(todo == 12 ? OP_DIAG : (todo == 8 ? OP_LEFT : OP_UP));
+      ++k;
+      if (todo == 8)
+        --x;
+      else if (todo == 4)
+        --y;
+      else {
+        --x;
+        --y;
+      }
+      prev = todo | (cur & 3);
+      cur = nib(x, y);
     }
-    prev = todo | (cur & 3);
-    cur = nib_at(fl, n, x, y);
+    if (lane == 0) a.lens[j] = k;
+    __syncthreads();  // the tile is rebuilt for the next job
   }
-  a.lens[j] = k;
 }
 
 void launch_align(int src, const AlignArgs& a, int grid, hipStream_t st) {
@@ -611,7 +651,8 @@ int64_t align_strips(int m) { return m <= 0 ? 1 : (m + ASH - 1) / ASH; }
 void launch_traceback(const TraceArgs& a, hipStream_t st) {
   const int64_t n = a.last - a.first;
   if (n <= 0) return;
-  hipLaunchKernelGGL(traceback_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, a);
+  const int64_t grid = n < 65536 ? n : 65536;
+  hipLaunchKernelGGL(traceback_kernel, dim3((unsigned)grid), dim3(MST_WAVE), 0, st, a);
 }
 
 int64_t flag_words(int n, int m) {
