@@ -18,8 +18,12 @@ roofline   the dominant launch's integer VALU rate: 7 algorithmic int ops per
 cpu_baseline  the oracle recurrence (oracle/liboracle.so, OpenMP over pairs)
            on a bounded random sample of the same pairs
 
-served     the served form (continuous batching) on the job workload: offline
-           and loaded throughput, p50/p99 latency loaded and unloaded
+traceback  the alignment form (flag kernels + traceback) on 20,000 random pairs
+           of the set: flag bytes written (0.5 B per cell) per second of the
+           flag kernels against HBM's 8 TB/s, and the traceback's time
+served     the served form (continuous batching) on the job workload, a job
+           being the whole run (scores, NJ tree, alignment): offline and loaded
+           throughput, p50/p99 latency loaded and unloaded
 
 Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu] [--no-serve]
 """
@@ -79,6 +83,48 @@ def cpu_baseline(seqs, target_seconds=12.0):
         "kind": "port",
         "sample": "%d random pairs (i != j) of the same 4,000-sequence set, %.3g cells, %.1f s wall"
                   % (len(pi), cells, secs),
+    }
+
+
+def traceback_bench(M, ctx, st, L, npairs=20000, seed=7):
+    """The alignment form (flag kernels + traceback, mst_align_pairs) on random
+    pairs of the same set: device time of the flag kernels and of the
+    traceback from HIP events, and the flag bytes they write: 0.5 B per cell
+    (a nibble: direction and two continuation bits), against HBM's 8 TB/s."""
+    rng = np.random.default_rng(seed)
+    n = len(L)
+    pi = rng.integers(0, n, npairs).astype(np.int32)
+    pj = rng.integers(0, n, npairs).astype(np.int32)
+    keep = pi != pj
+    pi, pj = pi[keep], pj[keep]
+    st.align_pairs(pi[:64], pj[:64], L)  # warm
+    t0 = time.perf_counter()
+    st.align_pairs(pi, pj, L)
+    wall = time.perf_counter() - t0
+    stats = ctx.last_launch_stats()
+    flag = [r for r in stats if r[0] in (-3001, -3004)]
+    trace = [r for r in stats if r[0] == -3100]
+    cells = int((L[pi].astype(np.int64) * L[pj]).sum())
+    ms_flag = float(sum(r[3] for r in flag))
+    ms_trace = float(sum(r[3] for r in trace))
+    m_rows, n_cols = L[pj].astype(np.int64), L[pi].astype(np.int64)
+    written = int((((m_rows + 511) // 512) * (n_cols + 63) * 64 * 4).sum())  # padded flag blocks
+    alg_bytes = cells * 0.5
+    gbs = alg_bytes / (ms_flag * 1e-3) / 1e9
+    return {
+        "workload": "%d random pairs (i != j) of the same set, %.3g cells" % (len(pi), cells),
+        "flag_kernel_ms": round(ms_flag, 3),
+        "flag_gcups": round(cells / (ms_flag * 1e-3) / 1e9, 1),
+        "traceback_ms": round(ms_trace, 3),
+        "flag_bytes_algorithmic": int(alg_bytes),
+        "flag_bytes_written": written,
+        "hbm": {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
+                "frac": round(gbs / 8000.0, 4),
+                "written_GBps": round(written / (ms_flag * 1e-3) / 1e9, 1)},
+        "launches": [{"kernel": {-3001: "align_kernel<SEQ>", -3004: "align_pipe_kernel<SEQ, 4>",
+                                 -3100: "traceback_kernel"}[k], "jobs": p, "cells": c,
+                      "ms": round(m, 3)} for k, p, c, m in stats],
+        "wall_s_with_host_strings": round(wall, 3),
     }
 
 
@@ -198,6 +244,9 @@ def main():
     ap.add_argument("--serve-what", choices=("msa", "tree", "scores"), default="msa",
                     help="what a served job computes (default: the whole run)")
     ap.add_argument("--serve-only", action="store_true", help="only the served-form measurement")
+    ap.add_argument("--no-traceback", action="store_true", help="skip the alignment-form measurement")
+    ap.add_argument("--traceback-only", action="store_true",
+                    help="only the alignment-form (flags + traceback) measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the all-pairs path runs on a single MI355X "
@@ -227,6 +276,10 @@ def main():
         ctx.tune_set(k, int(v))
         tunes[k] = int(v)
     st = ctx.upload(seqs)
+    if args.traceback_only:
+        print(json.dumps({"traceback": traceback_bench(M, ctx, st, L)}), flush=True)
+        ctx.close()
+        return 0
     print("bench: %d sequences, %.4g cells (i<j), planning" % (len(seqs), cells), file=sys.stderr,
           flush=True)
 
@@ -309,6 +362,7 @@ def main():
         "roofline": roof,
         "batched_launches": launches,
         "tunables": dict(ctx.tuning, **tunes),
+        "traceback": None if (args.no_traceback or args.batched_only) else traceback_bench(M, ctx, st, L),
         "served": None if (args.no_serve or args.batched_only) else served_bench(
             M, ctx, args.serve_jobs,
             sets=[(kv.split("=", 1)[0], float(kv.split("=", 1)[1])) for kv in args.serve_set],

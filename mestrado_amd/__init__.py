@@ -195,6 +195,22 @@ class Context:
         self._ck(self._L.mst_tune_get(self.h, key.encode(), C.byref(v)), "mst_tune_get")
         return v.value
 
+    def last_launch_stats(self):
+        """Per-launch statistics of the last call that records them (all-pairs
+        launches, or the flag kernels and traceback of align_pairs):
+        [(kernel_class, pairs, cells, ms)]."""
+        L = self._L
+        cnt = C.c_int(0)
+        L.mst_last_launch_stats(self.h, None, None, None, None, 0, C.byref(cnt))
+        k = np.zeros(max(1, cnt.value), np.int32)
+        p = np.zeros(max(1, cnt.value), np.int64)
+        c = np.zeros(max(1, cnt.value), np.int64)
+        m = np.zeros(max(1, cnt.value), np.float32)
+        self._ck(L.mst_last_launch_stats(self.h, _p(k), _p(p), _p(c), _p(m), cnt.value,
+                                         C.byref(cnt)), "mst_last_launch_stats")
+        n = cnt.value
+        return list(zip(k[:n].tolist(), p[:n].tolist(), c[:n].tolist(), m[:n].tolist()))
+
     def valu_peak(self, packed=False):
         """Measured integer VALU rate of the device in T ops/s (int32, or packed int16)."""
         t = C.c_double(0)

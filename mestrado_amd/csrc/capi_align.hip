@@ -41,6 +41,7 @@ int run_align_jobs(mst_ctx* ctx, int src, const int8_t* d_seqs, const uint8_t* d
   for (size_t k = 0; k < nj; ++k) sorted[k] = jobs[perm[k]];
   std::vector<int32_t> sc;
   std::vector<std::vector<uint8_t>> pa;
+  ctx->launch_stats.clear();
   int rc = run_align_batches(ctx, src, d_seqs, d_tab, sorted, sc, pa);
   if (rc) return rc;
   scores.assign(nj, 0);
@@ -100,6 +101,14 @@ static int run_align_batches(mst_ctx* ctx, int src, const int8_t* d_seqs, const 
     if (rc) return rc;
     CK(hipMemcpyAsync(ctx->d_jobs, jobs.data() + k0, sizeof(AlignJob) * nb, hipMemcpyHostToDevice,
                       ctx->stream));
+    // events around the flag kernels and the traceback (launch statistics)
+    while (ctx->launch_events.size() < 4) {
+      hipEvent_t e;
+      CK(hipEventCreate(&e));
+      ctx->launch_events.push_back(e);
+    }
+    hipEvent_t* ev = ctx->launch_events.data();
+    CK(hipEventRecord(ev[0], ctx->stream));
     AlignArgs a;
     a.jobs = (const AlignJob*)ctx->d_jobs;
     a.first = 0;
@@ -120,13 +129,15 @@ static int run_align_batches(mst_ctx* ctx, int src, const int8_t* d_seqs, const 
       launch_align(src, a, grid, ctx->stream);
       CK(hipGetLastError());
     }
+    CK(hipMemsetAsync(ctx->d_acounter, 0, sizeof(unsigned int), ctx->stream));
+    CK(hipEventRecord(ev[1], ctx->stream));
     if (n2 > 0) {
-      CK(hipMemsetAsync(ctx->d_acounter, 0, sizeof(unsigned int), ctx->stream));
       a.first = n1;
       a.last = nb;
       launch_align_pipe(src, 4, a, grid2, ctx->stream);
       CK(hipGetLastError());
     }
+    CK(hipEventRecord(ev[2], ctx->stream));
     a.first = 0;
     a.last = nb;
     TraceArgs t;
@@ -138,12 +149,27 @@ static int run_align_batches(mst_ctx* ctx, int src, const int8_t* d_seqs, const 
     t.lens = (int32_t*)ctx->d_jscore + nb;
     launch_traceback(t, ctx->stream);
     CK(hipGetLastError());
+    CK(hipEventRecord(ev[3], ctx->stream));
     std::vector<int32_t> sl(2 * nb);
     std::vector<uint8_t> ops(opsb);
     CK(hipMemcpyAsync(sl.data(), ctx->d_jscore, sizeof(int32_t) * 2 * nb, hipMemcpyDeviceToHost,
                       ctx->stream));
     if (opsb) CK(hipMemcpyAsync(ops.data(), ctx->d_ops, opsb, hipMemcpyDeviceToHost, ctx->stream));
     CK(hipStreamSynchronize(ctx->stream));
+    {
+      // launch statistics: -3001 one-strip flag kernel, -3004 pipelined flag
+      // kernel, -3100 traceback; cells = sum of n*m of their jobs
+      int64_t c1 = 0, c2 = 0;
+      for (int64_t k = k0; k < ks; ++k) c1 += (int64_t)jobs[k].n * jobs[k].m;
+      for (int64_t k = ks; k < k1; ++k) c2 += (int64_t)jobs[k].n * jobs[k].m;
+      float t1 = 0, t2 = 0, t3 = 0;
+      CK(hipEventElapsedTime(&t1, ev[0], ev[1]));
+      CK(hipEventElapsedTime(&t2, ev[1], ev[2]));
+      CK(hipEventElapsedTime(&t3, ev[2], ev[3]));
+      if (n1 > 0) ctx->launch_stats.push_back(LaunchStat{-3001, n1, c1, t1});
+      if (n2 > 0) ctx->launch_stats.push_back(LaunchStat{-3004, n2, c2, t2});
+      ctx->launch_stats.push_back(LaunchStat{-3100, nb, c1 + c2, t3});
+    }
     for (int64_t k = 0; k < nb; ++k) {
       const AlignJob& jb = jobs[k0 + k];
       scores[k0 + k] = sl[k];
