@@ -27,8 +27,13 @@ static int run_align_batches(mst_ctx* ctx, int src, const int8_t* d_seqs, const 
                              std::vector<AlignJob>& jobs, std::vector<int32_t>& scores,
                              std::vector<std::vector<uint8_t>>& paths);
 
+static int align_class(const AlignJob& j) {
+  return align_strips(j.m) == 1 ? 0 : (tiled_job(j.n, j.m) ? 2 : 1);
+}
+
 // Jobs of one 512-row strip run one per wave (align_kernel); longer ones one
-// per workgroup with a wave per strip (align_pipe_kernel). The jobs are put in
+// per workgroup with a wave per strip (align_pipe_kernel); the longest as
+// tiles over anti-diagonal launches (align_tile_kernel). The jobs are put in
 // that order for the batches and the results put back in the caller's order.
 int run_align_jobs(mst_ctx* ctx, int src, const int8_t* d_seqs, const uint8_t* d_tab,
                    std::vector<AlignJob>& jobs, std::vector<int32_t>& scores,
@@ -36,7 +41,8 @@ int run_align_jobs(mst_ctx* ctx, int src, const int8_t* d_seqs, const uint8_t* d
   const size_t nj = jobs.size();
   std::vector<size_t> perm(nj);
   for (size_t k = 0; k < nj; ++k) perm[k] = k;
-  std::stable_partition(perm.begin(), perm.end(), [&](size_t k) { return align_strips(jobs[k].m) == 1; });
+  std::stable_sort(perm.begin(), perm.end(),
+                   [&](size_t x, size_t y) { return align_class(jobs[x]) < align_class(jobs[y]); });
   std::vector<AlignJob> sorted(nj);
   for (size_t k = 0; k < nj; ++k) sorted[k] = jobs[perm[k]];
   std::vector<int32_t> sc;
@@ -86,10 +92,24 @@ static int run_align_batches(mst_ctx* ctx, int src, const int8_t* d_seqs, const 
     if (rc) return rc;
     rc = ctx->ensure((void**)&ctx->d_jscore, &ctx->jscore_cap, sizeof(int32_t) * 2 * nb);
     if (rc) return rc;
-    // jobs [k0, ks) are single-strip, [ks, k1) longer (the list is partitioned)
+    // jobs [k0, ks) are single-strip, [ks, kt) pipelined, [kt, k1) tiled
     int64_t ks = k0;
-    while (ks < k1 && align_strips(jobs[ks].m) == 1) ++ks;
-    const int64_t n1 = ks - k0, n2 = k1 - ks;
+    while (ks < k1 && align_class(jobs[ks]) == 0) ++ks;
+    int64_t kt = ks;
+    while (kt < k1 && align_class(jobs[kt]) == 1) ++kt;
+    const int64_t n1 = ks - k0, n2 = kt - ks, n3 = k1 - kt;
+    std::vector<TileJob> tiles;
+    std::vector<int64_t> diag_off;
+    int64_t trows = 0, tstate = 0;
+    if (n3 > 0) {
+      plan_tiles(jobs.data() + k0, kt - k0, k1 - k0, kTileWidth, tiles, diag_off, &trows, &tstate);
+      rc = ctx->ensure(&ctx->d_tiles, &ctx->tiles_cap, sizeof(TileJob) * tiles.size());
+      if (rc) return rc;
+      rc = ctx->ensure(&ctx->d_trows, &ctx->trows_cap, sizeof(int2) * trows);
+      if (rc) return rc;
+      rc = ctx->ensure(&ctx->d_tstate, &ctx->tstate_cap, sizeof(int) * tstate);
+      if (rc) return rc;
+    }
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ctx->num_cus * 16, n1));
     const int grid2 = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ctx->num_cus * 2, n2));
     const int64_t bufcap = maxn;
@@ -102,7 +122,7 @@ static int run_align_batches(mst_ctx* ctx, int src, const int8_t* d_seqs, const 
     CK(hipMemcpyAsync(ctx->d_jobs, jobs.data() + k0, sizeof(AlignJob) * nb, hipMemcpyHostToDevice,
                       ctx->stream));
     // events around the flag kernels and the traceback (launch statistics)
-    while (ctx->launch_events.size() < 4) {
+    while (ctx->launch_events.size() < 5) {
       hipEvent_t e;
       CK(hipEventCreate(&e));
       ctx->launch_events.push_back(e);
@@ -133,11 +153,31 @@ static int run_align_batches(mst_ctx* ctx, int src, const int8_t* d_seqs, const 
     CK(hipEventRecord(ev[1], ctx->stream));
     if (n2 > 0) {
       a.first = n1;
-      a.last = nb;
+      a.last = n1 + n2;
       launch_align_pipe(src, 4, a, grid2, ctx->stream);
       CK(hipGetLastError());
     }
     CK(hipEventRecord(ev[2], ctx->stream));
+    if (n3 > 0) {
+      CK(hipMemcpyAsync(ctx->d_tiles, tiles.data(), sizeof(TileJob) * tiles.size(), hipMemcpyHostToDevice,
+                        ctx->stream));
+      TileArgs ta;
+      ta.jobs = (const AlignJob*)ctx->d_jobs;
+      ta.tiles = nullptr;
+      ta.seqs = d_seqs;
+      ta.tab = d_tab;
+      ta.subst = ctx->d_S;
+      ta.flags = (uint32_t*)ctx->d_flags;
+      ta.scores = (int32_t*)ctx->d_jscore;
+      ta.rows = (int2*)ctx->d_trows;
+      ta.state = (int*)ctx->d_tstate;
+      ta.go = ctx->go;
+      ta.ge = ctx->ge;
+      ta.tw = kTileWidth;
+      launch_tile_diagonals(src, ta, (const TileJob*)ctx->d_tiles, diag_off, ctx->stream);
+      CK(hipGetLastError());
+    }
+    CK(hipEventRecord(ev[4], ctx->stream));
     a.first = 0;
     a.last = nb;
     TraceArgs t;
@@ -158,17 +198,20 @@ static int run_align_batches(mst_ctx* ctx, int src, const int8_t* d_seqs, const 
     CK(hipStreamSynchronize(ctx->stream));
     {
       // launch statistics: -3001 one-strip flag kernel, -3004 pipelined flag
-      // kernel, -3100 traceback; cells = sum of n*m of their jobs
-      int64_t c1 = 0, c2 = 0;
+      // kernel, -3008 tiled flag kernel, -3100 traceback; cells = sum of n*m
+      int64_t c1 = 0, c2 = 0, c3 = 0;
       for (int64_t k = k0; k < ks; ++k) c1 += (int64_t)jobs[k].n * jobs[k].m;
-      for (int64_t k = ks; k < k1; ++k) c2 += (int64_t)jobs[k].n * jobs[k].m;
-      float t1 = 0, t2 = 0, t3 = 0;
+      for (int64_t k = ks; k < kt; ++k) c2 += (int64_t)jobs[k].n * jobs[k].m;
+      for (int64_t k = kt; k < k1; ++k) c3 += (int64_t)jobs[k].n * jobs[k].m;
+      float t1 = 0, t2 = 0, t3 = 0, t4 = 0;
       CK(hipEventElapsedTime(&t1, ev[0], ev[1]));
       CK(hipEventElapsedTime(&t2, ev[1], ev[2]));
-      CK(hipEventElapsedTime(&t3, ev[2], ev[3]));
+      CK(hipEventElapsedTime(&t4, ev[2], ev[4]));
+      CK(hipEventElapsedTime(&t3, ev[4], ev[3]));
       if (n1 > 0) ctx->launch_stats.push_back(LaunchStat{-3001, n1, c1, t1});
       if (n2 > 0) ctx->launch_stats.push_back(LaunchStat{-3004, n2, c2, t2});
-      ctx->launch_stats.push_back(LaunchStat{-3100, nb, c1 + c2, t3});
+      if (n3 > 0) ctx->launch_stats.push_back(LaunchStat{-3008, n3, c3, t4});
+      ctx->launch_stats.push_back(LaunchStat{-3100, nb, c1 + c2 + c3, t3});
     }
     for (int64_t k = 0; k < nb; ++k) {
       const AlignJob& jb = jobs[k0 + k];

@@ -109,5 +109,11 @@ struct mst_ctx {
   int fail(int code, const char* what, const char* detail = nullptr);
   unsigned int* d_acounter = nullptr;  // work queue head of the pipelined flag kernel
   size_t acounter_cap = 0;
+  void* d_tiles = nullptr;  // tiled flag kernel: tile lists, row buffers, lane states
+  size_t tiles_cap = 0;
+  void* d_trows = nullptr;
+  size_t trows_cap = 0;
+  void* d_tstate = nullptr;
+  size_t tstate_cap = 0;
   int ensure(void** p, size_t* cap, size_t bytes);
 };

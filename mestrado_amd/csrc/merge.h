@@ -83,6 +83,13 @@ class MergeEngine {
   LeafDesc* d_leaf_ = nullptr;
   LeafDesc* h_leaf_ = nullptr;
   int64_t leaf_cap_ = 0, hleaf_cap_ = 0;
+  TileJob* d_tiles_ = nullptr;  // tiled flag kernel of the longest merges
+  TileJob* h_tiles_ = nullptr;
+  int64_t tiles_cap_ = 0, htiles_cap_ = 0;
+  int2* d_trows_ = nullptr;
+  int64_t trows_cap_ = 0;
+  int* d_tstate_ = nullptr;
+  int64_t tstate_cap_ = 0;
 };
 
 }  // namespace mst

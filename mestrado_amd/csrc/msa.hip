@@ -277,10 +277,11 @@ hipError_t MergeEngine::grow(void** p, int64_t* cap, int64_t bytes, bool pinned)
 }
 
 MergeEngine::~MergeEngine() {
-  void* dev[] = {d_scratch_, d_desc_, d_jobs_, d_res_, d_lens_, d_bufs_, d_counter_, d_leaf_};
+  void* dev[] = {d_scratch_, d_desc_, d_jobs_, d_res_,    d_lens_,  d_bufs_,
+                 d_counter_, d_leaf_, d_tiles_, d_trows_, d_tstate_};
   for (void* p : dev)
     if (p) hipFree(p);
-  void* host[] = {h_desc_, h_jobs_, h_lens_, h_leaf_};
+  void* host[] = {h_desc_, h_jobs_, h_lens_, h_leaf_, h_tiles_};
   for (void* p : host)
     if (p) hipHostFree(p);
 }
@@ -300,18 +301,19 @@ hipError_t MergeEngine::leaves(const int8_t* codes, const LeafDesc* l, int count
 hipError_t MergeEngine::enqueue(const MergeSpec* m, int count, const int8_t* d_S, int go, int ge,
                                 int num_cus, hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  // flag jobs: one strip, 2..4 strips (pipeline of 4 waves), more (8 waves)
+  // flag jobs: one strip, 2..4 strips (pipeline of 4 waves), more (8 waves),
+  // long ones (tiles over anti-diagonal launches)
   std::vector<int> cls(count, -1);
-  int64_t nb[3] = {0, 0, 0};
+  int64_t nb[4] = {0, 0, 0, 0};
   for (int i = 0; i < count; ++i) {
     if (m[i].nX > 0 && m[i].nY > 0) {
       const int s = strips_of(m[i].nY);
-      cls[i] = s == 1 ? 0 : (s <= 4 ? 1 : 2);
+      cls[i] = s == 1 ? 0 : (s <= 4 ? 1 : (tiled_job(m[i].nX, m[i].nY) ? 3 : 2));
       nb[cls[i]] += 1;
     }
   }
-  const int64_t njobs = nb[0] + nb[1] + nb[2];
-  int64_t next[3] = {0, nb[0], nb[0] + nb[1]};
+  const int64_t njobs = nb[0] + nb[1] + nb[2] + nb[3];
+  int64_t next[4] = {0, nb[0], nb[0] + nb[1], nb[0] + nb[1] + nb[2]};
   // layout of the scratch block and the prefix sums
   const int64_t desc_bytes = (int64_t)count * (sizeof(MergeDesc) + sizeof(Prefix));
   hipError_t e;
@@ -429,9 +431,37 @@ hipError_t MergeEngine::enqueue(const MergeSpec* m, int count, const int8_t* d_S
     }
     if (nb[2] > 0) {
       a.first = nb[0] + nb[1];
-      a.last = njobs;
+      a.last = nb[0] + nb[1] + nb[2];
       a.counter = d_counter_ + 1;
       launch_align_pipe(SRC_TAB, 8, a, (int)grid3, st);
+    }
+    if (nb[3] > 0) {
+      std::vector<TileJob> tiles;
+      std::vector<int64_t> diag_off;
+      int64_t trows = 0, tstate = 0;
+      plan_tiles(h_jobs_, njobs - nb[3], njobs, kTileWidth, tiles, diag_off, &trows, &tstate);
+      if ((e = grow((void**)&d_tiles_, &tiles_cap_, sizeof(TileJob) * tiles.size(), false)) != hipSuccess) return e;
+      if ((e = grow((void**)&h_tiles_, &htiles_cap_, sizeof(TileJob) * tiles.size(), true)) != hipSuccess) return e;
+      if ((e = grow((void**)&d_trows_, &trows_cap_, sizeof(int2) * trows, false)) != hipSuccess) return e;
+      if ((e = grow((void**)&d_tstate_, &tstate_cap_, sizeof(int) * tstate, false)) != hipSuccess) return e;
+      std::copy(tiles.begin(), tiles.end(), h_tiles_);
+      if ((e = hipMemcpyAsync(d_tiles_, h_tiles_, sizeof(TileJob) * tiles.size(), hipMemcpyHostToDevice, st)) !=
+          hipSuccess)
+        return e;
+      TileArgs ta;
+      ta.jobs = d_jobs_;
+      ta.tiles = nullptr;
+      ta.seqs = nullptr;
+      ta.tab = d_scratch_;
+      ta.subst = d_S;
+      ta.flags = (uint32_t*)d_scratch_;
+      ta.scores = d_res_;
+      ta.rows = d_trows_;
+      ta.state = d_tstate_;
+      ta.go = go;
+      ta.ge = ge;
+      ta.tw = kTileWidth;
+      launch_tile_diagonals(SRC_TAB, ta, d_tiles_, diag_off, st);
     }
     TraceArgs t;
     t.jobs = d_jobs_;

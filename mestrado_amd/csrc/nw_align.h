@@ -2,6 +2,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace mst {
 
 enum { SRC_SEQ = 0, SRC_TAB = 1 };
@@ -37,7 +39,48 @@ struct TraceArgs {
   int32_t* lens;
 };
 
+// Tiled form (an inter-block wavefront) for long jobs: tile (s, c) is strip s
+// (512 rows) by columns [c*tw, (c+1)*tw). One launch per anti-diagonal
+// s + c = d; `tiles` lists that launch's tiles, one wave each.
+struct TileJob {
+  int job;         // index into AlignArgs-style job list
+  int s, c;        // strip, column tile
+  int pad;
+  int64_t rb;      // int2 offset of the job's two row buffers (2 x n)
+  int64_t st;      // int offset of the job's lane states (strips x kTileStateInts)
+};
+
+struct TileArgs {
+  const AlignJob* jobs;
+  const TileJob* tiles;
+  const int8_t* seqs;
+  const uint8_t* tab;
+  const int8_t* subst;
+  uint32_t* flags;
+  int32_t* scores;  // indexed by TileJob::job
+  int2* rows;
+  int* state;
+  int go, ge, tw;
+};
+
 void launch_align(int src, const AlignArgs& a, int grid, hipStream_t st);
+void launch_align_tiles(int src, const TileArgs& a, int ntiles, hipStream_t st);
+// tile bookkeeping of a job: row buffers (int2) and lane states (int)
+// per strip: H and E of 8 rows for 64 lanes, then lane 0's next diagonal
+constexpr int kTileStateInts = 64 * 16 + 64;
+inline int64_t tile_rows_elems(int n) { return 2 * (int64_t)n; }
+inline int64_t tile_state_elems(int m) { return (int64_t)((m + 511) / 512) * kTileStateInts; }
+constexpr int kTileWidth = 256;  // columns of a tile
+// Jobs long enough for the tiled form: more strips than the 8-wave pipeline
+// holds at once and enough columns for several tiles.
+inline bool tiled_job(int n, int m) { return (m + 511) / 512 > 8 && n >= 8 * kTileWidth; }
+// Tiles of jobs[first, last) grouped by anti-diagonal: tiles[diag_off[d] ..
+// diag_off[d+1]) is launch d; *rows / *state: elements of the row buffers and
+// lane states (TileJob::rb, ::st index them).
+void plan_tiles(const AlignJob* jobs, int64_t first, int64_t last, int tw, std::vector<TileJob>& tiles,
+                std::vector<int64_t>& diag_off, int64_t* rows, int64_t* state);
+void launch_tile_diagonals(int src, TileArgs a, const TileJob* d_tiles, const std::vector<int64_t>& diag_off,
+                           hipStream_t st);
 // Pipelined form for jobs of more than one 512-row strip: one job per
 // workgroup of W waves (W = 4, or 8 with SRC_TAB); wave j computes strips j,
 // j+W, ...; a strip reads the row above from the wave of the previous strip

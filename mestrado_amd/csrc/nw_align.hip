@@ -100,9 +100,12 @@ __device__ __forceinline__ uint32_t cell_column_flags(uint32_t w0, uint32_t w1, 
   return word;
 }
 
+// One step of a strip: lane l computes column t - l of its 8 rows when that
+// column is in [lo, hi) (PRED), reading F/H of the row above from lane l-1 or,
+// for lane 0, from the strip above (bv) or the top boundary (FIRST).
 template <int SRC, bool PRED, bool FIRST, bool STORE>
 __device__ __forceinline__ void astep(ALane& L, int t, int x, uint2 tb, int2 bv,
-                                      const uint8_t* prof_lane, int n, int G, int2* bout,
+                                      const uint8_t* prof_lane, int lo, int hi, int G, int2* bout,
                                       uint32_t* fl_strip) {
   const int lane = lane_id();
   int fin, hup;
@@ -114,7 +117,7 @@ __device__ __forceinline__ void astep(ALane& L, int t, int x, uint2 tb, int2 bv,
     hup = shr1(bv.y, L.Hout);
   }
   const int c = t - lane;
-  const bool act = !PRED || (c >= 0 && c < n);
+  const bool act = !PRED || (c >= lo && c < hi);
   if (act) {
     uint32_t w0, w1;
     if (SRC == SRC_SEQ) {
@@ -176,13 +179,13 @@ __device__ void arun_strip(ALane& L, const int8_t* X, int n, const uint8_t* prof
     if (full) {
 #pragma unroll
       for (int j = 0; j < AU; ++j)
-        astep<SRC, false, FIRST, STORE>(L, t0 + j, xa[j], ta[j], ba[j], prof_lane, n, G, bout,
+        astep<SRC, false, FIRST, STORE>(L, t0 + j, xa[j], ta[j], ba[j], prof_lane, 0, n, G, bout,
                                         fl_strip);
     } else {
 #pragma unroll
       for (int j = 0; j < AU; ++j)
         if (t0 + j < T)
-          astep<SRC, true, FIRST, STORE>(L, t0 + j, xa[j], ta[j], ba[j], prof_lane, n, G, bout,
+          astep<SRC, true, FIRST, STORE>(L, t0 + j, xa[j], ta[j], ba[j], prof_lane, 0, n, G, bout,
                                          fl_strip);
     }
 #pragma unroll
@@ -191,6 +194,138 @@ __device__ void arun_strip(ALane& L, const int8_t* X, int n, const uint8_t* prof
       ba[j] = bb[j];
       ta[j] = tbb[j];
     }
+  }
+}
+
+// ---- tiled form: an inter-block wavefront for long jobs ----
+//
+// A job is cut into tiles of one strip (512 rows) by `tw` columns; tile
+// (s, c) needs the last row of tile (s-1, c) (F and H per column, a row
+// buffer per strip parity) and the last column of tile (s, c-1) (H and E of
+// each of the strip's rows, the lane states of strip s). Launch d runs every
+// tile with s + c = d, one wave each, so a job of S strips and C column tiles
+// takes S + C - 1 launches of up to min(S, C) waves, ordered by the stream
+// (no waits between workgroups). The flags are the one-strip kernel's: lane l
+// of tile (s, c) writes step t = column + l of strip s for its columns only.
+
+// Columns [c0, c1) of a strip, global steps c0 .. c1 + 62.
+template <int SRC, bool FIRST, bool STORE>
+__device__ void arun_tile(ALane& L, const int8_t* X, int n, int c0, int c1, const uint8_t* prof_lane,
+                          const uint2* tab_strip, const int2* bin, int2* bout, int G,
+                          uint32_t* fl_strip) {
+  const int T1 = c1 + MST_WAVE - 1;
+  int xa[AU], xb[AU];
+  int2 ba[AU], bb[AU];
+  uint2 ta[AU], tbb[AU];
+  aload<SRC, FIRST>(c0, X, n, bin, tab_strip, xa, ba, ta);
+  for (int t0 = c0; t0 < T1; t0 += AU) {
+    aload<SRC, FIRST>(t0 + AU, X, n, bin, tab_strip, xb, bb, tbb);
+    const bool full = (t0 >= c0 + MST_WAVE - 1) && (t0 + AU <= c1);
+    if (full) {
+#pragma unroll
+      for (int j = 0; j < AU; ++j)
+        astep<SRC, false, FIRST, STORE>(L, t0 + j, xa[j], ta[j], ba[j], prof_lane, c0, c1, G, bout,
+                                        fl_strip);
+    } else {
+#pragma unroll
+      for (int j = 0; j < AU; ++j)
+        if (t0 + j < T1)
+          astep<SRC, true, FIRST, STORE>(L, t0 + j, xa[j], ta[j], ba[j], prof_lane, c0, c1, G, bout,
+                                         fl_strip);
+    }
+#pragma unroll
+    for (int j = 0; j < AU; ++j) {
+      xa[j] = xb[j];
+      ba[j] = bb[j];
+      ta[j] = tbb[j];
+    }
+  }
+}
+
+template <int SRC>
+__global__ void __launch_bounds__(MST_WAVE) align_tile_kernel(TileArgs a) {
+  __shared__ int8_t S_lds[MST_LETTERS * MST_LETTERS];
+  __shared__ __attribute__((aligned(16))) uint8_t prof[SRC == SRC_SEQ ? MST_LETTERS * MST_WAVE * AK : 16];
+  const int lane = lane_id();
+  const TileJob tile = a.tiles[blockIdx.x];
+  const AlignJob job = a.jobs[tile.job];
+  const int n = job.n, m = job.m;
+  const int s = tile.s;
+  const int c0 = tile.c * a.tw;
+  const int c1 = c0 + a.tw < n ? c0 + a.tw : n;
+  const int Tn = n + MST_WAVE - 1;
+  const int nstrips = (m + ASH - 1) / ASH;
+  const int r0 = s * ASH;
+  const int G = a.go - a.ge;
+  const int8_t* X = a.seqs + job.x_off;
+  if (SRC == SRC_SEQ) {
+    for (int i = lane; i < MST_LETTERS * MST_LETTERS; i += MST_WAVE) S_lds[i] = a.subst[i];
+    __syncthreads();
+    build_profile8(prof, S_lds, a.seqs + job.y_off, m, r0, a.go + a.ge);
+    __syncthreads();
+  }
+  // lane state at column c0 - 1: H and E of the lane's rows (the left
+  // boundary: H = 0 in the biased form, E = -inf); Hout starts as H of the
+  // lane's last row there, which lane l+1 takes as its first diagonal. Lane
+  // 0's first diagonal, H of the strip above at c0 - 1, was saved by tile
+  // (s, c-1): the row buffer slot itself may be rewritten in this launch by
+  // tile (s+1, c-1), which shares its parity.
+  int* sblk = a.state + tile.st + (int64_t)s * kTileStateInts;
+  int* st = sblk + lane * (2 * AK);
+  int2* rows = a.rows + tile.rb;
+  const int2* bin = s > 0 ? rows + (int64_t)((s - 1) & 1) * n : rows;
+  int2* bout = rows + (int64_t)(s & 1) * n;
+  ALane L;
+  if (c0 == 0) {
+#pragma unroll
+    for (int i = 0; i < AK; ++i) {
+      L.Hm[i] = 0;
+      L.E[i] = MST_NEG_INF;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < AK; ++i) {
+      L.Hm[i] = st[i];
+      L.E[i] = st[AK + i];
+    }
+  }
+  L.Fout = MST_NEG_INF;
+  L.Hout = L.Hm[AK - 1];
+  L.hdiag = (c0 > 0 && s > 0) ? sblk[MST_WAVE * 2 * AK] : 0;
+  uint32_t* fl_strip = a.flags + job.flag_off + (int64_t)s * Tn * MST_WAVE;
+  const uint2* tab_strip =
+      (SRC == SRC_TAB) ? (const uint2*)(a.tab + job.tab_off) + (int64_t)s * Tn * MST_WAVE : nullptr;
+  const uint8_t* prof_lane = prof + lane * AK;
+  const bool last = (s == nstrips - 1);
+  if (s == 0) {
+    if (last)
+      arun_tile<SRC, true, false>(L, X, n, c0, c1, prof_lane, tab_strip, bin, bout, G, fl_strip);
+    else
+      arun_tile<SRC, true, true>(L, X, n, c0, c1, prof_lane, tab_strip, bin, bout, G, fl_strip);
+  } else {
+    if (last)
+      arun_tile<SRC, false, false>(L, X, n, c0, c1, prof_lane, tab_strip, bin, bout, G, fl_strip);
+    else
+      arun_tile<SRC, false, true>(L, X, n, c0, c1, prof_lane, tab_strip, bin, bout, G, fl_strip);
+  }
+  if (c1 < n) {
+#pragma unroll
+    for (int i = 0; i < AK; ++i) {
+      st[i] = L.Hm[i];
+      st[AK + i] = L.E[i];
+    }
+    // nobody rewrites this slot during the launch: the other tiles of the
+    // parity write columns before c0 or from c1 on
+    if (s > 0 && lane == 0) sblk[MST_WAVE * 2 * AK] = bin[c1 - 1].y;
+  } else if (last) {
+    const int rl = (m - 1) - r0;
+    const int owner = rl / AK, idx = rl % AK;
+    int v = L.Hm[0];
+#pragma unroll
+    for (int i = 1; i < AK; ++i)
+      if (idx == i) v = L.Hm[i];
+    v = __shfl(v, owner);
+    if (lane == 0) a.scores[tile.job] = v - a.ge * (m + n);
   }
 }
 
@@ -644,6 +779,44 @@ void launch_align_pipe(int src, int W, const AlignArgs& a, int grid, hipStream_t
     hipLaunchKernelGGL((align_pipe_kernel<SRC_TAB, 8>), dim3(grid), dim3(MST_WAVE * 8), 0, st, a);
   else
     hipLaunchKernelGGL((align_pipe_kernel<SRC_TAB, 4>), dim3(grid), dim3(MST_WAVE * 4), 0, st, a);
+}
+
+void launch_align_tiles(int src, const TileArgs& a, int ntiles, hipStream_t st) {
+  if (ntiles <= 0) return;
+  if (src == SRC_SEQ)
+    hipLaunchKernelGGL(align_tile_kernel<SRC_SEQ>, dim3(ntiles), dim3(MST_WAVE), 0, st, a);
+  else
+    hipLaunchKernelGGL(align_tile_kernel<SRC_TAB>, dim3(ntiles), dim3(MST_WAVE), 0, st, a);
+}
+
+void plan_tiles(const AlignJob* jobs, int64_t first, int64_t last, int tw, std::vector<TileJob>& tiles,
+                std::vector<int64_t>& diag_off, int64_t* rows, int64_t* state) {
+  std::vector<std::vector<TileJob>> by_d;
+  int64_t rb = 0, st = 0;
+  for (int64_t j = first; j < last; ++j) {
+    const int S = (jobs[j].m + ASH - 1) / ASH, C = (jobs[j].n + tw - 1) / tw;
+    if ((int)by_d.size() < S + C - 1) by_d.resize(S + C - 1);
+    for (int s = 0; s < S; ++s)
+      for (int c = 0; c < C; ++c) by_d[s + c].push_back(TileJob{(int)j, s, c, 0, rb, st});
+    rb += tile_rows_elems(jobs[j].n);
+    st += tile_state_elems(jobs[j].m);
+  }
+  tiles.clear();
+  diag_off.assign(1, 0);
+  for (auto& v : by_d) {
+    tiles.insert(tiles.end(), v.begin(), v.end());
+    diag_off.push_back((int64_t)tiles.size());
+  }
+  *rows = rb;
+  *state = st;
+}
+
+void launch_tile_diagonals(int src, TileArgs a, const TileJob* d_tiles, const std::vector<int64_t>& diag_off,
+                           hipStream_t st) {
+  for (size_t d = 0; d + 1 < diag_off.size(); ++d) {
+    a.tiles = d_tiles + diag_off[d];
+    launch_align_tiles(src, a, (int)(diag_off[d + 1] - diag_off[d]), st);
+  }
 }
 
 int64_t align_strips(int m) { return m <= 0 ? 1 : (m + ASH - 1) / ASH; }

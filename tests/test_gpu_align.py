@@ -53,6 +53,25 @@ def test_long_pairs(ctx):
     _check(ctx, seqs, pi, pj)
 
 
+def test_tiled_pairs(ctx):
+    # more than 8 strips and at least 2,048 columns: tiles of 512 rows x 256
+    # columns over anti-diagonal launches; exact and ragged tile edges
+    rng = np.random.default_rng(13)
+    root = "".join(rng.choice(list(O.STANDARD20), 5200))
+    lens = [2048, 4097, 5120, 2300]
+    seqs = []
+    for L in lens:  # related sequences: long diagonal runs across tile edges
+        s = list(root[:L])
+        for t in rng.choice(L, L // 6, replace=False):
+            s[t] = rng.choice(list(O.STANDARD20))
+        seqs.append("".join(s))
+    pi = np.array([0, 2, 1, 3, 1])
+    pj = np.array([1, 1, 2, 2, 0])
+    _check(ctx, seqs, pi, pj)
+    kinds = [r[0] for r in ctx.last_launch_stats()]
+    assert -3008 in kinds and -3004 in kinds  # tiled and pipelined jobs both ran
+
+
 def test_config0_pairs(ctx):
     seqs = M.synth(0, 0)
     rng = np.random.default_rng(8)

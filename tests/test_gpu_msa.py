@@ -62,3 +62,16 @@ def test_long_profiles_cross_strips(ctx):
             s[t] = rng.choice(list(O.STANDARD20))
         seqs.append("".join(s)[k * 7:])
     _check(ctx, seqs)
+
+
+def test_long_profiles_tiled(ctx):
+    # profiles of more than 8 strips and 2,048 columns take the tiled flag kernel
+    rng = np.random.default_rng(8)
+    root = "".join(rng.choice(list(O.STANDARD20), 4700))
+    seqs = []
+    for k in range(4):
+        s = list(root)
+        for t in rng.choice(len(s), 500, replace=False):
+            s[t] = rng.choice(list(O.STANDARD20))
+        seqs.append("".join(s)[k * 40: 4700 - k * 30])
+    _check(ctx, seqs)
