@@ -56,7 +56,7 @@ __device__ int find_merge(int count, int64_t x, Key key) {
 struct Prefix {
   int64_t col0;   // counts: columns of X then Y of nontrivial merges
   int64_t yrow0;  // P rows (nY of nontrivial merges)
-  int64_t item0;  // table entries, one per wave (strip, step)
+  int64_t item0;  // table work units: (strip, chunk of kTabChunk steps), one per wave
 };
 
 // counts[c][25] of column c of a group's k rows (row-major k x len)
@@ -91,39 +91,16 @@ __device__ __forceinline__ void p_row(const int32_t* __restrict__ countsY, int k
   }
 }
 
-// Entry (strip s, step t, lane) of the byte table of S''(r,c) = rint(s(r,c))
-// + go + ge in the skewed layout of the flag kernel: the 8 rows of the lane
-// (DP row = s*512 + lane*8 + i) at DP column c = t - lane. DP row r is
+// Byte table of S''(r,c) = rint(s(r,c)) + go + ge in the skewed layout of the
+// flag kernel: entry ((strip*(n+63) + t)*64 + lane) holds the 8 rows of the
+// lane (DP row = strip*512 + lane*8 + i) at DP column c = t - lane. DP row r is
 // forward Y column m-1-r and DP column c is forward X column n-1-c.
-__device__ __forceinline__ uint2 table_entry(const int32_t* __restrict__ countsX, int n, int kX,
-                                             const float* __restrict__ P, int m, int go_plus_ge,
-                                             int s, int t, int lane) {
-  const int c = t - lane;
-  uint32_t w[2] = {0, 0};
-  if (c >= 0 && c < n) {
-    const int xc = n - 1 - c;
-    float fx[24];
-#pragma unroll
-    for (int a = 0; a < 24; ++a) fx[a] = (float)countsX[(int64_t)xc * 25 + a] / (float)kX;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int r = s * kStripRows + lane * 8 + i;
-      int v = 0;
-      if (r < m) {
-        const float* p = P + (int64_t)(m - 1 - r) * 24;
-        float acc = 0.0f;
-#pragma unroll
-        for (int a = 0; a < 24; ++a) {
-          const float q = fx[a] * p[a];
-          acc = acc + q;
-        }
-        v = (int)__builtin_rintf(acc) + go_plus_ge;
-      }
-      w[i >> 2] |= ((uint32_t)v & 0xFFu) << (8 * (i & 3));
-    }
-  }
-  return make_uint2(w[0], w[1]);
-}
+//
+// A wave takes one strip and a chunk of kTabChunk steps: each lane keeps the
+// P rows of its 8 DP rows in registers for the whole chunk and reads one X
+// frequency row (24 floats, fX = count / kX, written by the counts kernel)
+// per step, so P is read once per chunk instead of once per entry.
+constexpr int kTabChunk = 256;
 
 __global__ void mb_counts_kernel(const MergeDesc* __restrict__ d, const Prefix* __restrict__ pre,
                                  int count, int64_t total, uint8_t* __restrict__ scratch) {
@@ -132,10 +109,15 @@ __global__ void mb_counts_kernel(const MergeDesc* __restrict__ d, const Prefix* 
   const int i = find_merge(count, x, [&](int k) { return pre[k].col0; });
   const MergeDesc m = d[i];
   const int64_t local = x - pre[i].col0;
-  if (local < m.nX)
-    count_column(m.rx, m.kX, m.nX, (int)local, (int32_t*)(scratch + m.cx));
-  else
+  if (local < m.nX) {
+    int32_t* cx = (int32_t*)(scratch + m.cx);
+    count_column(m.rx, m.kX, m.nX, (int)local, cx);
+    float* fx = (float*)(scratch + m.fx) + local * 24;
+#pragma unroll
+    for (int a = 0; a < 24; ++a) fx[a] = (float)cx[local * 25 + a] / (float)m.kX;
+  } else {
     count_column(m.ry, m.kY, m.nY, (int)(local - m.nX), (int32_t*)(scratch + m.cy));
+  }
 }
 
 __global__ void mb_p_kernel(const MergeDesc* __restrict__ d, const Prefix* __restrict__ pre,
@@ -148,22 +130,62 @@ __global__ void mb_p_kernel(const MergeDesc* __restrict__ d, const Prefix* __res
   p_row((const int32_t*)(scratch + m.cy), m.kY, S, (int)(x - pre[i].yrow0), (float*)(scratch + m.p));
 }
 
-__global__ void mb_table_kernel(const MergeDesc* __restrict__ d, const Prefix* __restrict__ pre,
-                                int count, int64_t total, int go_plus_ge,
-                                uint8_t* __restrict__ scratch) {
+__global__ void __launch_bounds__(256) mb_table_kernel(const MergeDesc* __restrict__ d,
+                                                       const Prefix* __restrict__ pre, int count,
+                                                       int64_t total, int go_plus_ge,
+                                                       uint8_t* __restrict__ scratch) {
   const int lane = threadIdx.x & 63;
   const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (w >= total) return;
   const int i = find_merge(count, w, [&](int k) { return pre[k].item0; });
   const MergeDesc m = d[i];
-  const int64_t item = w - pre[i].item0;
-  const int Tn = m.nX + MST_WAVE - 1;
-  const int s = (int)(item / Tn);
-  const int t = (int)(item - (int64_t)s * Tn);
-  uint2* tab = (uint2*)(scratch + m.tab);
-  tab[((int64_t)s * Tn + t) * MST_WAVE + lane] =
-      table_entry((const int32_t*)(scratch + m.cx), m.nX, m.kX, (const float*)(scratch + m.p), m.nY,
-                  go_plus_ge, s, t, lane);
+  const int64_t unit = w - pre[i].item0;
+  const int n = m.nX, rows = m.nY;
+  const int Tn = n + MST_WAVE - 1;
+  const int nch = (Tn + kTabChunk - 1) / kTabChunk;
+  const int s = (int)(unit / nch);
+  const int t0 = (int)(unit - (int64_t)s * nch) * kTabChunk;
+  const int t1 = t0 + kTabChunk < Tn ? t0 + kTabChunk : Tn;
+  const float* P = (const float*)(scratch + m.p);
+  const float* FX = (const float*)(scratch + m.fx);
+  uint2* tab = (uint2*)(scratch + m.tab) + (int64_t)s * Tn * MST_WAVE;
+  float pr[8][24];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int r = s * kStripRows + lane * 8 + k;
+    const float* p = P + (int64_t)(rows - 1 - (r < rows ? r : rows - 1)) * 24;
+#pragma unroll
+    for (int a = 0; a < 24; ++a) pr[k][a] = p[a];
+  }
+  for (int t = t0; t < t1; ++t) {
+    const int c = t - lane;
+    uint32_t wd[2] = {0, 0};
+    if (c >= 0 && c < n) {
+      const float4* f4 = (const float4*)(FX + (int64_t)(n - 1 - c) * 24);
+      float fx[24];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        const float4 v = f4[q];
+        fx[4 * q] = v.x;
+        fx[4 * q + 1] = v.y;
+        fx[4 * q + 2] = v.z;
+        fx[4 * q + 3] = v.w;
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int r = s * kStripRows + lane * 8 + k;
+        float acc = 0.0f;
+#pragma unroll
+        for (int a = 0; a < 24; ++a) {
+          const float q = fx[a] * pr[k][a];
+          acc = acc + q;
+        }
+        const int v = r < rows ? (int)__builtin_rintf(acc) + go_plus_ge : 0;
+        wd[k >> 2] |= ((uint32_t)v & 0xFFu) << (8 * (k & 3));
+      }
+    }
+    tab[(int64_t)t * MST_WAVE + lane] = make_uint2(wd[0], wd[1]);
+  }
 }
 
 // Source column of every output column: sx[t] = X column or -1 (gap in X),
@@ -254,7 +276,8 @@ int64_t merge_scratch_bytes(int nX, int nY) {
   int64_t b = r256(8 * cols);  // column maps
   if (nX > 0 && nY > 0) {
     const int64_t words = flag_words(nX, nY);
-    b += r256(100 * (int64_t)nX) + r256(100 * (int64_t)nY) + r256(96 * (int64_t)nY) + r256(8 * words) +
+    b += r256(100 * (int64_t)nX) + r256(100 * (int64_t)nY) + r256(96 * (int64_t)nY) +
+         r256(96 * (int64_t)nX) + r256(8 * words) +
          r256(4 * words) + r256(cols);
   }
   return b;
@@ -340,7 +363,7 @@ hipError_t MergeEngine::enqueue(const MergeSpec* m, int count, const int8_t* d_S
     hp[i].col0 = col0;
     hp[i].yrow0 = yrow0;
     hp[i].item0 = item0;
-    q.cx = q.cy = q.p = q.tab = q.ops = 0;
+    q.cx = q.cy = q.p = q.fx = q.tab = q.ops = 0;
     q.sxy = off;
     off += r256(8 * ((int64_t)q.nX + q.nY));
     if (cls[i] < 0) {
@@ -354,6 +377,8 @@ hipError_t MergeEngine::enqueue(const MergeSpec* m, int count, const int8_t* d_S
     off += r256(100 * (int64_t)q.nY);
     q.p = off;
     off += r256(96 * (int64_t)q.nY);
+    q.fx = off;
+    off += r256(96 * (int64_t)q.nX);
     q.tab = off;
     off += r256(8 * words);
     const int64_t fl = off;
@@ -362,7 +387,7 @@ hipError_t MergeEngine::enqueue(const MergeSpec* m, int count, const int8_t* d_S
     off += r256((int64_t)q.nX + q.nY);
     col0 += (int64_t)q.nX + q.nY;
     yrow0 += q.nY;
-    item0 += (int64_t)strips_of(q.nY) * (q.nX + MST_WAVE - 1);
+    item0 += (int64_t)strips_of(q.nY) * ((q.nX + MST_WAVE - 1 + kTabChunk - 1) / kTabChunk);
     q.job = (int)next[cls[i]]++;
     AlignJob& j = h_jobs_[q.job];
     j.n = q.nX;
