@@ -728,7 +728,8 @@ __global__ void __launch_bounds__(MST_WAVE) traceback_kernel(TraceArgs a) {
         cy1 = g * AK + AK - 1;
       }
       const int gi = y / AK - cy0 / AK;
-      const uint32_t w = tile[gi][x - cx0];
+      // every lane reads the same word: keep the walk in scalar registers
+      const uint32_t w = __builtin_amdgcn_readfirstlane(tile[gi][x - cx0]);
       return (int)((w >> (4 * (y % AK))) & 15u);
     };
     int x = n - 1, y = m - 1;
@@ -736,27 +737,18 @@ __global__ void __launch_bounds__(MST_WAVE) traceback_kernel(TraceArgs a) {
     int cur = nib(x, y);
     int k = 0;
     while (cur & 12) {
-      int todo;
-      if ((prev & 12) == 4 && (prev & 2))
-        todo = 4;
-      else if ((prev & 12) == 8 && (prev & 1))
-        todo = 8;
-      else if ((cur & 12) == 4)
-        todo = 4;
-      else if ((cur & 12) == 8)
-        todo = 8;
-      else
-        todo = 12;
+      // the state machine as selects (scalar, no branches): continue an
+      // up or left run when the previous move's continuation bit says so,
+      // else take the cell's direction (11 diagonal, 01 up, 10 left)
+      const int pd = prev & 12, cd = cur & 12;
+      const bool cont_up = pd == 4 && (prev & 2);
+      const bool cont_left = pd == 8 && (prev & 1);
+      const int own = cd == 4 ? 4 : (cd == 8 ? 8 : 12);
+      const int todo = cont_up ? 4 : (cont_left ? 8 : own);
       if (lane == 0) out[k] = (uint8_t)(todo == 12 ? OP_DIAG : (todo == 8 ? OP_LEFT : OP_UP));
       ++k;
-      if (todo == 8)
-        --x;
-      else if (todo == 4)
-        --y;
-      else {
-        --x;
-        --y;
-      }
+      x -= (todo >> 3) & 1;  // 8 and 12 move left
+      y -= (todo >> 2) & 1;  // 4 and 12 move up
       prev = todo | (cur & 3);
       cur = nib(x, y);
     }
