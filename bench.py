@@ -18,6 +18,8 @@ roofline   the dominant launch's integer VALU rate: 7 algorithmic int ops per
 cpu_baseline  the oracle recurrence (oracle/liboracle.so, OpenMP over pairs)
            on a bounded random sample of the same pairs
 
+long_pairs configuration 3 (256 sequences of 8,000-20,000): all-pairs GCUPS,
+           NJ and progressive alignment times
 traceback  the alignment form (flag kernels + traceback) on 20,000 random pairs
            of the set: flag bytes written (0.5 B per cell) per second of the
            flag kernels against HBM's 8 TB/s, and the traceback's time
@@ -47,6 +49,22 @@ sys.path.insert(0, ROOT)
 PEAK_INT32_TOPS = 256 * 64 * 2.4e9 / 1e12  # 39.3 T int32 VALU ops/s
 PEAK_INT16_TOPS = 2 * PEAK_INT32_TOPS       # 78.6 T int16 ops/s (packed, 2 per lane)
 OPS_PER_CELL = 7  # algorithmic int ops per DP cell, whatever the kernel issues
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed counter profile
+    (profiles/counters_lognormal4000.md: FETCH_SIZE and WRITE_SIZE scaled by a
+    plain copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "counters_lognormal4000.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic "):
+            name, rest = line[len("traffic "):].split(" fetch_KiB=")
+            if name == kernel:
+                fetch, write = rest.split(" write_KiB=")
+                return int((float(fetch) + float(write)) * 1024)
+    return None
 
 
 def cpu_baseline(seqs, target_seconds=12.0):
@@ -125,6 +143,36 @@ def traceback_bench(M, ctx, st, L, npairs=20000, seed=7):
                                  -3100: "traceback_kernel"}[k], "jobs": p, "cells": c,
                       "ms": round(m, 3)} for k, p, c, m in stats],
         "wall_s_with_host_strings": round(wall, 3),
+    }
+
+
+def long_pairs_bench(M, ctx):
+    """Configuration 3 (256 sequences of 8,000-20,000 residues): all pairs in
+    the batched form (device time of the resident pass), then NJ and the
+    progressive alignment along its tree (wall time; 255 profile merges,
+    the longest through the tiled flag kernel)."""
+    seqs = M.synth(3, 0)
+    L = np.array([len(s) for s in seqs], dtype=np.int64)
+    cells = int((L.sum() ** 2 - (L * L).sum()) // 2)
+    st = ctx.upload(seqs)
+    ms = st.all_pairs_resident(M.FORM_BATCHED)  # first call: plan and one pass
+    ms = st.all_pairs_resident(M.FORM_BATCHED)
+    t0 = time.perf_counter()
+    joins = st.nj()
+    t1 = time.perf_counter()
+    rows = st.msa(joins)
+    t2 = time.perf_counter()
+    st.free()
+    gcups = cells / (ms * 1e-3) / 1e9
+    return {
+        "workload": "configuration 3: %d sequences, lengths %d..%d, %.3g cells (i<j)"
+                    % (len(seqs), L.min(), L.max(), cells),
+        "all_pairs_ms": round(ms, 1),
+        "all_pairs_gcups": round(gcups, 1),
+        "all_pairs_roofline_frac": round(gcups * 1e9 * OPS_PER_CELL / (PEAK_INT32_TOPS * 1e12), 4),
+        "nj_ms": round((t1 - t0) * 1e3, 1),
+        "msa_s": round(t2 - t1, 3),
+        "msa_columns": len(rows[0]) if rows else 0,
     }
 
 
@@ -245,6 +293,10 @@ def main():
                     help="what a served job computes (default: the whole run)")
     ap.add_argument("--serve-only", action="store_true", help="only the served-form measurement")
     ap.add_argument("--no-traceback", action="store_true", help="skip the alignment-form measurement")
+    ap.add_argument("--no-long", action="store_true", help="skip the configuration 3 measurement")
+    ap.add_argument("--profile-set", action="store_true",
+                    help="one pass of each profiled kernel (plain copy of 1 GiB, batched all pairs, "
+                         "alignment form) for counter runs")
     ap.add_argument("--traceback-only", action="store_true",
                     help="only the alignment-form (flags + traceback) measurement")
     args = ap.parse_args()
@@ -276,6 +328,14 @@ def main():
         ctx.tune_set(k, int(v))
         tunes[k] = int(v)
     st = ctx.upload(seqs)
+    if args.profile_set:
+        copy = ctx.copy_probe(1 << 30, 1)
+        ms = st.all_pairs_resident(M.FORM_BATCHED)
+        tb = traceback_bench(M, ctx, st, L)
+        print(json.dumps({"copy_GBps": round(copy, 1), "batched_ms": round(ms, 3),
+                          "flag_kernel_ms": tb["flag_kernel_ms"]}), flush=True)
+        ctx.close()
+        return 0
     if args.traceback_only:
         print(json.dumps({"traceback": traceback_bench(M, ctx, st, L)}), flush=True)
         ctx.close()
@@ -331,7 +391,8 @@ def main():
         "unit": "Tops/s (%d algorithmic int ops per cell; %s VALU peak)" % (opc, kind),
         "frac": round(achieved / peak, 4),
         "peak_measured": peak_meas[kind],
-        "traffic": None,
+        "traffic": profiled_traffic(kname),
+        "traffic_source": "profiles/counters_lognormal4000.md (rocprofv3 FETCH_SIZE, WRITE_SIZE, copy-calibrated)",
         "launch_ms": round(lms, 3),
         "launch_cells": lcells,
     }
@@ -363,6 +424,7 @@ def main():
         "batched_launches": launches,
         "tunables": dict(ctx.tuning, **tunes),
         "traceback": None if (args.no_traceback or args.batched_only) else traceback_bench(M, ctx, st, L),
+        "long_pairs": None if (args.no_long or args.batched_only) else long_pairs_bench(M, ctx),
         "served": None if (args.no_serve or args.batched_only) else served_bench(
             M, ctx, args.serve_jobs,
             sets=[(kv.split("=", 1)[0], float(kv.split("=", 1)[1])) for kv in args.serve_set],

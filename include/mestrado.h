@@ -184,6 +184,11 @@ int mst_server_set(mst_server* srv, const char* key, double value);
  * (packed = 1, two ops per lane and instruction), from a kernel of
  * independent add/max chains. */
 int mst_valu_peak(mst_ctx* ctx, int packed, double* tops);
+/* Measurement helper (no counterpart in the reference): a plain device copy
+ * of `bytes` (copy_kernel, 16 B per thread), best of `reps` runs; *gbps =
+ * bytes read + written per second. Its profiler byte counts calibrate those
+ * of the other kernels. */
+int mst_copy_probe(mst_ctx* ctx, int64_t bytes, int reps, double* gbps);
 
 
 /* Replaces AlignmentMatch::computePairwise (AlignmentMatch.cu:140-317) with

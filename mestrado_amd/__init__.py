@@ -46,6 +46,7 @@ def lib():
         L.mst_all_pairs_launches.argtypes = [P, C.c_int, C.c_int]
         L.mst_last_launch_stats.argtypes = [P, P, P, P, P, C.c_int, C.POINTER(C.c_int)]
         L.mst_valu_peak.argtypes = [P, C.c_int, C.POINTER(C.c_double)]
+        L.mst_copy_probe.argtypes = [P, C.c_int64, C.c_int, C.POINTER(C.c_double)]
         L.mst_tune_set.argtypes = [P, C.c_char_p, C.c_int]
         L.mst_server_create.argtypes = [P, C.c_int64, C.POINTER(P)]
         L.mst_server_destroy.argtypes = [P]
@@ -210,6 +211,12 @@ class Context:
                                          C.byref(cnt)), "mst_last_launch_stats")
         n = cnt.value
         return list(zip(k[:n].tolist(), p[:n].tolist(), c[:n].tolist(), m[:n].tolist()))
+
+    def copy_probe(self, nbytes=1 << 30, reps=3):
+        """GB/s (read + written) of a plain device copy of nbytes."""
+        g = C.c_double(0)
+        self._ck(self._L.mst_copy_probe(self.h, int(nbytes), int(reps), C.byref(g)), "mst_copy_probe")
+        return g.value
 
     def valu_peak(self, packed=False):
         """Measured integer VALU rate of the device in T ops/s (int32, or packed int16)."""
