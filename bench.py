@@ -35,11 +35,11 @@ import os
 import sys
 import time
 
-# One hardware queue per stream: the served form runs 13 streams (score
+# One hardware queue per stream: the served form runs 17 streams (score
 # rounds, NJ and merge batches); with HIP's default of 4 queues, streams
 # sharing a queue serialise and a merge waits behind unrelated score rounds.
 # Read when the runtime starts, so set before the first HIP call.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "24")
 
 import numpy as np
 
@@ -289,6 +289,8 @@ def main():
     ap.add_argument("--serve-jobs", type=int, default=2000)
     ap.add_argument("--serve-set", action="append", default=[], metavar="KEY=VALUE",
                     help="a parameter of the served form (mst_server_set), repeatable")
+    ap.add_argument("--serve-load", type=float, default=0.75,
+                    help="arrival rate of the loaded run as a fraction of the offline job rate")
     ap.add_argument("--serve-what", choices=("msa", "tree", "scores"), default="msa",
                     help="what a served job computes (default: the whole run)")
     ap.add_argument("--serve-only", action="store_true", help="only the served-form measurement")
@@ -316,7 +318,7 @@ def main():
     ctx = M.Context(device=0)
     if args.serve_only:
         served = served_bench(
-            M, ctx, args.serve_jobs,
+            M, ctx, args.serve_jobs, load=args.serve_load,
             sets=[(kv.split("=", 1)[0], float(kv.split("=", 1)[1])) for kv in args.serve_set],
             what={"msa": M.SERVE_MSA, "tree": M.SERVE_TREE, "scores": M.SERVE_SCORES}[args.serve_what])
         print(json.dumps({"served": served}), flush=True)
@@ -426,7 +428,7 @@ def main():
         "traceback": None if (args.no_traceback or args.batched_only) else traceback_bench(M, ctx, st, L),
         "long_pairs": None if (args.no_long or args.batched_only) else long_pairs_bench(M, ctx),
         "served": None if (args.no_serve or args.batched_only) else served_bench(
-            M, ctx, args.serve_jobs,
+            M, ctx, args.serve_jobs, load=args.serve_load,
             sets=[(kv.split("=", 1)[0], float(kv.split("=", 1)[1])) for kv in args.serve_set],
             what={"msa": M.SERVE_MSA, "tree": M.SERVE_TREE, "scores": M.SERVE_SCORES}[args.serve_what]),
         "cpu_baseline": None if args.no_cpu else cpu_baseline(seqs),

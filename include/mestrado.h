@@ -119,13 +119,13 @@ typedef struct {
  * earliest deadline first, packed into shared launches of the batched form's
  * kernels; NJ launches for the jobs whose scores are done (a workgroup per
  * job); merge batches of the ready profile merges of all jobs in their
- * alignment stage (up to four NJ launches and four merge batches in
+ * alignment stage (up to four NJ launches and eight merge batches in
  * flight). Results stay on the device until copied out; finished jobs hold
  * their memory until mst_server_result frees them. A job larger
  * than the budget is rejected at submission. Jobs are independent: a job's
  * results are identical alone and in any mix. Not thread-safe. The server
- * uses 13 streams; give the process as many hardware queues
- * (GPU_MAX_HW_QUEUES=16 in the environment before the first HIP call), or
+ * uses 17 streams; give the process as many hardware queues
+ * (GPU_MAX_HW_QUEUES=24 in the environment before the first HIP call), or
  * streams sharing a queue serialise. */
 typedef struct mst_server mst_server;
 enum { MST_JOB_WAITING = 0, MST_JOB_RUNNING = 1, MST_JOB_DONE = 2, MST_JOB_CANCELLED = 3,
@@ -177,7 +177,11 @@ int mst_server_stats(mst_server* srv, int* waiting, int* running, int* inflight,
  * those stages' launches find free compute units sooner at some cost in
  * throughput), "nj_ratio" (NJ rebuild ratio,
  * default 0.5),
- * "merge_bytes" (scratch of one merge batch, default 512 MiB). */
+ * "merge_bytes" (scratch of one merge batch, default 512 MiB), "small_cells"
+ * (merges of at most this many cells batch apart from larger ones, default
+ * 4e6), "rounds" (score rounds kept in flight, 1..3, default 3),
+ * "persistent" (1: score launches of as many workgroups as fit, each taking
+ * items until none are left; 0: a workgroup per item; default 1). */
 int mst_server_set(mst_server* srv, const char* key, double value);
 /* Measurement helper (no counterpart in the reference): integer VALU rate of
  * the context's device in T ops/s, int32 (packed = 0) or packed int16

@@ -22,8 +22,9 @@
 // launches of different kernel classes run side by side on four side streams
 // (default priority) after the round's copies on the context's stream; a
 // round is over when every stream it used has passed its end event, and the
-// next round's copies do not wait for it. NJ launches and merge batches run
-// on four streams each, one batch per stream in flight, so a long NJ or a
+// next round's copies do not wait for it. NJ launches run on four streams and
+// merge batches on eight, one batch per stream in flight, and a merge batch
+// holds small merges (up to small_cells) or large ones, so a long NJ or a
 // large merge does not hold back the small jobs behind it.
 #include <algorithm>
 #include <chrono>
@@ -173,7 +174,8 @@ struct mst_server {
   int2* d_bufs = nullptr;  // int32 kernel strip buffers
   int64_t bufs_cap_cols = 0;
   int grid32 = 1;
-  static constexpr int kBufs = 3;  // rounds in flight at most
+  static constexpr int kBufs = 3;  // round buffers: rounds in flight at most
+  int max_rounds = 3;              // rounds kept in flight (1..kBufs)
   RoundBuf rb[kBufs];
   hipStream_t side[4] = {nullptr, nullptr, nullptr, nullptr};  // class launches of a round
   std::map<int64_t, Job> jobs;
@@ -185,6 +187,7 @@ struct mst_server {
   double round_cells = 2e10;   // work per round (about 4 ms of the device)
   double round_cells_shared = 2e10;  // per round while jobs are in later stages
   double score_share = 1.0;          // share of the score kernels' grid meanwhile
+  int persistent = 1;                // persistent score launches (else one workgroup per item)
   double slack = 2.0;          // deadline = submission + slack x estimated run time
   double t_retire = 0;         // when the last round was seen finished
   int chunk = 8;
@@ -206,6 +209,7 @@ struct mst_server {
   };
   static constexpr int kSlots = 4;
   NjSlot nj[kSlots];
+  static constexpr int kMsaSlots = 8;
   // merges: batches in flight (one per slot)
   double merge_bytes = (double)(512 << 20);
   struct MsaSlot {
@@ -217,7 +221,8 @@ struct mst_server {
     std::vector<int64_t> jobs;                    // jobs with work in the batch
     std::vector<int64_t> leaf_jobs;               // jobs whose leaves the batch writes
   };
-  MsaSlot ms[kSlots];
+  MsaSlot ms[kMsaSlots];
+  double small_cells = 4e6;  // merges up to this many cells batch apart from larger ones
 };
 
 namespace {
@@ -441,7 +446,14 @@ int admit(mst_server* S, Job& J) {
   J.state = MST_JOB_RUNNING;
   J.stage = kStageScores;
   J.t_admit = now_s();
-  J.deadline = J.t_submit + S->slack * ((double)J.cells_total / S->rate + 0.0005);
+  // deadline: slack x the job's estimated run time alone, all its stages:
+  // scores at the measured rate; NJ on one workgroup ~ 34 ms x (N/512)^2;
+  // the alignment ~ 60 us per sequence (its levels, each a merge batch)
+  double est = (double)J.cells_total / S->rate + 0.0005;
+  const double nn = (double)N / 512.0;
+  if ((J.what & MST_SERVE_TREE) == MST_SERVE_TREE && N >= 2) est += 0.034 * nn * nn;
+  if ((J.what & MST_SERVE_MSA) == MST_SERVE_MSA) est += 60e-6 * N;
+  J.deadline = J.t_submit + S->slack * est;
   J.ascii.clear();
   J.ascii.shrink_to_fit();
   return MST_OK;
@@ -597,8 +609,12 @@ int launch_round(mst_server* S, int* launched) {
       launch_score6(a, (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)(S->grid6 * share), nit)), st);
     } else {
       const int W = (c - 1) / 3 + 1, K = (c - 1) % 3 + 3;
-      const int64_t g =
-          std::min<int64_t>((int64_t)(ctx->num_cus * score5_groups_per_cu(K, W) * share), nit);
+      // persistent: as many workgroups as fit, each taking items until none
+      // are left; else a workgroup per item, so workgroups retire throughout
+      // the launch and other stages' kernels find free slots
+      const int64_t g = S->persistent
+                            ? std::min<int64_t>((int64_t)(ctx->num_cus * score5_groups_per_cu(K, W) * share), nit)
+                            : nit;
       launch_score5(K, W, a, (int)std::max<int64_t>(g, 1), st);
     }
     CK(hipGetLastError());
@@ -889,6 +905,7 @@ int launch_merges(mst_server* S) {
     const int64_t budget = (int64_t)S->merge_bytes;
     int64_t bytes = 0;
     bool full = false;
+    int size_class = -1;  // a batch holds small merges or large ones, not both
     for (int64_t id : order) {
       Job& J = S->jobs[id];
       bool touch = false;
@@ -913,6 +930,8 @@ int launch_merges(mst_server* S) {
         Group& X = J.groups[jn.left];
         Group& Y = J.groups[jn.right];
         if (!X.ready || !Y.ready) continue;
+        const int cls = (double)X.len * Y.len <= S->small_cells ? 0 : 1;
+        if (size_class >= 0 && cls != size_class) continue;
         const int64_t b = merge_scratch_bytes(X.len, Y.len);
         if (!spec.empty() && bytes + b > budget) {
           full = true;
@@ -934,6 +953,7 @@ int launch_merges(mst_server* S) {
         Z.in_arena = in_arena;
         Z.k = X.k + Y.k;
         bytes += b;
+        size_class = cls;
         J.merge_state[k] = 1;
         spec.push_back(MergeSpec{S->d_arena + X.off, X.k, X.len, S->d_arena + Y.off, Y.k, Y.len,
                                  S->d_arena + Z.off});
@@ -1171,7 +1191,7 @@ extern "C" int mst_server_step(mst_server* S, int* ncompleted) {
     S->waiting.pop_front();
   }
   // keep up to three rounds queued on the device
-  while ((int)S->inflight.size() < mst_server::kBufs) {
+  while ((int)S->inflight.size() < S->max_rounds) {
     int launched = 0;
     if ((rc = launch_round(S, &launched))) return rc;
     if (!launched) break;
@@ -1353,6 +1373,18 @@ extern "C" int mst_server_set(mst_server* S, const char* key, double value) {
   }
   if (!strcmp(key, "score_share") && value > 0 && value <= 1) {
     S->score_share = value;
+    return MST_OK;
+  }
+  if (!strcmp(key, "rounds") && value >= 1 && value <= mst_server::kBufs) {
+    S->max_rounds = (int)value;
+    return MST_OK;
+  }
+  if (!strcmp(key, "small_cells") && value >= 0) {
+    S->small_cells = value;
+    return MST_OK;
+  }
+  if (!strcmp(key, "persistent") && (value == 0 || value == 1)) {
+    S->persistent = (int)value;
     return MST_OK;
   }
   if (!strcmp(key, "nj_ratio") && value >= 0 && value <= 1) {

@@ -97,10 +97,11 @@ __device__ __forceinline__ void p_row(const int32_t* __restrict__ countsY, int k
 // forward Y column m-1-r and DP column c is forward X column n-1-c.
 //
 // A wave takes one strip and a chunk of kTabChunk steps: each lane keeps the
-// P rows of its 8 DP rows in registers for the whole chunk and reads one X
-// frequency row (24 floats, fX = count / kX, written by the counts kernel)
-// per step, so P is read once per chunk instead of once per entry.
-constexpr int kTabChunk = 256;
+// P rows of 4 of its 8 DP rows in registers for the whole chunk (two passes)
+// and reads one X frequency row (24 floats, fX = count / kX, written by the
+// counts kernel) per step, so P is read once per chunk instead of once per
+// entry.
+constexpr int kTabChunk = 64;
 
 __global__ void mb_counts_kernel(const MergeDesc* __restrict__ d, const Prefix* __restrict__ pre,
                                  int count, int64_t total, uint8_t* __restrict__ scratch) {
@@ -149,42 +150,48 @@ __global__ void __launch_bounds__(256) mb_table_kernel(const MergeDesc* __restri
   const float* P = (const float*)(scratch + m.p);
   const float* FX = (const float*)(scratch + m.fx);
   uint2* tab = (uint2*)(scratch + m.tab) + (int64_t)s * Tn * MST_WAVE;
-  float pr[8][24];
+  // two passes of 4 rows (the low and the high dword of each entry), so a
+  // lane holds 96 floats of P: ~140 VGPRs, room for other kernels' waves
+#pragma unroll 1
+  for (int half = 0; half < 2; ++half) {
+    float pr[4][24];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const int r = s * kStripRows + lane * 8 + k;
-    const float* p = P + (int64_t)(rows - 1 - (r < rows ? r : rows - 1)) * 24;
+    for (int k = 0; k < 4; ++k) {
+      const int r = s * kStripRows + lane * 8 + half * 4 + k;
+      const float* p = P + (int64_t)(rows - 1 - (r < rows ? r : rows - 1)) * 24;
 #pragma unroll
-    for (int a = 0; a < 24; ++a) pr[k][a] = p[a];
-  }
-  for (int t = t0; t < t1; ++t) {
-    const int c = t - lane;
-    uint32_t wd[2] = {0, 0};
-    if (c >= 0 && c < n) {
-      const float4* f4 = (const float4*)(FX + (int64_t)(n - 1 - c) * 24);
-      float fx[24];
-#pragma unroll
-      for (int q = 0; q < 6; ++q) {
-        const float4 v = f4[q];
-        fx[4 * q] = v.x;
-        fx[4 * q + 1] = v.y;
-        fx[4 * q + 2] = v.z;
-        fx[4 * q + 3] = v.w;
-      }
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const int r = s * kStripRows + lane * 8 + k;
-        float acc = 0.0f;
-#pragma unroll
-        for (int a = 0; a < 24; ++a) {
-          const float q = fx[a] * pr[k][a];
-          acc = acc + q;
-        }
-        const int v = r < rows ? (int)__builtin_rintf(acc) + go_plus_ge : 0;
-        wd[k >> 2] |= ((uint32_t)v & 0xFFu) << (8 * (k & 3));
-      }
+      for (int a = 0; a < 24; ++a) pr[k][a] = p[a];
     }
-    tab[(int64_t)t * MST_WAVE + lane] = make_uint2(wd[0], wd[1]);
+    uint32_t* tab32 = (uint32_t*)tab + half;
+    for (int t = t0; t < t1; ++t) {
+      const int c = t - lane;
+      uint32_t wd = 0;
+      if (c >= 0 && c < n) {
+        const float4* f4 = (const float4*)(FX + (int64_t)(n - 1 - c) * 24);
+        float fx[24];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+          const float4 v = f4[q];
+          fx[4 * q] = v.x;
+          fx[4 * q + 1] = v.y;
+          fx[4 * q + 2] = v.z;
+          fx[4 * q + 3] = v.w;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int r = s * kStripRows + lane * 8 + half * 4 + k;
+          float acc = 0.0f;
+#pragma unroll
+          for (int a = 0; a < 24; ++a) {
+            const float q = fx[a] * pr[k][a];
+            acc = acc + q;
+          }
+          const int v = r < rows ? (int)__builtin_rintf(acc) + go_plus_ge : 0;
+          wd |= ((uint32_t)v & 0xFFu) << (8 * k);
+        }
+      }
+      tab32[((int64_t)t * MST_WAVE + lane) * 2] = wd;
+    }
   }
 }
 
