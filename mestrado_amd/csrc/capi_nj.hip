@@ -45,6 +45,10 @@ void mst::nj_to_join(const NjJoin& in, mst_join* out) {
   j.len_right = R;
 }
 
+// Matrices from this size on run as launches per step (nj_scan_kernel over
+// the device); smaller ones in one workgroup (nj_kernel).
+constexpr int kNjStepsMinN = 256;
+
 // Runs NJ on the device matrix ctx->d_nj (n x n, filled by the caller).
 static int run_nj(mst_ctx* ctx, int n, float ratio, mst_join* joins) {
   std::vector<int> init(2 * (size_t)n, 0);
@@ -67,7 +71,17 @@ static int run_nj(mst_ctx* ctx, int n, float ratio, mst_join* joins) {
   a.ids = ctx->d_nj_act + 2 * n;
   a.joins = (NjJoin*)ctx->d_nj_joins;
   a.ratio = ratio;
-  CK(launch_nj(a, ctx->stream));
+  if (n >= kNjStepsMinN) {
+    // large N: one launch per step, the scan over the whole device
+    const int nparts = ctx->num_cus * 2;
+    rc = ctx->ensure(&ctx->d_nj_state, &ctx->nj_state_cap, nj_state_bytes());
+    if (rc) return rc;
+    rc = ctx->ensure(&ctx->d_nj_parts, &ctx->nj_parts_cap, nj_part_bytes(nparts));
+    if (rc) return rc;
+    CK(run_nj_steps(a, ctx->d_nj_state, ctx->d_nj_parts, nparts, ctx->stream));
+  } else {
+    CK(launch_nj(a, ctx->stream));
+  }
   std::vector<NjJoin> hj(n > 1 ? n - 1 : 1);
   if (n > 1)
     CK(hipMemcpyAsync(hj.data(), ctx->d_nj_joins, sizeof(NjJoin) * (n - 1), hipMemcpyDeviceToHost,

@@ -28,6 +28,13 @@ struct NjArgs {
 hipError_t launch_nj(const NjArgs& a, hipStream_t st);
 // One workgroup per matrix: d_args[0 .. count) in device memory.
 hipError_t launch_nj_batch(const NjArgs* d_args, int count, hipStream_t st);
+// The join loop as launches per step: the O(m^2) scan on `nparts`
+// workgroups, the join on one (large N, one matrix). `state` and `parts`:
+// device buffers of nj_state_bytes() and nj_part_bytes(nparts); act/ids set
+// as for launch_nj.
+hipError_t run_nj_steps(const NjArgs& a, void* state, void* parts, int nparts, hipStream_t st);
+int64_t nj_state_bytes();
+int64_t nj_part_bytes(int nparts);
 // act/ids of NjArgs before the first step
 void launch_nj_init(int* act, int* ids, int n, hipStream_t st);
 // D = (float)scores, n elements (the NJ input of a score matrix)

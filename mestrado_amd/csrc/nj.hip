@@ -46,8 +46,27 @@ __device__ __forceinline__ bool better(const Best& a, const Best& b) {
   return a.I > b.I;
 }
 
+// New row of a join, kept in LDS as it is written (up to kNjLdsRow active
+// nodes), so the fp64 sum that gives T of the new node, which must run in id
+// order on one thread, reads LDS instead of a chain of dependent loads.
+constexpr int kNjLdsRow = 8192;
+
+__device__ __forceinline__ double new_row_sum(const float* rowv, const float* Drow, const int* act, int m,
+                                              int pa, int pb) {
+  double s = 0.0;
+  if (m <= kNjLdsRow) {
+    for (int p = 0; p < m; ++p)
+      if (p != pa && p != pb) s += (double)rowv[p];
+  } else {
+    for (int p = 0; p < m; ++p)
+      if (p != pa && p != pb) s += (double)Drow[act[p]];
+  }
+  return s;
+}
+
 // The whole join loop of one matrix, run by one workgroup of NJ_THREADS.
 __device__ __forceinline__ void nj_run(const NjArgs a) {
+  __shared__ float rowv[kNjLdsRow];
   __shared__ Best red[NJ_THREADS / 64];
   __shared__ Best win;
   __shared__ int s_m, s_R, s_steps, s_next;
@@ -153,16 +172,10 @@ __device__ __forceinline__ void nj_run(const NjArgs a) {
       T[rk] += (double)dlt;
       D[(int64_t)ra * ld + rk] = tmp;
       D[(int64_t)rk * ld + ra] = tmp;
+      if (m <= kNjLdsRow) rowv[p] = tmp;
     }
     __syncthreads();
-    if (tid == 0) {
-      double s = 0.0;
-      for (int p = 0; p < m; ++p) {
-        if (p == w.pa || p == w.pb) continue;
-        s += (double)D[(int64_t)ra * ld + act[p]];
-      }
-      T[ra] = s;
-    }
+    if (tid == 0) T[ra] = new_row_sum(rowv, D + (int64_t)ra * ld, act, m, w.pa, w.pb);
     __syncthreads();
     // ---- compact the active array: drop pa, pb, append the new node ----
     for (int p = tid; p < m; p += NJ_THREADS) {
@@ -209,6 +222,217 @@ __device__ __forceinline__ void nj_run(const NjArgs a) {
     __syncthreads();
   }
 }
+
+// ---- one launch per step, the scan over the whole device (large N) ----
+//
+// The same steps as nj_run, split so that the O(m^2) scan of a step runs on
+// many workgroups: nj_scan_kernel (every wave takes rows pa round robin over
+// the grid, a best candidate per workgroup), then nj_join_kernel (one
+// workgroup: the winner from the partial bests, the join record, the new row,
+// T of the new node, the compacted active list, the bookkeeping), then, at
+// the steps the host's copy of the rebuild schedule names, nj_rebuild_kernel
+// (T of every active row, a thread per row). The partial bests reduce with
+// the same total order as the workgroup reduction, so the winner, and every
+// value after it, equals nj_run's.
+
+struct NjState {
+  int m, R, next, flip;  // active count, rebuild marker, next node id, active half
+};
+
+__device__ __forceinline__ Best wave_best(Best best) {
+  for (int off = 32; off > 0; off >>= 1) {
+    Best o;
+    o.q = __shfl_down(best.q, off);
+    o.J = __shfl_down(best.J, off);
+    o.I = __shfl_down(best.I, off);
+    o.pa = __shfl_down(best.pa, off);
+    o.pb = __shfl_down(best.pb, off);
+    if (better(o, best)) best = o;
+  }
+  return best;
+}
+
+// T[r] = sum over j of D(j, r) in fp64, j in storage order (all nodes active)
+__global__ void nj_rowsum_kernel(const float* __restrict__ D, int ld, int n0, double* __restrict__ T) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n0) return;
+  double s = 0.0;
+  for (int j = 0; j < n0; ++j) s += (double)D[(int64_t)j * ld + r];
+  T[r] = s;
+}
+
+constexpr int NJ_SCAN_THREADS = 256;
+
+__global__ void __launch_bounds__(NJ_SCAN_THREADS) nj_scan_kernel(NjArgs a, const NjState* __restrict__ st,
+                                                                  Best* __restrict__ part) {
+  __shared__ Best red[NJ_SCAN_THREADS / 64];
+  const NjState S = *st;
+  const int m = S.m, R = S.R;
+  const int* act = a.act + (S.flip ? a.n0 : 0);
+  const int* ids = a.ids + (S.flip ? a.n0 : 0);
+  const float* D = a.D;
+  const double* T = a.T;
+  const int ld = a.ld;
+  const double nm2 = (double)(m - 2);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nwaves = gridDim.x * (NJ_SCAN_THREADS / 64);
+  Best best;
+  best.q = DBL_MAX;
+  best.J = -1;
+  best.I = -1;
+  best.pa = best.pb = -1;
+  for (int pa = blockIdx.x * (NJ_SCAN_THREADS / 64) + wave; pa < m - 1; pa += nwaves) {
+    const int ra = act[pa], ia = ids[pa];
+    const double Ta = T[ra];
+    const float* Drow = D + (int64_t)ra * ld;
+    for (int pb = pa + 1 + lane; pb < m; pb += 64) {
+      const int rb = act[pb], ib = ids[pb];
+      const double dq = (double)Drow[rb] * nm2;
+      const bool swap = (ib >= R) && (rb < ra);
+      const double Tb = T[rb];
+      const double q = swap ? (dq - Tb) - Ta : (dq - Ta) - Tb;
+      Best c;
+      c.q = q;
+      c.I = ia;
+      c.J = ib;
+      c.pa = pa;
+      c.pb = pb;
+      if (better(c, best)) best = c;
+    }
+  }
+  best = wave_best(best);
+  if (lane == 0) red[wave] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    Best b = red[0];
+    for (int w = 1; w < NJ_SCAN_THREADS / 64; ++w)
+      if (better(red[w], b)) b = red[w];
+    part[blockIdx.x] = b;
+  }
+}
+
+__global__ void __launch_bounds__(NJ_THREADS) nj_join_kernel(NjArgs a, NjState* __restrict__ st,
+                                                             const Best* __restrict__ part, int nparts,
+                                                             int step, int rebuild) {
+  __shared__ float rowv[kNjLdsRow];
+  __shared__ Best red[NJ_THREADS / 64];
+  __shared__ Best win;
+  const int tid = threadIdx.x;
+  const NjState S = *st;
+  const int n0 = a.n0, m = S.m, ld = a.ld;
+  float* D = a.D;
+  double* T = a.T;
+  const int* act = a.act + (S.flip ? n0 : 0);
+  const int* ids = a.ids + (S.flip ? n0 : 0);
+  int* act2 = a.act + (S.flip ? 0 : n0);
+  int* ids2 = a.ids + (S.flip ? 0 : n0);
+  Best best;
+  best.q = DBL_MAX;
+  best.J = -1;
+  best.I = -1;
+  best.pa = best.pb = -1;
+  for (int k = tid; k < nparts; k += NJ_THREADS)
+    if (better(part[k], best)) best = part[k];
+  best = wave_best(best);
+  if ((tid & 63) == 0) red[tid >> 6] = best;
+  __syncthreads();
+  if (tid == 0) {
+    Best b = red[0];
+    for (int w = 1; w < NJ_THREADS / 64; ++w)
+      if (better(red[w], b)) b = red[w];
+    win = b;
+  }
+  __syncthreads();
+  const Best w = win;
+  const int ra = act[w.pa], rb = act[w.pb];
+  const float d = D[(int64_t)ra * ld + rb];
+  if (tid == 0) {
+    NjJoin& J = a.joins[step];
+    J.node = S.next;
+    J.left = w.I;
+    J.right = w.J;
+    J.d = d;
+    J.Ti = T[ra];
+    J.Tj = T[rb];
+    J.n = m;
+    J.q = w.q;
+  }
+  __syncthreads();
+  for (int p = tid; p < m; p += NJ_THREADS) {
+    if (p == w.pa || p == w.pb) continue;
+    const int rk = act[p];
+    const float Dik = D[(int64_t)ra * ld + rk];
+    const float Djk = D[(int64_t)rb * ld + rk];
+    const float tmp = ((Dik + Djk) - d) / 2;
+    const float dlt = tmp - (Dik + Djk);
+    T[rk] += (double)dlt;
+    D[(int64_t)ra * ld + rk] = tmp;
+    D[(int64_t)rk * ld + ra] = tmp;
+    if (m <= kNjLdsRow) rowv[p] = tmp;
+  }
+  __syncthreads();
+  if (tid == 0) T[ra] = new_row_sum(rowv, D + (int64_t)ra * ld, act, m, w.pa, w.pb);
+  for (int p = tid; p < m; p += NJ_THREADS) {
+    if (p == w.pa || p == w.pb) continue;
+    const int np = p - (p > w.pa) - (p > w.pb);
+    act2[np] = act[p];
+    ids2[np] = ids[p];
+  }
+  if (tid == 0) {
+    act2[m - 2] = ra;
+    ids2[m - 2] = S.next;
+    NjState o = S;
+    if (rebuild) o.R = S.next + 1;
+    o.m = m - 1;
+    o.next = S.next + 1;
+    o.flip = S.flip ^ 1;
+    *st = o;
+  }
+}
+
+// T of every active row after a rebuild step (the active list just compacted)
+__global__ void nj_rebuild_kernel(NjArgs a, const NjState* __restrict__ st) {
+  const NjState S = *st;
+  const int m1 = S.m;
+  const int* act = a.act + (S.flip ? a.n0 : 0);
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= m1) return;
+  const int r = act[p];
+  double s = 0.0;
+  for (int j = 0; j < m1; ++j) s += (double)a.D[(int64_t)act[j] * a.ld + r];
+  a.T[r] = s;
+}
+
+hipError_t run_nj_steps(const NjArgs& a, void* state, void* parts, int nparts, hipStream_t st) {
+  const int n0 = a.n0;
+  NjState init{n0, n0, n0, 0};
+  hipError_t e = hipMemcpyAsync(state, &init, sizeof init, hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(nj_rowsum_kernel, dim3((n0 + 255) / 256), dim3(256), 0, st, a.D, a.ld, n0, a.T);
+  // the rebuild schedule of nj_run (SpeciesBlock.cpp:864,1225-1247)
+  int steps = (int)((float)n0 * a.ratio);
+  steps = steps > 200 ? steps : 200;
+  for (int step = 0; step < n0 - 1; ++step) {
+    const int m = n0 - step;
+    const bool rebuild = steps == 0;
+    hipLaunchKernelGGL(nj_scan_kernel, dim3(nparts), dim3(NJ_SCAN_THREADS), 0, st, a, (const NjState*)state,
+                       (Best*)parts);
+    hipLaunchKernelGGL(nj_join_kernel, dim3(1), dim3(NJ_THREADS), 0, st, a, (NjState*)state,
+                       (const Best*)parts, nparts, step, rebuild ? 1 : 0);
+    if (rebuild) {
+      const int m1 = m - 1;
+      hipLaunchKernelGGL(nj_rebuild_kernel, dim3((m1 + 255) / 256), dim3(256), 0, st, a, (const NjState*)state);
+      steps = (m1 < 200) ? INT_MAX : (int)((float)m1 * a.ratio);
+    } else {
+      steps -= 1;
+    }
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+int64_t nj_state_bytes() { return sizeof(NjState); }
+int64_t nj_part_bytes(int nparts) { return (int64_t)sizeof(Best) * nparts; }
 
 __global__ void __launch_bounds__(NJ_THREADS) nj_kernel(NjArgs a) { nj_run(a); }
 
