@@ -2,7 +2,8 @@
 
 Searches the tunables of include/mestrado.h (mst_tune_set: "chunk" column
 sequences per work item, "shared3" routing of rows 129..320, "wg5"
-workgroups per CU of the streamed kernel) on a workload, timing each
+workgroups per CU of the streamed kernel, "k5min" least rows per lane of its
+one-wave classes) on a workload, timing each
 candidate with the resident all-pairs pass (device time, plan and graph built
 outside the timed region), and persists the best per device architecture and
 length bucket in mestrado_amd/tuning/<arch>.json:
@@ -25,7 +26,7 @@ import sys
 import numpy as np
 
 TUNING_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning")
-KEYS = ("chunk", "shared3", "wg5")
+KEYS = ("chunk", "shared3", "wg5", "k5min")
 BUCKETS = ((256, "<=256"), (1024, "<=1024"), (None, ">1024"))
 
 
@@ -80,8 +81,8 @@ def apply_for(ctx, lengths):
 
 
 def candidates():
-    for shared3, chunk, wg5 in itertools.product((0, 1), (8, 16, 32, 64), (0,)):
-        yield {"shared3": shared3, "chunk": chunk, "wg5": wg5}
+    for shared3, chunk, wg5, k5min in itertools.product((0, 1), (8, 16, 32, 64), (0,), (1, 2, 3)):
+        yield {"shared3": shared3, "chunk": chunk, "wg5": wg5, "k5min": k5min}
 
 
 def tune(ctx, seqs, reps=2, log=None):
@@ -106,7 +107,7 @@ def tune(ctx, seqs, reps=2, log=None):
 def save(arch, bucket, best, workload, default=False):
     os.makedirs(TUNING_DIR, exist_ok=True)
     t = load(arch) or {"arch": arch_key(arch), "buckets": {}}
-    entry = {k: best[k] for k in KEYS}
+    entry = {k: best[k] for k in KEYS if k in best}
     entry["ms"] = best["ms"]
     entry["workload"] = workload
     t["buckets"][bucket] = entry

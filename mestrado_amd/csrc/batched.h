@@ -23,6 +23,7 @@ struct Tune {
   int wg_per_cu3[17] = {0};  // score3 workgroups per CU by rows-per-lane K (0: default)
   int wg_per_cu2 = 0;        // score2 workgroups per CU (0: default)
   int wg5 = 0;               // score5 workgroups per CU (0: as many as LDS allows)
+  int k5min = 1;             // least rows per lane of the one-wave streamed kernel (1..3)
 };
 
 struct BatchedPlan {

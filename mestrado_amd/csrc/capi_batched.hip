@@ -70,7 +70,7 @@ bool pk16_usable(const mst_ctx* ctx) {
 
 // Classes of the planner, by rows m of the row pair (the longer of Ya, Yb):
 //  * m <= 1024: the pipelined packed kernel (score5), W = ceil(m/256) waves of
-//    64*K rows, K >= 3 (so rows 1..320 are one wave of K = 3..5);
+//    64*K rows, K >= tune.k5min for one wave (rows 1..320: K = k5min..5);
 //  * with tune.shared3 = 1, rows 129..320 take the shared-profile kernel
 //    (score3) instead, in its classes K = 3..5;
 //  * m > 1024: the multi-sweep packed kernel (score6), up to the 16-bit bound;
@@ -141,7 +141,7 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
       if (ctx->tune.shared3 && K >= kPackedMinK) {
         kind = 3;
       } else if (kPipelined) {
-        K = std::max(K, kPackedMinK);  // one wave, columns streamed
+        K = std::max(K, ctx->tune.k5min);  // one wave, columns streamed
         kind = 5;
       } else {
         kind = 0;
@@ -565,6 +565,7 @@ static int* tune_slot(mst_ctx* ctx, const char* key) {
   if (!strcmp(key, "chunk")) return &ctx->tune.chunk;
   if (!strcmp(key, "shared3")) return &ctx->tune.shared3;
   if (!strcmp(key, "wg5")) return &ctx->tune.wg5;
+  if (!strcmp(key, "k5min")) return &ctx->tune.k5min;
   if (!strcmp(key, "wg2")) return &ctx->tune.wg_per_cu2;
   if (!strncmp(key, "wg3.", 4)) {
     const int k = atoi(key + 4);
@@ -581,6 +582,8 @@ extern "C" int mst_tune_set(mst_ctx* ctx, const char* key, int value) {
     return ctx->fail(MST_E_ARG, "mst_tune_set", "chunk out of range 1..4096");
   if (!strcmp(key, "shared3") && value != 0 && value != 1)
     return ctx->fail(MST_E_ARG, "mst_tune_set", "shared3 is 0 or 1");
+  if (!strcmp(key, "k5min") && (value < 1 || value > 3))
+    return ctx->fail(MST_E_ARG, "mst_tune_set", "k5min out of range 1..3");
   if (strcmp(key, "chunk") && strcmp(key, "shared3") && (value < 0 || value > 64))
     return ctx->fail(MST_E_ARG, "mst_tune_set", "workgroups per CU out of range 0..64");
   if (*slot == value) return MST_OK;

@@ -7,7 +7,7 @@
 // and columns are laid out (device parts in nw_stream.h):
 //
 //  * rows: a pair of row sequences with m rows is cut into W strips of 64*K
-//    rows (K = 3..5 rows per lane, so no lane holds many rows); wave j of a
+//    rows (K = 1..5 rows per lane, so no lane holds many rows); wave j of a
 //    pipeline owns strip j. Lane 63 of wave j writes F and H of its last row for
 //    every column into an LDS ring; lane 0 of wave j+1 reads them one column
 //    behind. Progress counters in LDS (columns produced, columns consumed)
@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(MST_WAVE * W * Pipes<W>::P) score5_kernel(Scor
 }
 
 bool score5_supported(int K, int W) {
-  if (W == 1) return K >= 3 && K <= 5;
+  if (W == 1) return K >= 1 && K <= 5;
   return W >= 2 && W <= 4 && K >= 3 && K <= 4;
 }
 
@@ -137,6 +137,8 @@ void launch_score5(int K, int W, const Score5Args& a, int grid, hipStream_t st) 
                        a);                                                                      \
     return;                                                                                     \
   }
+  MST_CASE(1, 1)
+  MST_CASE(2, 1)
   MST_CASE(3, 1)
   MST_CASE(4, 1)
   MST_CASE(5, 1)

@@ -32,4 +32,4 @@ def test_committed_tables_are_well_formed():
         t = json.load(open(os.path.join(T.TUNING_DIR, name)))
         assert t["default"] in t["buckets"]
         for entry in t["buckets"].values():
-            assert set(T.KEYS) <= set(entry)
+            assert set(entry) - {"ms", "workload"} <= set(T.KEYS) and "chunk" in entry

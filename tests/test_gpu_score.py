@@ -128,6 +128,28 @@ def test_batched_tunables_keep_results(ctx, shared3, chunk):
         ctx.tune_set("chunk", 0)
 
 
+@pytest.mark.parametrize("k5min", [1, 2])
+def test_batched_one_wave_small_k(ctx, k5min):
+    # short row pairs with 1 or 2 rows per lane (64 or 128 rows per wave)
+    rng = np.random.default_rng(30 + k5min)
+    lens = list(rng.integers(1, 200, 60)) + [1, 2, 63, 64, 65, 127, 128, 129, 300, 700]
+    seqs = [_rand(rng, int(L)) for L in lens]
+    o, _ = O.all_pairs(seqs, S)
+    st = ctx.upload(seqs)
+    saved = ctx.tune_get("k5min")
+    try:
+        ctx.tune_set("k5min", k5min)
+        g = st.all_pairs(M.FORM_BATCHED)
+        assert (g == o).all()
+        classes = {k for k, _, _, _ in st.launch_stats(M.FORM_BATCHED)}
+        assert -(1000 + 100 + k5min) in classes
+    finally:
+        ctx.tune_set("k5min", saved)
+        st.free()
+    with pytest.raises(M.MestradoError):
+        ctx.tune_set("k5min", 0)
+
+
 @pytest.mark.parametrize("seed,n,lo,hi", [(21, 9, 1025, 2100), (22, 14, 800, 1600)])
 def test_batched_multi_sweep_rows(ctx, seed, n, lo, hi):
     # rows beyond 1024: four waves sweep the column stream once per four
