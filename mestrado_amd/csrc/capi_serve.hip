@@ -189,6 +189,7 @@ struct mst_server {
   double score_share = 1.0;          // share of the score kernels' grid meanwhile
   int persistent = 1;                // persistent score launches (else one workgroup per item)
   double slack = 2.0;          // deadline = submission + slack x estimated run time
+  double deadline_add = 0.0;   //   + this many seconds
   double t_retire = 0;         // when the last round was seen finished
   int chunk = 8;
   int64_t done_count = 0, cells_done = 0;
@@ -453,7 +454,7 @@ int admit(mst_server* S, Job& J) {
   const double nn = (double)N / 512.0;
   if ((J.what & MST_SERVE_TREE) == MST_SERVE_TREE && N >= 2) est += 0.034 * nn * nn;
   if ((J.what & MST_SERVE_MSA) == MST_SERVE_MSA) est += 60e-6 * N;
-  J.deadline = J.t_submit + S->slack * est;
+  J.deadline = J.t_submit + S->slack * est + S->deadline_add;
   J.ascii.clear();
   J.ascii.shrink_to_fit();
   return MST_OK;
@@ -1359,7 +1360,7 @@ extern "C" int mst_server_set(mst_server* S, const char* key, double value) {
     S->chunk = (int)value;
     return MST_OK;
   }
-  if (!strcmp(key, "slack") && value >= 1 && value <= 100) {
+  if (!strcmp(key, "slack") && value > 0 && value <= 100) {
     S->slack = value;
     return MST_OK;
   }
@@ -1373,6 +1374,10 @@ extern "C" int mst_server_set(mst_server* S, const char* key, double value) {
   }
   if (!strcmp(key, "score_share") && value > 0 && value <= 1) {
     S->score_share = value;
+    return MST_OK;
+  }
+  if (!strcmp(key, "deadline_add") && value >= 0 && value <= 100) {
+    S->deadline_add = value;
     return MST_OK;
   }
   if (!strcmp(key, "rounds") && value >= 1 && value <= mst_server::kBufs) {
