@@ -296,6 +296,7 @@ def main():
     ap.add_argument("--serve-only", action="store_true", help="only the served-form measurement")
     ap.add_argument("--no-traceback", action="store_true", help="skip the alignment-form measurement")
     ap.add_argument("--no-long", action="store_true", help="skip the configuration 3 measurement")
+    ap.add_argument("--long-only", action="store_true", help="only the configuration 3 measurement")
     ap.add_argument("--profile-set", action="store_true",
                     help="one pass of each profiled kernel (plain copy of 1 GiB, batched all pairs, "
                          "alignment form) for counter runs")
@@ -316,6 +317,10 @@ def main():
     self_cells = int((L * L).sum())
 
     ctx = M.Context(device=0)
+    if args.long_only:
+        print(json.dumps({"long_pairs": long_pairs_bench(M, ctx)}), flush=True)
+        ctx.close()
+        return 0
     if args.serve_only:
         served = served_bench(
             M, ctx, args.serve_jobs, load=args.serve_load,

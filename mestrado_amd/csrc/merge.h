@@ -28,7 +28,7 @@ struct MergeDesc {
   uint8_t* out;
   int kX, nX, kY, nY;
   int job;                 // index of its flag job, -1 when nX or nY is 0
-  int pad;
+  int tchunk;              // steps per wave of the score table kernel
   int64_t cx, cy, p, fx, tab, ops, sxy;
   int64_t row0;            // first output row of the merge over the batch
 };

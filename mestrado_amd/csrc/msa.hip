@@ -96,12 +96,13 @@ __device__ __forceinline__ void p_row(const int32_t* __restrict__ countsY, int k
 // lane (DP row = strip*512 + lane*8 + i) at DP column c = t - lane. DP row r is
 // forward Y column m-1-r and DP column c is forward X column n-1-c.
 //
-// A wave takes one strip and a chunk of kTabChunk steps: each lane keeps the
-// P rows of 4 of its 8 DP rows in registers for the whole chunk (two passes)
-// and reads one X frequency row (24 floats, fX = count / kX, written by the
-// counts kernel) per step, so P is read once per chunk instead of once per
-// entry.
-constexpr int kTabChunk = 64;
+// A wave takes one strip and a chunk of steps: each lane keeps the P rows of
+// 4 of its 8 DP rows in registers for the whole chunk (two passes) and reads
+// one X frequency row (24 floats, fX = count / kX, written by the counts
+// kernel) per step, so P is read once per chunk instead of once per entry.
+// Chunks are 64 steps (more waves per merge, for the latency of the merges
+// of a served level) or 256 for the longest merges (P reloaded less often).
+constexpr int kTabChunk = 64, kTabChunkLong = 256;
 
 __global__ void mb_counts_kernel(const MergeDesc* __restrict__ d, const Prefix* __restrict__ pre,
                                  int count, int64_t total, uint8_t* __restrict__ scratch) {
@@ -131,6 +132,11 @@ __global__ void mb_p_kernel(const MergeDesc* __restrict__ d, const Prefix* __res
   p_row((const int32_t*)(scratch + m.cy), m.kY, S, (int)(x - pre[i].yrow0), (float*)(scratch + m.p));
 }
 
+// RP rows per pass: 4 (two passes, ~145 VGPRs: the merges of served levels
+// share compute units with other kernels) or 8 (one pass, ~250 VGPRs, half the
+// X frequency reads: the longest merges, tchunk == kTabChunkLong). Each
+// instance handles the merges of its kind and skips the others.
+template <int RP>
 __global__ void __launch_bounds__(256) mb_table_kernel(const MergeDesc* __restrict__ d,
                                                        const Prefix* __restrict__ pre, int count,
                                                        int64_t total, int go_plus_ge,
@@ -140,32 +146,33 @@ __global__ void __launch_bounds__(256) mb_table_kernel(const MergeDesc* __restri
   if (w >= total) return;
   const int i = find_merge(count, w, [&](int k) { return pre[k].item0; });
   const MergeDesc m = d[i];
+  if ((m.tchunk == kTabChunkLong) != (RP == 8)) return;
   const int64_t unit = w - pre[i].item0;
   const int n = m.nX, rows = m.nY;
   const int Tn = n + MST_WAVE - 1;
-  const int nch = (Tn + kTabChunk - 1) / kTabChunk;
+  const int ch = m.tchunk;
+  const int nch = (Tn + ch - 1) / ch;
   const int s = (int)(unit / nch);
-  const int t0 = (int)(unit - (int64_t)s * nch) * kTabChunk;
-  const int t1 = t0 + kTabChunk < Tn ? t0 + kTabChunk : Tn;
+  const int t0 = (int)(unit - (int64_t)s * nch) * ch;
+  const int t1 = t0 + ch < Tn ? t0 + ch : Tn;
   const float* P = (const float*)(scratch + m.p);
   const float* FX = (const float*)(scratch + m.fx);
-  uint2* tab = (uint2*)(scratch + m.tab) + (int64_t)s * Tn * MST_WAVE;
-  // two passes of 4 rows (the low and the high dword of each entry), so a
-  // lane holds 96 floats of P: ~140 VGPRs, room for other kernels' waves
+  uint32_t* tab32 = (uint32_t*)((uint2*)(scratch + m.tab) + (int64_t)s * Tn * MST_WAVE);
 #pragma unroll 1
-  for (int half = 0; half < 2; ++half) {
-    float pr[4][24];
+  for (int part = 0; part < 8 / RP; ++part) {
+    float pr[RP][24];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int r = s * kStripRows + lane * 8 + half * 4 + k;
+    for (int k = 0; k < RP; ++k) {
+      const int r = s * kStripRows + lane * 8 + part * RP + k;
       const float* p = P + (int64_t)(rows - 1 - (r < rows ? r : rows - 1)) * 24;
 #pragma unroll
       for (int a = 0; a < 24; ++a) pr[k][a] = p[a];
     }
-    uint32_t* tab32 = (uint32_t*)tab + half;
     for (int t = t0; t < t1; ++t) {
       const int c = t - lane;
-      uint32_t wd = 0;
+      uint32_t wd[RP / 4];
+#pragma unroll
+      for (int q = 0; q < RP / 4; ++q) wd[q] = 0;
       if (c >= 0 && c < n) {
         const float4* f4 = (const float4*)(FX + (int64_t)(n - 1 - c) * 24);
         float fx[24];
@@ -178,8 +185,8 @@ __global__ void __launch_bounds__(256) mb_table_kernel(const MergeDesc* __restri
           fx[4 * q + 3] = v.w;
         }
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const int r = s * kStripRows + lane * 8 + half * 4 + k;
+        for (int k = 0; k < RP; ++k) {
+          const int r = s * kStripRows + lane * 8 + part * RP + k;
           float acc = 0.0f;
 #pragma unroll
           for (int a = 0; a < 24; ++a) {
@@ -187,10 +194,15 @@ __global__ void __launch_bounds__(256) mb_table_kernel(const MergeDesc* __restri
             acc = acc + q;
           }
           const int v = r < rows ? (int)__builtin_rintf(acc) + go_plus_ge : 0;
-          wd |= ((uint32_t)v & 0xFFu) << (8 * k);
+          wd[k >> 2] |= ((uint32_t)v & 0xFFu) << (8 * (k & 3));
         }
       }
-      tab32[((int64_t)t * MST_WAVE + lane) * 2] = wd;
+      uint32_t* e = tab32 + ((int64_t)t * MST_WAVE + lane) * 2;
+      if (RP == 8) {
+        *(uint2*)e = make_uint2(wd[0], wd[RP / 4 - 1]);
+      } else {
+        e[part] = wd[0];
+      }
     }
   }
 }
@@ -364,7 +376,7 @@ hipError_t MergeEngine::enqueue(const MergeSpec* m, int count, const int8_t* d_S
     q.nX = m[i].nX;
     q.kY = m[i].kY;
     q.nY = m[i].nY;
-    q.pad = 0;
+    q.tchunk = tiled_job(q.nX, q.nY) ? kTabChunkLong : kTabChunk;
     q.row0 = row0;
     row0 += q.kX + q.kY;
     hp[i].col0 = col0;
@@ -394,7 +406,7 @@ hipError_t MergeEngine::enqueue(const MergeSpec* m, int count, const int8_t* d_S
     off += r256((int64_t)q.nX + q.nY);
     col0 += (int64_t)q.nX + q.nY;
     yrow0 += q.nY;
-    item0 += (int64_t)strips_of(q.nY) * ((q.nX + MST_WAVE - 1 + kTabChunk - 1) / kTabChunk);
+    item0 += (int64_t)strips_of(q.nY) * ((q.nX + MST_WAVE - 1 + q.tchunk - 1) / q.tchunk);
     q.job = (int)next[cls[i]]++;
     AlignJob& j = h_jobs_[q.job];
     j.n = q.nX;
@@ -436,8 +448,11 @@ hipError_t MergeEngine::enqueue(const MergeSpec* m, int count, const int8_t* d_S
                        count, col0, d_scratch_);
     hipLaunchKernelGGL(mb_p_kernel, dim3((unsigned)((yrow0 + 255) / 256)), dim3(256), 0, st, dd, dp, count,
                        yrow0, d_S, d_scratch_);
-    hipLaunchKernelGGL(mb_table_kernel, dim3((unsigned)((item0 + 3) / 4)), dim3(256), 0, st, dd, dp, count,
-                       item0, go + ge, d_scratch_);
+    hipLaunchKernelGGL(mb_table_kernel<4>, dim3((unsigned)((item0 + 3) / 4)), dim3(256), 0, st, dd, dp,
+                       count, item0, go + ge, d_scratch_);
+    if (nb[3] > 0)
+      hipLaunchKernelGGL(mb_table_kernel<8>, dim3((unsigned)((item0 + 3) / 4)), dim3(256), 0, st, dd, dp,
+                         count, item0, go + ge, d_scratch_);
     AlignArgs a;
     a.jobs = d_jobs_;
     a.seqs = nullptr;
