@@ -606,7 +606,8 @@ __global__ void __launch_bounds__(MST_WAVE* W) align_pipe_kernel(AlignArgs a) {
 template <int SRC>
 __global__ void __launch_bounds__(MST_WAVE) align_kernel(AlignArgs a) {
   __shared__ int8_t S_lds[MST_LETTERS * MST_LETTERS];
-  __shared__ __attribute__((aligned(16))) uint8_t prof[MST_LETTERS * MST_WAVE * AK];
+  // the profile only for scores from sequences; a score table needs none
+  __shared__ __attribute__((aligned(16))) uint8_t prof[SRC == SRC_SEQ ? MST_LETTERS * MST_WAVE * AK : 16];
   const int lane = lane_id();
   if (SRC == SRC_SEQ) {
     for (int i = lane; i < MST_LETTERS * MST_LETTERS; i += MST_WAVE) S_lds[i] = a.subst[i];
