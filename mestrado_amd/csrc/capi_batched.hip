@@ -182,13 +182,15 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
         }
       }
       if (kind == 3 || kind == 5 || kind == 6) {
+        // pipelines of three or four waves take their own item size
+        const int Cw = (kind == 5 && W >= 3 && ctx->tune.chunk_w3 > 0) ? ctx->tune.chunk_w3 : C;
         size_t c0 = 0;
         while (c0 < elig.size()) {
           // a chunk is a run of consecutive positions; score6 keeps its column
           // stream within the per-workgroup row buffer
           size_t c1 = c0 + 1;
           int64_t cols = len(order[elig[c0]]);
-          while (c1 < elig.size() && (int)(c1 - c0) < C && elig[c1] == elig[c1 - 1] + 1 &&
+          while (c1 < elig.size() && (int)(c1 - c0) < Cw && elig[c1] == elig[c1 - 1] + 1 &&
                  (kind != 6 || cols + len(order[elig[c1]]) <= kScore6MaxCols)) {
             cols += len(order[elig[c1]]);
             ++c1;
@@ -566,6 +568,7 @@ static int* tune_slot(mst_ctx* ctx, const char* key) {
   if (!strcmp(key, "shared3")) return &ctx->tune.shared3;
   if (!strcmp(key, "wg5")) return &ctx->tune.wg5;
   if (!strcmp(key, "k5min")) return &ctx->tune.k5min;
+  if (!strcmp(key, "chunk_w3")) return &ctx->tune.chunk_w3;
   if (!strcmp(key, "wg2")) return &ctx->tune.wg_per_cu2;
   if (!strncmp(key, "wg3.", 4)) {
     const int k = atoi(key + 4);
@@ -584,7 +587,9 @@ extern "C" int mst_tune_set(mst_ctx* ctx, const char* key, int value) {
     return ctx->fail(MST_E_ARG, "mst_tune_set", "shared3 is 0 or 1");
   if (!strcmp(key, "k5min") && (value < 1 || value > 3))
     return ctx->fail(MST_E_ARG, "mst_tune_set", "k5min out of range 1..3");
-  if (strcmp(key, "chunk") && strcmp(key, "shared3") && (value < 0 || value > 64))
+  if (!strcmp(key, "chunk_w3") && (value < 0 || value > 4096))
+    return ctx->fail(MST_E_ARG, "mst_tune_set", "chunk_w3 out of range 0..4096");
+  if (strcmp(key, "chunk") && strcmp(key, "shared3") && strcmp(key, "chunk_w3") && (value < 0 || value > 64))
     return ctx->fail(MST_E_ARG, "mst_tune_set", "workgroups per CU out of range 0..64");
   if (*slot == value) return MST_OK;
   hipSetDevice(ctx->device);
