@@ -92,7 +92,9 @@ int mst_last_launch_stats(mst_ctx* ctx, int* rows_per_lane, int64_t* pairs, int6
  * "k5min" (least rows per lane of the one-wave streamed kernel, 1..3: short
  * row pairs of m rows take ceil(m/64) rows per lane when that is larger;
  * default 1), "chunk_w3" (column sequences per work item of the pipelines
- * of three and four waves, 0 = "chunk"; default 0), "wg5", "wg2" and
+ * of three and four waves, 0 = "chunk"; default 0), "k1max" (most rows per
+ * lane of the one-wave streamed kernel, 5..16: row pairs of up to 64 * k1max
+ * rows take one wave; default 13), "wg5", "wg2" and
  * "wg3.<K>" (workgroups per CU, 0 = default). Setting a value drops the cached
  * batched plans. Unknown keys or values out of range return MST_E_ARG. */
 int mst_tune_set(mst_ctx* ctx, const char* key, int value);

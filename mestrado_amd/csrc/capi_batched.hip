@@ -69,8 +69,9 @@ bool pk16_usable(const mst_ctx* ctx) {
 }
 
 // Classes of the planner, by rows m of the row pair (the longer of Ya, Yb):
-//  * m <= 1024: the pipelined packed kernel (score5), W = ceil(m/256) waves of
-//    64*K rows, K >= tune.k5min for one wave (rows 1..320: K = k5min..5);
+//  * m <= 1024: the pipelined packed kernel (score5): one wave of K =
+//    max(k5min, ceil(m/64)) rows per lane for m <= 64 * k1max (K up to 16),
+//    else W = ceil(m/256) waves of 64*K rows;
 //  * with tune.shared3 = 1, rows 129..320 take the shared-profile kernel
 //    (score3) instead, in its classes K = 3..5;
 //  * m > 1024: the multi-sweep packed kernel (score6), up to the 16-bit bound;
@@ -146,6 +147,9 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
       } else {
         kind = 0;
       }
+    } else if (nstr == 1 && kPipelined && m <= 64 * ctx->tune.k1max) {
+      K = (m + 63) / 64;  // one wave of up to k1max rows per lane
+      kind = 5;
     } else if (nstr == 1 && kPipelined) {
       W = (m + 255) / 256;
       K = (m + 64 * W - 1) / (64 * W);
@@ -569,6 +573,7 @@ static int* tune_slot(mst_ctx* ctx, const char* key) {
   if (!strcmp(key, "wg5")) return &ctx->tune.wg5;
   if (!strcmp(key, "k5min")) return &ctx->tune.k5min;
   if (!strcmp(key, "chunk_w3")) return &ctx->tune.chunk_w3;
+  if (!strcmp(key, "k1max")) return &ctx->tune.k1max;
   if (!strcmp(key, "wg2")) return &ctx->tune.wg_per_cu2;
   if (!strncmp(key, "wg3.", 4)) {
     const int k = atoi(key + 4);
@@ -589,6 +594,8 @@ extern "C" int mst_tune_set(mst_ctx* ctx, const char* key, int value) {
     return ctx->fail(MST_E_ARG, "mst_tune_set", "k5min out of range 1..3");
   if (!strcmp(key, "chunk_w3") && (value < 0 || value > 4096))
     return ctx->fail(MST_E_ARG, "mst_tune_set", "chunk_w3 out of range 0..4096");
+  if (!strcmp(key, "k1max") && (value < 5 || value > 16))
+    return ctx->fail(MST_E_ARG, "mst_tune_set", "k1max out of range 5..16");
   if (strcmp(key, "chunk") && strcmp(key, "shared3") && strcmp(key, "chunk_w3") && (value < 0 || value > 64))
     return ctx->fail(MST_E_ARG, "mst_tune_set", "workgroups per CU out of range 0..64");
   if (*slot == value) return MST_OK;

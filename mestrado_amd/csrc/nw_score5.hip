@@ -7,7 +7,7 @@
 // and columns are laid out (device parts in nw_stream.h):
 //
 //  * rows: a pair of row sequences with m rows is cut into W strips of 64*K
-//    rows (K = 1..5 rows per lane, so no lane holds many rows); wave j of a
+//    rows (K = 1..16 rows per lane for one wave, 3..4 for more); wave j of a
 //    pipeline owns strip j. Lane 63 of wave j writes F and H of its last row for
 //    every column into an LDS ring; lane 0 of wave j+1 reads them one column
 //    behind. Progress counters in LDS (columns produced, columns consumed)
@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(MST_WAVE * W * Pipes<W>::P) score5_kernel(Scor
 }
 
 bool score5_supported(int K, int W) {
-  if (W == 1) return K >= 1 && K <= 5;
+  if (W == 1) return K >= 1 && K <= 16;
   return W >= 2 && W <= 4 && K >= 3 && K <= 4;
 }
 
@@ -120,7 +120,7 @@ int score5_block_threads(int W) { return MST_WAVE * W * pipes(W); }
 // LDS of one workgroup (profile, rings, counters, matrix) -> workgroups per CU
 int score5_groups_per_cu(int K, int W) {
   const int P = pipes(W);
-  const int D = K <= 4 ? 2 : 4;
+  const int D = K <= 2 ? 1 : (K <= 4 ? 2 : (K <= 8 ? 4 : 8));
   const int nr = W > 1 ? P * (W - 1) : 1;
   const int lds = W * MST_LETTERS * MST_WAVE * D * 4 + nr * RING5 * 8 + 8 * W * P + 4 +
                   MST_LETTERS * MST_LETTERS;
@@ -142,6 +142,17 @@ void launch_score5(int K, int W, const Score5Args& a, int grid, hipStream_t st) 
   MST_CASE(3, 1)
   MST_CASE(4, 1)
   MST_CASE(5, 1)
+  MST_CASE(6, 1)
+  MST_CASE(7, 1)
+  MST_CASE(8, 1)
+  MST_CASE(9, 1)
+  MST_CASE(10, 1)
+  MST_CASE(11, 1)
+  MST_CASE(12, 1)
+  MST_CASE(13, 1)
+  MST_CASE(14, 1)
+  MST_CASE(15, 1)
+  MST_CASE(16, 1)
   MST_CASE(3, 2)
   MST_CASE(4, 2)
   MST_CASE(3, 3)

@@ -151,6 +151,29 @@ def test_batched_one_wave_small_k(ctx, k5min):
         ctx.tune_set("k5min", 0)
 
 
+@pytest.mark.parametrize("k1max,hi", [(8, 530), (16, 1030)])
+def test_batched_one_wave_large_k(ctx, k1max, hi):
+    # rows up to 64 * k1max in one wave of up to k1max rows per lane
+    rng = np.random.default_rng(40 + k1max)
+    lens = list(rng.integers(300, hi, 30)) + [320, 321, 384, 385, 448, 449, 512, 513, 100, 1100]
+    lens += [576, 577, 640, 641, 768, 769, 1023, 1024] if k1max == 16 else []
+    seqs = [_rand(rng, int(L)) for L in lens]
+    o, _ = O.all_pairs(seqs, S)
+    st = ctx.upload(seqs)
+    saved = ctx.tune_get("k1max")
+    try:
+        ctx.tune_set("k1max", k1max)
+        g = st.all_pairs(M.FORM_BATCHED)
+        assert (g == o).all()
+        classes = {k for k, _, _, _ in st.launch_stats(M.FORM_BATCHED)}
+        assert {-1106, -1107, -1108} <= classes
+        if k1max == 16:
+            assert {-1109, -1112, -1116} <= classes
+    finally:
+        ctx.tune_set("k1max", saved)
+        st.free()
+
+
 @pytest.mark.parametrize("seed,n,lo,hi", [(21, 9, 1025, 2100), (22, 14, 800, 1600)])
 def test_batched_multi_sweep_rows(ctx, seed, n, lo, hi):
     # rows beyond 1024: four waves sweep the column stream once per four
