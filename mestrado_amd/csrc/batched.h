@@ -26,6 +26,7 @@ struct Tune {
   int k5min = 1;             // least rows per lane of the one-wave streamed kernel (1..3)
   int chunk_w3 = 0;          // column sequences per item of 3- and 4-wave pipelines (0: chunk)
   int k1max = 13;            // most rows per lane of the one-wave streamed kernel (5..16)
+  int k2max = 8;             // most rows per lane of the two-wave streamed kernel (4..8)
 };
 
 struct BatchedPlan {

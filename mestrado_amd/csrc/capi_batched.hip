@@ -150,6 +150,10 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
     } else if (nstr == 1 && kPipelined && m <= 64 * ctx->tune.k1max) {
       K = (m + 63) / 64;  // one wave of up to k1max rows per lane
       kind = 5;
+    } else if (nstr == 1 && kPipelined && (m + 127) / 128 <= ctx->tune.k2max) {
+      W = 2;  // two waves of up to k2max rows per lane
+      K = (m + 127) / 128;
+      kind = 5;
     } else if (nstr == 1 && kPipelined) {
       W = (m + 255) / 256;
       K = (m + 64 * W - 1) / (64 * W);
@@ -574,6 +578,7 @@ static int* tune_slot(mst_ctx* ctx, const char* key) {
   if (!strcmp(key, "k5min")) return &ctx->tune.k5min;
   if (!strcmp(key, "chunk_w3")) return &ctx->tune.chunk_w3;
   if (!strcmp(key, "k1max")) return &ctx->tune.k1max;
+  if (!strcmp(key, "k2max")) return &ctx->tune.k2max;
   if (!strcmp(key, "wg2")) return &ctx->tune.wg_per_cu2;
   if (!strncmp(key, "wg3.", 4)) {
     const int k = atoi(key + 4);
@@ -594,6 +599,8 @@ extern "C" int mst_tune_set(mst_ctx* ctx, const char* key, int value) {
     return ctx->fail(MST_E_ARG, "mst_tune_set", "k5min out of range 1..3");
   if (!strcmp(key, "chunk_w3") && (value < 0 || value > 4096))
     return ctx->fail(MST_E_ARG, "mst_tune_set", "chunk_w3 out of range 0..4096");
+  if (!strcmp(key, "k2max") && (value < 4 || value > 8))
+    return ctx->fail(MST_E_ARG, "mst_tune_set", "k2max out of range 4..8");
   if (!strcmp(key, "k1max") && (value < 5 || value > 16))
     return ctx->fail(MST_E_ARG, "mst_tune_set", "k1max out of range 5..16");
   if (strcmp(key, "chunk") && strcmp(key, "shared3") && strcmp(key, "chunk_w3") && (value < 0 || value > 64))

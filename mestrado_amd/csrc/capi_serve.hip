@@ -79,9 +79,28 @@ double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// Classes of the streamed kernels: 0 = score6, 1 + (W-1)*3 + (K-3) = score5<K, W>.
-constexpr int kClasses = 13;
-int class_of(int kind, int K, int W) { return kind == 6 ? 0 : 1 + (W - 1) * 3 + (K - 3); }
+// Classes of the streamed kernels: 0 = score6; then score5<K, 1> for K =
+// 1..16, score5<K, 2> for K = 3..8, score5<K, 3> and score5<K, 4> for K = 3..4
+// (the routes of the batched form's planner).
+constexpr int kClasses = 1 + 16 + 6 + 2 + 2;
+int class_of(int kind, int K, int W) {
+  if (kind == 6) return 0;
+  if (W == 1) return K;
+  if (W == 2) return 17 + (K - 3);
+  return 23 + (W - 3) * 2 + (K - 3);
+}
+void class_shape(int c, int* K, int* W) {
+  if (c <= 16) {
+    *W = 1;
+    *K = c;
+  } else if (c <= 22) {
+    *W = 2;
+    *K = c - 17 + 3;
+  } else {
+    *W = 3 + (c - 23) / 2;
+    *K = 3 + (c - 23) % 2;
+  }
+}
 
 // Stages of a running job.
 enum { kStageScores = 0, kStageNj = 1, kStageMsa = 2, kStageDone = 3 };
@@ -327,8 +346,12 @@ void plan_job(const mst_ctx* ctx, Job& J, const std::vector<int>& lens, int chun
     if (m == 0) {
       kind = 0;
       K = 3;
-    } else if (m <= 320) {  // one wave of K = 3..5 rows per lane
-      K = std::max(3, (m + 63) / 64);
+    } else if (m <= 64 * ctx->tune.k1max) {  // one wave (the batched form's routes)
+      K = std::max(ctx->tune.k5min, (m + 63) / 64);
+      kind = 5;
+    } else if ((m + 127) / 128 <= ctx->tune.k2max) {  // two waves
+      W = 2;
+      K = (m + 127) / 128;
       kind = 5;
     } else if (m <= 1024) {
       W = (m + 255) / 256;
@@ -609,7 +632,8 @@ int launch_round(mst_server* S, int* launched) {
     if (c == 0) {
       launch_score6(a, (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)(S->grid6 * share), nit)), st);
     } else {
-      const int W = (c - 1) / 3 + 1, K = (c - 1) % 3 + 3;
+      int W, K;
+      class_shape(c, &K, &W);
       // persistent: as many workgroups as fit, each taking items until none
       // are left; else a workgroup per item, so workgroups retire throughout
       // the launch and other stages' kernels find free slots

@@ -110,7 +110,8 @@ __global__ void __launch_bounds__(MST_WAVE * W * Pipes<W>::P) score5_kernel(Scor
 
 bool score5_supported(int K, int W) {
   if (W == 1) return K >= 1 && K <= 16;
-  return W >= 2 && W <= 4 && K >= 3 && K <= 4;
+  if (W == 2) return K >= 3 && K <= 8;
+  return W >= 3 && W <= 4 && K >= 3 && K <= 4;
 }
 
 static int pipes(int W) { return W == 1 ? 4 : (W == 2 ? 2 : 1); }
@@ -155,6 +156,10 @@ void launch_score5(int K, int W, const Score5Args& a, int grid, hipStream_t st) 
   MST_CASE(16, 1)
   MST_CASE(3, 2)
   MST_CASE(4, 2)
+  MST_CASE(5, 2)
+  MST_CASE(6, 2)
+  MST_CASE(7, 2)
+  MST_CASE(8, 2)
   MST_CASE(3, 3)
   MST_CASE(4, 3)
   MST_CASE(3, 4)

@@ -174,6 +174,25 @@ def test_batched_one_wave_large_k(ctx, k1max, hi):
         st.free()
 
 
+def test_batched_two_wave_large_k(ctx):
+    # rows beyond the one-wave classes in two waves of up to 8 rows per lane
+    rng = np.random.default_rng(50)
+    lens = list(rng.integers(800, 1030, 24)) + [832, 833, 896, 897, 1023, 1024, 300]
+    seqs = [_rand(rng, int(L)) for L in lens]
+    o, _ = O.all_pairs(seqs, S)
+    st = ctx.upload(seqs)
+    saved = ctx.tune_get("k2max")
+    try:
+        ctx.tune_set("k2max", 8)
+        g = st.all_pairs(M.FORM_BATCHED)
+        assert (g == o).all()
+        classes = {k for k, _, _, _ in st.launch_stats(M.FORM_BATCHED)}
+        assert {-1207, -1208} <= classes
+    finally:
+        ctx.tune_set("k2max", saved)
+        st.free()
+
+
 @pytest.mark.parametrize("seed,n,lo,hi", [(21, 9, 1025, 2100), (22, 14, 800, 1600)])
 def test_batched_multi_sweep_rows(ctx, seed, n, lo, hi):
     # rows beyond 1024: four waves sweep the column stream once per four
