@@ -96,7 +96,9 @@ int mst_last_launch_stats(mst_ctx* ctx, int* rows_per_lane, int64_t* pairs, int6
  * lane of the one-wave streamed kernel, 5..16: row pairs of up to 64 * k1max
  * rows take one wave; default 13), "k2max" (most rows per lane of the
  * two-wave pipelines, 4..8: longer row pairs of up to 128 * k2max rows take
- * two waves; default 8), "wg5", "wg2" and
+ * two waves; default 8), "branches" (parallel branches of the captured
+ * graph's class launches, 1..4, so one class's tail overlaps the next
+ * class's start; default 2), "wg5", "wg2" and
  * "wg3.<K>" (workgroups per CU, 0 = default). Setting a value drops the cached
  * batched plans. Unknown keys or values out of range return MST_E_ARG. */
 int mst_tune_set(mst_ctx* ctx, const char* key, int value);

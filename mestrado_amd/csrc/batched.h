@@ -27,6 +27,7 @@ struct Tune {
   int chunk_w3 = 0;          // column sequences per item of 3- and 4-wave pipelines (0: chunk)
   int k1max = 13;            // most rows per lane of the one-wave streamed kernel (5..16)
   int k2max = 8;             // most rows per lane of the two-wave streamed kernel (4..8)
+  int branches = 2;          // parallel branches of the graph's class launches (1..4)
 };
 
 struct BatchedPlan {

@@ -134,6 +134,10 @@ mst_ctx::~mst_ctx() {
   for (hipEvent_t e : launch_events) hipEventDestroy(e);
   if (ev0) hipEventDestroy(ev0);
   if (ev1) hipEventDestroy(ev1);
+  for (hipStream_t st : bstream)
+    if (st) hipStreamDestroy(st);
+  for (hipEvent_t e : bevent)
+    if (e) hipEventDestroy(e);
   if (stream) hipStreamDestroy(stream);
 }
 

@@ -433,7 +433,7 @@ int enqueue_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P, int32
     a.job_ld = nullptr;
     a.first = c.begin;
     a.last = c.end;
-    a.counter = P.d_counters + ctr++;
+    a.counter = nullptr;  // set by the caller
     a.out_mat = d_out;
     a.nseq = s.n;
     a.subst = ctx->d_S;
@@ -443,21 +443,66 @@ int enqueue_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P, int32
     a.carry_cap = std::max<int64_t>(P.carry_cols, 1);
     return a;
   };
-  for (const BClass& c : P.c6) {
-    const Score5Args a = args5(c, P.d_items6);
-    if (timed) CK(hipEventRecord(ctx->launch_events[ev++], ctx->stream));
-    launch_score6(a, P.grid6, ctx->stream);
-    CK(hipGetLastError());
-    if (timed) CK(hipEventRecord(ctx->launch_events[ev++], ctx->stream));
+  // Untimed, the streamed classes may run as parallel branches (fork from
+  // ctx->stream, join back into it) so that one class's tail, when few of
+  // its workgroups are left, overlaps the next class's start. Classes go to
+  // the branch with the fewest cells so far, largest first.
+  const int nb = timed ? 1 : std::max(1, std::min(4, ctx->tune.branches));
+  std::vector<hipStream_t> br(1, ctx->stream);
+  if (nb > 1) {
+    for (int b = 0; b < nb; ++b) {
+      if (!ctx->bstream[b]) CK(hipStreamCreateWithFlags(&ctx->bstream[b], hipStreamNonBlocking));
+      if (!ctx->bevent[b]) CK(hipEventCreateWithFlags(&ctx->bevent[b], hipEventDisableTiming));
+    }
+    if (!ctx->bevent[4]) CK(hipEventCreateWithFlags(&ctx->bevent[4], hipEventDisableTiming));
+    CK(hipEventRecord(ctx->bevent[4], ctx->stream));
+    br.clear();
+    for (int b = 0; b < nb; ++b) {
+      CK(hipStreamWaitEvent(ctx->bstream[b], ctx->bevent[4], 0));
+      br.push_back(ctx->bstream[b]);
+    }
   }
-  for (const BClass& c : P.c5) {
-    const Score5Args a = args5(c, P.d_items5);
-    const int per = ctx->tune.wg5 > 0 ? ctx->tune.wg5 : score5_groups_per_cu(c.K, c.W);
-    const int64_t g = std::min<int64_t>((int64_t)ctx->num_cus * per, c.end - c.begin);
-    if (timed) CK(hipEventRecord(ctx->launch_events[ev++], ctx->stream));
-    launch_score5(c.K, c.W, a, (int)std::max<int64_t>(g, 1), ctx->stream);
+  struct Launch {
+    bool six;
+    const BClass* c;
+    int ctr;
+  };
+  std::vector<Launch> ls;
+  for (const BClass& c : P.c6) ls.push_back(Launch{true, &c, ctr++});
+  for (const BClass& c : P.c5) ls.push_back(Launch{false, &c, ctr++});
+  std::vector<int> where(ls.size(), 0);
+  if (nb > 1) {
+    std::vector<size_t> ord(ls.size());
+    for (size_t k = 0; k < ls.size(); ++k) ord[k] = k;
+    std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) { return ls[x].c->cells > ls[y].c->cells; });
+    std::vector<int64_t> load(nb, 0);
+    for (size_t k : ord) {
+      const int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      where[k] = b;
+      load[b] += ls[k].c->cells;
+    }
+  }
+  for (size_t k = 0; k < ls.size(); ++k) {
+    const BClass& c = *ls[k].c;
+    hipStream_t st = br[where[k]];
+    Score5Args a = args5(c, ls[k].six ? P.d_items6 : P.d_items5);
+    a.counter = P.d_counters + ls[k].ctr;
+    if (timed) CK(hipEventRecord(ctx->launch_events[ev++], st));
+    if (ls[k].six) {
+      launch_score6(a, P.grid6, st);
+    } else {
+      const int per = ctx->tune.wg5 > 0 ? ctx->tune.wg5 : score5_groups_per_cu(c.K, c.W);
+      const int64_t g = std::min<int64_t>((int64_t)ctx->num_cus * per, c.end - c.begin);
+      launch_score5(c.K, c.W, a, (int)std::max<int64_t>(g, 1), st);
+    }
     CK(hipGetLastError());
-    if (timed) CK(hipEventRecord(ctx->launch_events[ev++], ctx->stream));
+    if (timed) CK(hipEventRecord(ctx->launch_events[ev++], st));
+  }
+  if (nb > 1) {
+    for (int b = 0; b < nb; ++b) {
+      CK(hipEventRecord(ctx->bevent[b], ctx->bstream[b]));
+      CK(hipStreamWaitEvent(ctx->stream, ctx->bevent[b], 0));
+    }
   }
   if (!P.c2.empty()) {
     int64_t bufcap = std::max(s.maxlen, 1);
@@ -579,6 +624,7 @@ static int* tune_slot(mst_ctx* ctx, const char* key) {
   if (!strcmp(key, "chunk_w3")) return &ctx->tune.chunk_w3;
   if (!strcmp(key, "k1max")) return &ctx->tune.k1max;
   if (!strcmp(key, "k2max")) return &ctx->tune.k2max;
+  if (!strcmp(key, "branches")) return &ctx->tune.branches;
   if (!strcmp(key, "wg2")) return &ctx->tune.wg_per_cu2;
   if (!strncmp(key, "wg3.", 4)) {
     const int k = atoi(key + 4);
@@ -599,6 +645,8 @@ extern "C" int mst_tune_set(mst_ctx* ctx, const char* key, int value) {
     return ctx->fail(MST_E_ARG, "mst_tune_set", "k5min out of range 1..3");
   if (!strcmp(key, "chunk_w3") && (value < 0 || value > 4096))
     return ctx->fail(MST_E_ARG, "mst_tune_set", "chunk_w3 out of range 0..4096");
+  if (!strcmp(key, "branches") && (value < 1 || value > 4))
+    return ctx->fail(MST_E_ARG, "mst_tune_set", "branches out of range 1..4");
   if (!strcmp(key, "k2max") && (value < 4 || value > 8))
     return ctx->fail(MST_E_ARG, "mst_tune_set", "k2max out of range 4..8");
   if (!strcmp(key, "k1max") && (value < 5 || value > 16))

@@ -51,6 +51,9 @@ struct mst_ctx {
   char arch[64] = {0};  // gcnArchName of the device, e.g. gfx950:sramecc+:xnack-
   int64_t hbm_bytes = 0;
   hipStream_t stream = nullptr;
+  // branches of the batched form's graph (created on first use, default priority)
+  hipStream_t bstream[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t bevent[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int go = 10, ge = 2;
   int8_t S[576];

@@ -103,22 +103,27 @@ def test_batched_pipelined_rows(ctx, seed, n, lo, hi):
     st.free()
 
 
-@pytest.mark.parametrize("shared3,chunk,chunk_w3", [(1, 32, 0), (1, 1, 0), (0, 5, 0), (0, 32, 3)])
-def test_batched_tunables_keep_results(ctx, shared3, chunk, chunk_w3):
+@pytest.mark.parametrize("shared3,chunk,chunk_w3,branches",
+                         [(1, 32, 0, 1), (1, 1, 0, 1), (0, 5, 0, 2), (0, 32, 3, 3)])
+def test_batched_tunables_keep_results(ctx, shared3, chunk, chunk_w3, branches):
     # every routing / item size of the batched planner gives the same matrix
     rng = np.random.default_rng(11)
     lens = list(rng.integers(1, 900, 45)) + [128, 129, 320, 321, 1024]
     seqs = [_rand(rng, int(L)) for L in lens]
     o, _ = O.all_pairs(seqs, S)
     st = ctx.upload(seqs)
-    saved = {k: ctx.tune_get(k) for k in ("shared3", "chunk", "chunk_w3")}
+    saved = {k: ctx.tune_get(k) for k in ("shared3", "chunk", "chunk_w3", "branches")}
     try:
         ctx.tune_set("shared3", shared3)
         ctx.tune_set("chunk", chunk)
         ctx.tune_set("chunk_w3", chunk_w3)
+        ctx.tune_set("branches", branches)
         assert ctx.tune_get("chunk") == chunk
         g = st.all_pairs(M.FORM_BATCHED)
         assert (g == o).all()
+        st.all_pairs_resident(M.FORM_BATCHED)  # the captured graph, replayed twice
+        st.all_pairs_resident(M.FORM_BATCHED)
+        assert (st.all_pairs(M.FORM_BATCHED) == o).all()
     finally:
         for k, v in saved.items():
             ctx.tune_set(k, v)
