@@ -303,7 +303,8 @@ int score_grid(mst_ctx* ctx, int64_t npairs) {
 }
 
 int run_plan_on(mst_ctx* ctx, const SeqSet& s, const PairPlan& plan, const int2* d_pairs, int form,
-                int32_t* d_out_pair, int32_t* d_out_mat, bool timed, int event_base) {
+                int32_t* d_out_pair, int32_t* d_out_mat, bool timed, int event_base, hipStream_t st) {
+  if (!st) st = ctx->stream;
   int grid_max = score_grid(ctx, (int64_t)plan.pairs.size());
   int64_t bufcap = std::max<int64_t>(s.maxlen, 1);
   int rc = ctx->ensure((void**)&ctx->d_bufs, &ctx->bufs_cap,
@@ -327,18 +328,18 @@ int run_plan_on(mst_ctx* ctx, const SeqSet& s, const PairPlan& plan, const int2*
     a.ge = ctx->ge;
     a.transpose_ok = ctx->symmetric ? 1 : 0;
     int grid = score_grid(ctx, b.end - b.begin);
-    if (timed) CK(hipEventRecord(ctx->launch_events[event_base + 2 * bi], ctx->stream));
-    launch_score_pairs(b.K, a, grid, ctx->stream);
+    if (timed) CK(hipEventRecord(ctx->launch_events[event_base + 2 * bi], st));
+    launch_score_pairs(b.K, a, grid, st);
     CK(hipGetLastError());
-    if (timed) CK(hipEventRecord(ctx->launch_events[event_base + 2 * bi + 1], ctx->stream));
-    if (form == MST_FORM_PLAIN) CK(hipStreamSynchronize(ctx->stream));
+    if (timed) CK(hipEventRecord(ctx->launch_events[event_base + 2 * bi + 1], st));
+    if (form == MST_FORM_PLAIN) CK(hipStreamSynchronize(st));
   }
   return MST_OK;
 }
 
 int run_plan(mst_ctx* ctx, const SeqSet& s, const PairPlan& plan, int form, int32_t* d_out_pair,
              int32_t* d_out_mat, bool timed = false) {
-  return run_plan_on(ctx, s, plan, ctx->d_pairs, form, d_out_pair, d_out_mat, timed, 0);
+  return run_plan_on(ctx, s, plan, ctx->d_pairs, form, d_out_pair, d_out_mat, timed, 0, ctx->stream);
 }
 
 bool pk16_usable(const mst_ctx* ctx);

@@ -320,7 +320,8 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
 int build_plan(mst_ctx* ctx, const SeqSet& s, const int32_t* pi, const int32_t* pj, int64_t np,
                PairPlan& plan);
 int run_plan_on(mst_ctx* ctx, const SeqSet& s, const PairPlan& plan, const int2* d_pairs, int form,
-                int32_t* d_out_pair, int32_t* d_out_mat, bool timed, int event_base);
+                int32_t* d_out_pair, int32_t* d_out_mat, bool timed, int event_base,
+                hipStream_t st = nullptr);
 
 static int grid3(mst_ctx* ctx, int K, int64_t items) {
   int per = ctx->tune.wg_per_cu3[K < 17 ? K : 16];
@@ -386,12 +387,12 @@ int upload_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P) {
 }
 
 // The int32 launches of the plan's remaining pairs (never captured in the
-// graph: they follow its replay on the same stream).
+// graph), on st (ctx->stream when null).
 static int enqueue_fallback(mst_ctx* ctx, const SeqSet& s, int id, int32_t* d_out, bool timed,
-                            int event_base) {
+                            int event_base, hipStream_t st = nullptr) {
   const PairPlan& fb = ctx->fbplans[id];
   if (fb.pairs.empty()) return MST_OK;
-  return run_plan_on(ctx, s, fb, fb.d_pairs, MST_FORM_BATCHED, nullptr, d_out, timed, event_base);
+  return run_plan_on(ctx, s, fb, fb.d_pairs, MST_FORM_BATCHED, nullptr, d_out, timed, event_base, st);
 }
 
 // Enqueues the packed launches of the batched plan on ctx->stream, then the
@@ -600,9 +601,25 @@ int run_batched(mst_ctx* ctx, int id, int32_t* d_out, bool timed) {
     CK(hipGraphDestroy(g));
     P.graph_out = d_out;
   }
+  // The int32 pairs run beside the replay on a stream of their own (forked
+  // from ctx->stream and joined back): they write other entries of d_out, and
+  // the graph uses ctx->d_bufs only through the score2 classes.
+  const bool side = P.c2.empty() && !ctx->fbplans[id].pairs.empty();
+  if (!side) {
+    CK(hipGraphLaunch(P.graph, ctx->stream));
+    return enqueue_fallback(ctx, s, id, d_out, false, 0);
+  }
+  if (!ctx->bstream[3]) CK(hipStreamCreateWithFlags(&ctx->bstream[3], hipStreamNonBlocking));
+  for (int b : {3, 4})
+    if (!ctx->bevent[b]) CK(hipEventCreateWithFlags(&ctx->bevent[b], hipEventDisableTiming));
+  CK(hipEventRecord(ctx->bevent[4], ctx->stream));
+  CK(hipStreamWaitEvent(ctx->bstream[3], ctx->bevent[4], 0));
+  rc = enqueue_fallback(ctx, s, id, d_out, false, 0, ctx->bstream[3]);
+  if (rc) return rc;
   CK(hipGraphLaunch(P.graph, ctx->stream));
-  const int nev = 2 * (int)(P.c3.size() + P.c6.size() + P.c5.size() + P.c2.size());
-  return enqueue_fallback(ctx, s, id, d_out, false, nev);
+  CK(hipEventRecord(ctx->bevent[3], ctx->bstream[3]));
+  CK(hipStreamWaitEvent(ctx->stream, ctx->bevent[3], 0));
+  return MST_OK;
 }
 
 void free_batched(BatchedPlan& P) {
