@@ -28,6 +28,8 @@ struct Tune {
   int k1max = 13;            // most rows per lane of the one-wave streamed kernel (5..16)
   int k2max = 8;             // most rows per lane of the two-wave streamed kernel (4..8)
   int branches = 2;          // parallel branches of the graph's class launches (1..4)
+  int wmax = 4;              // most waves of a one-sweep pipeline of 4 rows per lane (4..8);
+                             // longer rows take the multi-sweep kernel
 };
 
 struct BatchedPlan {

@@ -158,6 +158,10 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
       W = (m + 255) / 256;
       K = (m + 64 * W - 1) / (64 * W);
       kind = 5;
+    } else if (nstr > 1 && kPipelined && (m + 255) / 256 <= ctx->tune.wmax) {
+      W = (m + 255) / 256;  // one sweep of W waves, 4 rows per lane (m > 1024: K = 4)
+      K = (m + 64 * W - 1) / (64 * W);
+      kind = 5;
     } else if (nstr > 1 && kSweeps) {
       K = kScore6K;
       kind = 6;
@@ -642,6 +646,7 @@ static int* tune_slot(mst_ctx* ctx, const char* key) {
   if (!strcmp(key, "k1max")) return &ctx->tune.k1max;
   if (!strcmp(key, "k2max")) return &ctx->tune.k2max;
   if (!strcmp(key, "branches")) return &ctx->tune.branches;
+  if (!strcmp(key, "wmax")) return &ctx->tune.wmax;
   if (!strcmp(key, "wg2")) return &ctx->tune.wg_per_cu2;
   if (!strncmp(key, "wg3.", 4)) {
     const int k = atoi(key + 4);
@@ -664,6 +669,8 @@ extern "C" int mst_tune_set(mst_ctx* ctx, const char* key, int value) {
     return ctx->fail(MST_E_ARG, "mst_tune_set", "chunk_w3 out of range 0..4096");
   if (!strcmp(key, "branches") && (value < 1 || value > 4))
     return ctx->fail(MST_E_ARG, "mst_tune_set", "branches out of range 1..4");
+  if (!strcmp(key, "wmax") && (value < 4 || value > 8))
+    return ctx->fail(MST_E_ARG, "mst_tune_set", "wmax out of range 4..8");
   if (!strcmp(key, "k2max") && (value < 4 || value > 8))
     return ctx->fail(MST_E_ARG, "mst_tune_set", "k2max out of range 4..8");
   if (!strcmp(key, "k1max") && (value < 5 || value > 16))

@@ -7,7 +7,8 @@
 // and columns are laid out (device parts in nw_stream.h):
 //
 //  * rows: a pair of row sequences with m rows is cut into W strips of 64*K
-//    rows (K = 1..16 rows per lane for one wave, 3..4 for more); wave j of a
+//    rows (K = 1..16 rows per lane for one wave, 3..8 for two, 3..4 for three
+//    or four, 4 for five to eight); wave j of a
 //    pipeline owns strip j. Lane 63 of wave j writes F and H of its last row for
 //    every column into an LDS ring; lane 0 of wave j+1 reads them one column
 //    behind. Progress counters in LDS (columns produced, columns consumed)
@@ -111,6 +112,7 @@ __global__ void __launch_bounds__(MST_WAVE * W * Pipes<W>::P) score5_kernel(Scor
 bool score5_supported(int K, int W) {
   if (W == 1) return K >= 1 && K <= 16;
   if (W == 2) return K >= 3 && K <= 8;
+  if (W >= 5) return W <= 8 && K == 4;
   return W >= 3 && W <= 4 && K >= 3 && K <= 4;
 }
 
@@ -164,6 +166,10 @@ void launch_score5(int K, int W, const Score5Args& a, int grid, hipStream_t st) 
   MST_CASE(4, 3)
   MST_CASE(3, 4)
   MST_CASE(4, 4)
+  MST_CASE(4, 5)
+  MST_CASE(4, 6)
+  MST_CASE(4, 7)
+  MST_CASE(4, 8)
 #undef MST_CASE
 }
 

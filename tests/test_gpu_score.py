@@ -216,6 +216,30 @@ def test_batched_multi_sweep_rows(ctx, seed, n, lo, hi):
     st.free()
 
 
+@pytest.mark.parametrize("wmax", [6, 8])
+def test_batched_many_wave_rows(ctx, wmax):
+    # rows 1025..256 * wmax in one sweep of five to wmax waves of 4 rows per
+    # lane; longer rows take the multi-sweep kernel
+    rng = np.random.default_rng(60 + wmax)
+    lens = list(rng.integers(1025, 2100, 20)) + [1025, 1280, 1281, 1536, 1537, 1792, 1793, 2048, 2049]
+    seqs = [_rand(rng, int(L)) for L in lens]
+    o, _ = O.all_pairs(seqs, S)
+    st = ctx.upload(seqs)
+    saved = ctx.tune_get("wmax")
+    try:
+        ctx.tune_set("wmax", wmax)
+        g = st.all_pairs(M.FORM_BATCHED)
+        bad = np.argwhere(g != o)
+        assert len(bad) == 0, [(lens[i], lens[j], g[i, j], o[i, j]) for i, j in bad[:10]]
+        g = st.all_pairs(M.FORM_BATCHED)  # graph replay
+        assert (g == o).all()
+        classes = {k for k, _, _, _ in st.launch_stats(M.FORM_BATCHED)}
+        assert {-1504, -1000 - 100 * wmax - 4} <= classes
+    finally:
+        ctx.tune_set("wmax", saved)
+        st.free()
+
+
 def test_config1_sampled_pairs(ctx):
     seqs = M.synth(1, 0)
     st = ctx.upload(seqs)
