@@ -406,6 +406,10 @@ def main():
     launches = [{"kernel": kernel_of(c)[0], "pairs": p, "cells": ce, "ms": round(m, 3),
                  "gcups": round(ce / (m * 1e-3) / 1e9, 1) if m > 0 else None}
                 for c, p, ce, m in stats_b]
+    tb = None if (args.no_traceback or args.batched_only) else traceback_bench(M, ctx, st, L)
+    # the set's plans and graph (whose parallel branches hold a stream of
+    # their own) are dropped before the other workloads
+    st.free()
 
     line = {
         "metric": "all-pairs NW GCUPS (bit-exact), batched form",
@@ -430,7 +434,7 @@ def main():
         "roofline": roof,
         "batched_launches": launches,
         "tunables": dict(ctx.tuning, **tunes),
-        "traceback": None if (args.no_traceback or args.batched_only) else traceback_bench(M, ctx, st, L),
+        "traceback": tb,
         "long_pairs": None if (args.no_long or args.batched_only) else long_pairs_bench(M, ctx),
         "served": None if (args.no_serve or args.batched_only) else served_bench(
             M, ctx, args.serve_jobs, load=args.serve_load,
