@@ -376,7 +376,7 @@ def main():
 
     def kernel_of(cls):
         if cls <= -2000:
-            return "score6_kernel", OPS_PER_CELL, PEAK_INT16_TOPS, "int16x2"
+            return "score6_kernel<%d>" % (-2000 - cls), OPS_PER_CELL, PEAK_INT16_TOPS, "int16x2"
         if cls <= -1000:
             w, k = divmod(-cls - 1000, 100)
             return "score5_kernel<%d, %d>" % (k, w), OPS_PER_CELL, PEAK_INT16_TOPS, "int16x2"

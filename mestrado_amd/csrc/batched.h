@@ -30,6 +30,7 @@ struct Tune {
   int branches = 2;          // parallel branches of the graph's class launches (1..4)
   int wmax = 4;              // most waves of a one-sweep pipeline of 4 rows per lane (4..8);
                              // longer rows take the multi-sweep kernel
+  int k6 = 0;                // rows per lane of the multi-sweep kernel: 3, 4, or 0 (per row pair)
 };
 
 struct BatchedPlan {

@@ -74,7 +74,10 @@ bool pk16_usable(const mst_ctx* ctx) {
 //    else W = ceil(m/256) waves of 64*K rows;
 //  * with tune.shared3 = 1, rows 129..320 take the shared-profile kernel
 //    (score3) instead, in its classes K = 3..5;
-//  * m > 1024: the multi-sweep packed kernel (score6), up to the 16-bit bound;
+//  * m > 1024: the multi-sweep packed kernel (score6), up to the 16-bit bound,
+//    in strips of 64*K rows, K = 3 or 4 (tune.k6, 0: per row pair, the K
+//    whose four-strip sweeps cover m with fewer idle rows, K = 3 counted at
+//    the 0.89 of K = 4's rate per row that one-wave launches measure);
 //  * every pair beyond the bound: the int32 kernel.
 // score3's other classes and the multi-strip packed kernel (score2) are
 // switched off: a run that used them (lengths 1..3100, classes K = 1, 2, 8, 9,
@@ -163,7 +166,12 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
       K = (m + 64 * W - 1) / (64 * W);
       kind = 5;
     } else if (nstr > 1 && kSweeps) {
-      K = kScore6K;
+      K = ctx->tune.k6;
+      if (K == 0) {
+        // row capacity of whole four-strip sweeps, over the per-row rate
+        auto cap = [&](int k) { return (double)((m + 256 * k - 1) / (256 * k)) * 256 * k; };
+        K = cap(3) / 0.89 < cap(4) ? 3 : 4;
+      }
       kind = 6;
     } else if (nstr > 1 && kMultiStripPacked) {
       kind = 2;
@@ -269,10 +277,13 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
     if (a.K != b.K) return a.K < b.K;
     return a.cost > b.cost;
   });
-  std::stable_sort(v6.begin(), v6.end(), [](const It3& a, const It3& b) { return a.cost > b.cost; });
+  std::stable_sort(v6.begin(), v6.end(), [](const It3& a, const It3& b) {
+    if (a.K != b.K) return a.K > b.K;
+    return a.cost > b.cost;
+  });
   for (size_t q = 0; q < v6.size(); ++q) {
     P.items6.push_back(v6[q].it);
-    if (P.c6.empty()) P.c6.push_back(BClass{kScore6K, 0, 0});
+    if (P.c6.empty() || P.c6.back().K != v6[q].K) P.c6.push_back(BClass{v6[q].K, (int64_t)q, (int64_t)q});
     P.c6.back().end = q + 1;
     P.c6.back().work += v6[q].cost;
     P.c6.back().cells += v6[q].cells;
@@ -494,7 +505,7 @@ int enqueue_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P, int32
     a.counter = P.d_counters + ls[k].ctr;
     if (timed) CK(hipEventRecord(ctx->launch_events[ev++], st));
     if (ls[k].six) {
-      launch_score6(a, P.grid6, st);
+      launch_score6(c.K, a, P.grid6, st);
     } else {
       const int per = ctx->tune.wg5 > 0 ? ctx->tune.wg5 : score5_groups_per_cu(c.K, c.W);
       const int64_t g = std::min<int64_t>((int64_t)ctx->num_cus * per, c.end - c.begin);
@@ -639,6 +650,7 @@ static int* tune_slot(mst_ctx* ctx, const char* key) {
   if (!strcmp(key, "k2max")) return &ctx->tune.k2max;
   if (!strcmp(key, "branches")) return &ctx->tune.branches;
   if (!strcmp(key, "wmax")) return &ctx->tune.wmax;
+  if (!strcmp(key, "k6")) return &ctx->tune.k6;
   if (!strcmp(key, "wg2")) return &ctx->tune.wg_per_cu2;
   if (!strncmp(key, "wg3.", 4)) {
     const int k = atoi(key + 4);
@@ -661,6 +673,8 @@ extern "C" int mst_tune_set(mst_ctx* ctx, const char* key, int value) {
     return ctx->fail(MST_E_ARG, "mst_tune_set", "chunk_w3 out of range 0..4096");
   if (!strcmp(key, "branches") && (value < 1 || value > 4))
     return ctx->fail(MST_E_ARG, "mst_tune_set", "branches out of range 1..4");
+  if (!strcmp(key, "k6") && value != 0 && value != 3 && value != 4)
+    return ctx->fail(MST_E_ARG, "mst_tune_set", "k6 is 0, 3 or 4");
   if (!strcmp(key, "wmax") && (value < 4 || value > 8))
     return ctx->fail(MST_E_ARG, "mst_tune_set", "wmax out of range 4..8");
   if (!strcmp(key, "k2max") && (value < 4 || value > 8))

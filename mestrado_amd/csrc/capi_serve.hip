@@ -630,7 +630,7 @@ int launch_round(mst_server* S, int* launched) {
     a.carry_cap = 16384;
     const int64_t nit = ranges[c].second - ranges[c].first;
     if (c == 0) {
-      launch_score6(a, (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)(S->grid6 * share), nit)), st);
+      launch_score6(kScore6K, a, (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)(S->grid6 * share), nit)), st);
     } else {
       int W, K;
       class_shape(c, &K, &W);

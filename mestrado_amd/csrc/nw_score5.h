@@ -36,12 +36,12 @@ int score5_block_threads(int W);
 int score5_groups_per_cu(int K, int W);
 void launch_score5(int K, int W, const Score5Args& a, int grid, hipStream_t st);
 
-// score6: any number of 256-row strips (K = 4), four waves sweeping the
-// stream once per four strips; the last strip of a sweep hands its row to the
-// first of the next through `carry`.
+// score6: any number of strips of 64*K rows (K = 3 or 4), four waves sweeping
+// the stream once per four strips; the last strip of a sweep hands its row to
+// the first of the next through `carry`. kScore6K: the served form's K.
 constexpr int kScore6K = 4;
 int score6_block_threads();
 int score6_groups_per_cu();
-void launch_score6(const Score5Args& a, int grid, hipStream_t st);
+void launch_score6(int K, const Score5Args& a, int grid, hipStream_t st);
 
 }  // namespace mst

@@ -4,7 +4,8 @@
 // strips than waves.
 //
 // A workgroup of four waves takes one item. The rows are cut into NS strips of
-// 256 rows (K = 4 per lane). Wave j sweeps the item's column stream once for
+// 64*K rows (K = 3 or 4 per lane, chosen per row pair by the planner: the one
+// whose sweeps leave fewer idle rows). Wave j sweeps the item's column stream once for
 // each of its strips j, j+4, j+8, ...: in sweep q its strip is s = j + 4q.
 // Strip s reads the row above it from strip s-1: from wave j-1 through an LDS
 // ring in the same sweep, or, for wave 0, from wave 3's previous sweep through
@@ -25,8 +26,8 @@ namespace mst {
 
 constexpr int W6 = 4;
 
+template <int K>
 __global__ void __launch_bounds__(MST_WAVE * W6) score6_kernel(Score5Args a) {
-  constexpr int K = kScore6K;
   constexpr int D = KD<K>::v;
   constexpr int STRIP = MST_LETTERS * MST_WAVE * D * 4;
   constexpr int ROWS = MST_WAVE * K;  // rows per strip
@@ -132,8 +133,12 @@ int score6_groups_per_cu() {
   return g < 1 ? 1 : g;
 }
 
-void launch_score6(const Score5Args& a, int grid, hipStream_t st) {
-  hipLaunchKernelGGL(score6_kernel, dim3(grid), dim3(score6_block_threads()), 0, st, a);
+// K = 3 and 4 have the same profile size (KD = 2), so the same LDS and grid
+void launch_score6(int K, const Score5Args& a, int grid, hipStream_t st) {
+  if (K == 3)
+    hipLaunchKernelGGL(score6_kernel<3>, dim3(grid), dim3(score6_block_threads()), 0, st, a);
+  else
+    hipLaunchKernelGGL(score6_kernel<4>, dim3(grid), dim3(score6_block_threads()), 0, st, a);
 }
 
 }  // namespace mst

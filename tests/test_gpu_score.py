@@ -198,22 +198,31 @@ def test_batched_two_wave_large_k(ctx):
         st.free()
 
 
-@pytest.mark.parametrize("seed,n,lo,hi", [(21, 9, 1025, 2100), (22, 14, 800, 1600)])
-def test_batched_multi_sweep_rows(ctx, seed, n, lo, hi):
+@pytest.mark.parametrize("seed,n,lo,hi,k6", [(21, 9, 1025, 2100, 0), (22, 14, 800, 1600, 0),
+                                              (23, 9, 1025, 2300, 3), (24, 9, 1025, 2300, 4)])
+def test_batched_multi_sweep_rows(ctx, seed, n, lo, hi, k6):
     # rows beyond 1024: four waves sweep the column stream once per four
-    # 256-row strips, the row between sweeps kept in HBM; the longest pairs
-    # exceed the 16-bit bound and take the int32 kernel
+    # strips of 64*K rows (K = 3 or 4), the row between sweeps kept in HBM;
+    # the longest pairs exceed the 16-bit bound and take the int32 kernel
     rng = np.random.default_rng(seed)
     lens = list(rng.integers(lo, hi + 1, n)) + [1025, 1280, 1281, 2048]
     seqs = [_rand(rng, int(L)) for L in lens]
     st = ctx.upload(seqs)
-    g = st.all_pairs(M.FORM_BATCHED)
-    o, _ = O.all_pairs(seqs, S)
-    bad = np.argwhere(g != o)
-    assert len(bad) == 0, [(lens[i], lens[j], g[i, j], o[i, j]) for i, j in bad[:10]]
-    kinds = {r[0] for r in st.launch_stats(M.FORM_BATCHED)}
-    assert -2004 in kinds
-    st.free()
+    saved = ctx.tune_get("k6")
+    try:
+        ctx.tune_set("k6", k6)
+        g = st.all_pairs(M.FORM_BATCHED)
+        o, _ = O.all_pairs(seqs, S)
+        bad = np.argwhere(g != o)
+        assert len(bad) == 0, [(lens[i], lens[j], g[i, j], o[i, j]) for i, j in bad[:10]]
+        kinds = {r[0] for r in st.launch_stats(M.FORM_BATCHED)}
+        if k6:
+            assert -2000 - k6 in kinds and -2007 + k6 not in kinds
+        else:
+            assert {-2003, -2004} <= kinds  # rows 1025..1536 take K = 3, 1537..2048 K = 4
+    finally:
+        ctx.tune_set("k6", saved)
+        st.free()
 
 
 @pytest.mark.parametrize("wmax", [6, 8])
