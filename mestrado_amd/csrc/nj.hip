@@ -46,20 +46,102 @@ __device__ __forceinline__ bool better(const Best& a, const Best& b) {
   return a.I > b.I;
 }
 
+// q of the pairs (pa, pb), pb > pa, of row pa for the lane's columns
+// pb = pa + 1 + lane + 64k, folded into best. Four columns per lane are loaded
+// before any is used (act/ids, then D and T through them), so the two
+// dependent loads of a column overlap those of the next three.
+__device__ __forceinline__ void scan_row(Best& best, int pa, int m, int R, double nm2, const int* act,
+                                         const int* ids, const float* D, const double* T, int ld,
+                                         int lane) {
+  const int ra = act[pa], ia = ids[pa];
+  const double Ta = T[ra];
+  const float* Drow = D + (int64_t)ra * ld;
+  auto fold = [&](int pb, int rb, int ib, float dv, double Tb) {
+    const double dq = (double)dv * nm2;
+    // pair created by an insertion after the last rebuild: storage order
+    const bool swap = (ib >= R) && (rb < ra);
+    const double q = swap ? (dq - Tb) - Ta : (dq - Ta) - Tb;
+    Best c;
+    c.q = q;
+    c.I = ia;
+    c.J = ib;
+    c.pa = pa;
+    c.pb = pb;
+    if (better(c, best)) best = c;
+  };
+  int pb = pa + 1 + lane;
+  for (; pb + 3 * 64 < m; pb += 4 * 64) {
+    int rb[4], ib[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      rb[k] = act[pb + 64 * k];
+      ib[k] = ids[pb + 64 * k];
+    }
+    float dv[4];
+    double tb[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      dv[k] = Drow[rb[k]];
+      tb[k] = T[rb[k]];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) fold(pb + 64 * k, rb[k], ib[k], dv[k], tb[k]);
+  }
+  for (; pb < m; pb += 64) {
+    const int rb = act[pb];
+    fold(pb, rb, ids[pb], Drow[rb], T[rb]);
+  }
+}
+
 // New row of a join, kept in LDS as it is written (up to kNjLdsRow active
 // nodes), so the fp64 sum that gives T of the new node, which must run in id
 // order on one thread, reads LDS instead of a chain of dependent loads.
 constexpr int kNjLdsRow = 8192;
 
+// s + x[b] + x[b+1] + ... + x[e-1], left to right in fp64, x[p] = get(p):
+// blocks of 16 values loaded one block ahead of the adds that use them, so the
+// only serial chain is the adds' own (the rounding order stays position order)
+template <typename Get>
+__device__ __forceinline__ double run_sum(Get get, int b, int e, double s) {
+  constexpr int B = 16;
+  if (e - b < B) {
+    for (int p = b; p < e; ++p) s += (double)get(p);
+    return s;
+  }
+  float cur[B], nxt[B];
+#pragma unroll
+  for (int k = 0; k < B; ++k) cur[k] = get(b + k);
+  int p = b;
+  for (; p + 2 * B <= e; p += B) {
+#pragma unroll
+    for (int k = 0; k < B; ++k) nxt[k] = get(p + B + k);
+#pragma unroll
+    for (int k = 0; k < B; ++k) s += (double)cur[k];
+#pragma unroll
+    for (int k = 0; k < B; ++k) cur[k] = nxt[k];
+  }
+#pragma unroll
+  for (int k = 0; k < B; ++k) s += (double)cur[k];
+  for (p += B; p < e; ++p) s += (double)get(p);
+  return s;
+}
+
+// T of the new node: the new row summed in position order, positions pa and
+// pb (the joined pair) left out, as three runs without a test per element
 __device__ __forceinline__ double new_row_sum(const float* rowv, const float* Drow, const int* act, int m,
                                               int pa, int pb) {
+  const int lo = pa < pb ? pa : pb, hi = pa < pb ? pb : pa;
   double s = 0.0;
   if (m <= kNjLdsRow) {
-    for (int p = 0; p < m; ++p)
-      if (p != pa && p != pb) s += (double)rowv[p];
+    auto get = [&](int p) { return rowv[p]; };
+    s = run_sum(get, 0, lo, s);
+    s = run_sum(get, lo + 1, hi, s);
+    s = run_sum(get, hi + 1, m, s);
   } else {
-    for (int p = 0; p < m; ++p)
-      if (p != pa && p != pb) s += (double)Drow[act[p]];
+    auto get = [&](int p) { return Drow[act[p]]; };
+    s = run_sum(get, 0, lo, s);
+    s = run_sum(get, lo + 1, hi, s);
+    s = run_sum(get, hi + 1, m, s);
   }
   return s;
 }
@@ -107,26 +189,8 @@ __device__ __forceinline__ void nj_run(const NjArgs a) {
     best.pa = best.pb = -1;
     // one wave per row pa (rows dealt round robin), lanes stride the columns
     const int wave = tid >> 6, lane = tid & 63;
-    for (int pa = wave; pa < m - 1; pa += NJ_THREADS / 64) {
-      const int ra = act[pa], ia = ids[pa];
-      const double Ta = T[ra];
-      const float* Drow = D + (int64_t)ra * ld;
-      for (int pb = pa + 1 + lane; pb < m; pb += 64) {
-        const int rb = act[pb], ib = ids[pb];
-        const double dq = (double)Drow[rb] * nm2;
-        // pair created by an insertion after the last rebuild: storage order
-        const bool swap = (ib >= R) && (rb < ra);
-        const double Tb = T[rb];
-        const double q = swap ? (dq - Tb) - Ta : (dq - Ta) - Tb;
-        Best c;
-        c.q = q;
-        c.I = ia;
-        c.J = ib;
-        c.pa = pa;
-        c.pb = pb;
-        if (better(c, best)) best = c;
-      }
-    }
+    for (int pa = wave; pa < m - 1; pa += NJ_THREADS / 64)
+      scan_row(best, pa, m, R, nm2, act, ids, D, T, ld, lane);
     // wave reduction then workgroup reduction
     for (int off = 32; off > 0; off >>= 1) {
       Best o;
@@ -281,25 +345,8 @@ __global__ void __launch_bounds__(NJ_SCAN_THREADS) nj_scan_kernel(NjArgs a, cons
   best.J = -1;
   best.I = -1;
   best.pa = best.pb = -1;
-  for (int pa = blockIdx.x * (NJ_SCAN_THREADS / 64) + wave; pa < m - 1; pa += nwaves) {
-    const int ra = act[pa], ia = ids[pa];
-    const double Ta = T[ra];
-    const float* Drow = D + (int64_t)ra * ld;
-    for (int pb = pa + 1 + lane; pb < m; pb += 64) {
-      const int rb = act[pb], ib = ids[pb];
-      const double dq = (double)Drow[rb] * nm2;
-      const bool swap = (ib >= R) && (rb < ra);
-      const double Tb = T[rb];
-      const double q = swap ? (dq - Tb) - Ta : (dq - Ta) - Tb;
-      Best c;
-      c.q = q;
-      c.I = ia;
-      c.J = ib;
-      c.pa = pa;
-      c.pb = pb;
-      if (better(c, best)) best = c;
-    }
-  }
+  for (int pa = blockIdx.x * (NJ_SCAN_THREADS / 64) + wave; pa < m - 1; pa += nwaves)
+    scan_row(best, pa, m, R, nm2, act, ids, D, T, ld, lane);
   best = wave_best(best);
   if (lane == 0) red[wave] = best;
   __syncthreads();
