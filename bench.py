@@ -297,6 +297,8 @@ def main():
     ap.add_argument("--no-traceback", action="store_true", help="skip the alignment-form measurement")
     ap.add_argument("--no-long", action="store_true", help="skip the configuration 3 measurement")
     ap.add_argument("--long-only", action="store_true", help="only the configuration 3 measurement")
+    ap.add_argument("--nj-only", action="store_true",
+                    help="only the NJ guide tree of configurations 1 and 2 (after their batched all pairs)")
     ap.add_argument("--profile-set", action="store_true",
                     help="one pass of each profiled kernel (plain copy of 1 GiB, batched all pairs, "
                          "alignment form) for counter runs")
@@ -317,6 +319,20 @@ def main():
     self_cells = int((L * L).sum())
 
     ctx = M.Context(device=0)
+    if args.nj_only:
+        out = {}
+        for cfg in (1, 2):
+            s = M.synth(cfg, 0)
+            st = ctx.upload(s)
+            st.all_pairs_resident(M.FORM_BATCHED)
+            t0 = time.perf_counter()
+            joins = st.nj()
+            out["config%d" % cfg] = {"sequences": len(s), "joins": len(joins),
+                                     "nj_ms": round((time.perf_counter() - t0) * 1e3, 1)}
+            st.free()
+        print(json.dumps({"nj": out}), flush=True)
+        ctx.close()
+        return 0
     if args.long_only:
         print(json.dumps({"long_pairs": long_pairs_bench(M, ctx)}), flush=True)
         ctx.close()
@@ -407,6 +423,11 @@ def main():
                  "gcups": round(ce / (m * 1e-3) / 1e9, 1) if m > 0 else None}
                 for c, p, ce, m in stats_b]
     tb = None if (args.no_traceback or args.batched_only) else traceback_bench(M, ctx, st, L)
+    nj_ms = None
+    if not args.batched_only:
+        t0 = time.perf_counter()
+        st.nj()  # guide tree of the set from the resident scores
+        nj_ms = round((time.perf_counter() - t0) * 1e3, 1)
     # the set's plans and graph (whose parallel branches hold a stream of
     # their own) are dropped before the other workloads
     st.free()
@@ -435,6 +456,7 @@ def main():
         "batched_launches": launches,
         "tunables": dict(ctx.tuning, **tunes),
         "traceback": tb,
+        "nj_ms": nj_ms,
         "long_pairs": None if (args.no_long or args.batched_only) else long_pairs_bench(M, ctx),
         "served": None if (args.no_serve or args.batched_only) else served_bench(
             M, ctx, args.serve_jobs, load=args.serve_load,
