@@ -33,11 +33,21 @@ struct Pipes {
   static constexpr int P = W == 1 ? 4 : (W == 2 ? 2 : 1);
 };
 
+// One-wave pipelines of up to 8 rows per lane keep the profile in the wide
+// form (a dword per row, no unpacking per cell): its LDS, up to 49 KB for the
+// four pipelines' shared profile, still leaves three workgroups per CU.
+constexpr int kWideMaxK = 8;
+template <int K, int W>
+struct Wide {
+  static constexpr bool v = W == 1 && K <= kWideMaxK;
+};
+
 template <int K, int W>
 __global__ void __launch_bounds__(MST_WAVE * W * Pipes<W>::P) score5_kernel(Score5Args a) {
   constexpr int P = Pipes<W>::P;
   constexpr int NW = W * P;
-  constexpr int D = KD<K>::v;
+  constexpr bool WIDE = Wide<K, W>::v;
+  constexpr int D = KDW<K, WIDE>::v;
   constexpr int NR = W > 1 ? P * (W - 1) : 1;
   constexpr int STRIP = MST_LETTERS * MST_WAVE * D * 4;  // profile bytes of one strip
   __shared__ int8_t S_lds[MST_LETTERS * MST_LETTERS];
@@ -64,7 +74,7 @@ __global__ void __launch_bounds__(MST_WAVE * W * Pipes<W>::P) score5_kernel(Scor
     Stream S = item_stream(item, it, a);
     // profile of every strip: wave wv fills residues wv, wv+NW, ...
     for (int js = 0; js < W; ++js)
-      build_strip_profile<K>(prof + js * STRIP, S_lds, S, a, js * MST_WAVE * K, wv, NW, bonus);
+      build_strip_profile<K, WIDE>(prof + js * STRIP, S_lds, S, a, js * MST_WAVE * K, wv, NW, bonus);
     if (threadIdx.x < NW) {
       s_prog[threadIdx.x] = 0;
       s_cons[threadIdx.x] = 0;
@@ -92,16 +102,16 @@ __global__ void __launch_bounds__(MST_WAVE * W * Pipes<W>::P) score5_kernel(Scor
       int* prog = s_prog + p * W;
       int* cons = s_cons + p * W;
       if (W == 1) {
-        sweep5<K, true, false>(kb, r0, S, prof_lane, G2, in, 0, nullptr, nullptr, out, 0, nullptr,
+        sweep5<K, true, false, WIDE>(kb, r0, S, prof_lane, G2, in, 0, nullptr, nullptr, out, 0, nullptr,
                                nullptr, 0, 1, a);
       } else if (j == 0) {
-        sweep5<K, true, true>(kb, r0, S, prof_lane, G2, in, 0, nullptr, nullptr, out, 0, prog + j,
+        sweep5<K, true, true, WIDE>(kb, r0, S, prof_lane, G2, in, 0, nullptr, nullptr, out, 0, prog + j,
                               cons + j + 1, RING5, 1, a);
       } else if (j == W - 1) {
-        sweep5<K, false, false>(kb, r0, S, prof_lane, G2, in, 0, prog + j - 1, cons + j, out, 0,
+        sweep5<K, false, false, WIDE>(kb, r0, S, prof_lane, G2, in, 0, prog + j - 1, cons + j, out, 0,
                                 nullptr, nullptr, 0, 1, a);
       } else {
-        sweep5<K, false, true>(kb, r0, S, prof_lane, G2, in, 0, prog + j - 1, cons + j, out, 0,
+        sweep5<K, false, true, WIDE>(kb, r0, S, prof_lane, G2, in, 0, prog + j - 1, cons + j, out, 0,
                                prog + j, cons + j + 1, RING5, 1, a);
       }
     }
@@ -123,7 +133,9 @@ int score5_block_threads(int W) { return MST_WAVE * W * pipes(W); }
 // LDS of one workgroup (profile, rings, counters, matrix) -> workgroups per CU
 int score5_groups_per_cu(int K, int W) {
   const int P = pipes(W);
-  const int D = K <= 2 ? 1 : (K <= 4 ? 2 : (K <= 8 ? 4 : 8));
+  const bool wide = W == 1 && K <= kWideMaxK;
+  const int D = wide ? (K <= 1 ? 1 : (K <= 2 ? 2 : (K <= 4 ? 4 : (K <= 8 ? 8 : 16))))
+                     : (K <= 2 ? 1 : (K <= 4 ? 2 : (K <= 8 ? 4 : 8)));
   const int nr = W > 1 ? P * (W - 1) : 1;
   const int lds = W * MST_LETTERS * MST_WAVE * D * 4 + nr * RING5 * 8 + 8 * W * P + 4 +
                   MST_LETTERS * MST_LETTERS;

@@ -87,7 +87,7 @@ __device__ __forceinline__ void emit(const Lane2<K>& L, const Own& ow, int kx, c
 // One step t of the wave: lane l computes column c = t - l of its K rows.
 // w: the lane's profile dwords for the column's residue; bv: for lane 0, F and
 // H of the row above at column t (the other lanes take them from lane l-1).
-template <int K, bool PRED, bool CHECK, bool FIRSTW, bool STORE>
+template <int K, bool PRED, bool CHECK, bool FIRSTW, bool STORE, bool WIDE>
 __device__ __forceinline__ void step5(Lane2<K>& L, Cursor& cu, const Own& ow, int t,
                                       const uint32_t* w, int2 bv, const Stream& S, s2 G2,
                                       const Link& out, const Score5Args& a) {
@@ -122,7 +122,7 @@ __device__ __forceinline__ void step5(Lane2<K>& L, Cursor& cu, const Own& ow, in
     for (int i = 0; i < K; ++i) {
       const s2 e = pmax(L.E[i], L.Hm[i]);
       f = pmax(f, hu);
-      const s2 dg = hd + prow(w, i);
+      const s2 dg = hd + prow_w<WIDE>(w, i);
       const s2 h = pmax(pmax(e, f), dg) - G2;
       hd = L.Hm[i];
       L.Hm[i] = h;
@@ -178,13 +178,13 @@ __device__ __forceinline__ int code_at(uint2 v, int j) {
 // 8-byte load per lane), reads the 8 steps' profile dwords from LDS before the
 // first step, and reloads each step's `in` word for the next block right after
 // the step has used it, so no register moves between blocks.
-template <int K, bool FIRSTW, bool STORE>
+template <int K, bool FIRSTW, bool STORE, bool WIDE = false>
 __device__ __forceinline__ void sweep5(int kb, int r0, const Stream& S, const uint8_t* prof_lane,
                                        s2 G2, const Link& in, int gin, const int* prog_in,
                                        int* cons_out, const Link& out, int gout, int* prog_out,
                                        const int* cons_in, int out_span, int pub_every,
                                        const Score5Args& a) {
-  constexpr int D = KD<K>::v;
+  constexpr int D = KDW<K, WIDE>::v;
   const int lane = lane_id();
   const int ncol = S.ncol;
   const int T = ncol + MST_WAVE - 1;
@@ -236,7 +236,7 @@ __device__ __forceinline__ void sweep5(int kb, int r0, const Stream& S, const ui
 #define MST_STEPS(PRED, CHECK)                                                            \
   _Pragma("unroll") for (int j = 0; j < U5; ++j) {                                        \
     if (!PRED || tb + j < T)                                                              \
-      step5<K, PRED, CHECK, FIRSTW, STORE>(L, cu, ow, tb + j, w[j], bv[j], S, G2, out, a); \
+      step5<K, PRED, CHECK, FIRSTW, STORE, WIDE>(L, cu, ow, tb + j, w[j], bv[j], S, G2, out, a); \
     if (!FIRSTW) bv[j] = in.p[(in.base + min(tb + U5 + j, ncol - 1)) & in.mask];           \
   }
     if (full && !cross) {
@@ -261,13 +261,13 @@ __device__ __forceinline__ void sweep5(int kb, int r0, const Stream& S, const ui
 }
 
 // Profile bytes S[Y_r][x] + go + ge of rows r0 + lane*K + i for the residues
-// x = x0, x0 + dx, ... < 24, two bytes (Ya, Yb) per row, into prof laid out
-// [residue][lane][D dwords].
-template <int K>
+// x = x0, x0 + dx, ... < 24, two bytes (Ya, Yb) per row (WIDE: each in a
+// 16-bit half of the row's dword), into prof laid out [residue][lane][D dwords].
+template <int K, bool WIDE = false>
 __device__ __forceinline__ void build_strip_profile(uint8_t* prof, const int8_t* S_lds, const Stream& S,
                                                     const Score5Args& a, int r0, int x0, int dx,
                                                     int bonus) {
-  constexpr int D = KD<K>::v;
+  constexpr int D = KDW<K, WIDE>::v;
   const int lane = lane_id();
   const int8_t* Ya = a.scodes + a.soff[S.ya];
   const int8_t* Yb = a.scodes + a.soff[S.yb >= 0 ? S.yb : S.ya];
@@ -287,7 +287,10 @@ __device__ __forceinline__ void build_strip_profile(uint8_t* prof, const int8_t*
     for (int i = 0; i < K; ++i) {
       const uint32_t b0 = (uint32_t)((int)S_lds[ra[i] + x] + bonus) & 0xFFu;
       const uint32_t b1 = (uint32_t)((int)S_lds[rb[i] + x] + bonus) & 0xFFu;
-      v[i >> 1] |= (b0 | (b1 << 8)) << (16 * (i & 1));
+      if (WIDE)
+        v[i] = b0 | (b1 << 16);
+      else
+        v[i >> 1] |= (b0 | (b1 << 8)) << (16 * (i & 1));
     }
     uint32_t* dst = (uint32_t*)(prof + (x * MST_WAVE + lane) * D * 4);
 #pragma unroll

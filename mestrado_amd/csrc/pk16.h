@@ -18,6 +18,15 @@ struct KD {
   static constexpr int v = raw <= 1 ? 1 : (raw <= 2 ? 2 : (raw <= 4 ? 4 : 8));
 };
 
+// The wide form: one dword per row, the (pair a, pair b) bytes already
+// zero-extended into the two 16-bit halves, so a cell update reads its row's
+// profile word as is (no v_perm per row); twice the LDS of the byte form.
+template <int K, bool WIDE>
+struct KDW {
+  static constexpr int v =
+      !WIDE ? KD<K>::v : (K <= 1 ? 1 : (K <= 2 ? 2 : (K <= 4 ? 4 : (K <= 8 ? 8 : 16))));
+};
+
 template <int N>
 __device__ __forceinline__ void read_dwords(const uint8_t* p, uint32_t* w) {
   if (N == 1) {
@@ -42,6 +51,10 @@ __device__ __forceinline__ void read_dwords(const uint8_t* p, uint32_t* w) {
 __device__ __forceinline__ s2 prow(const uint32_t* w, int i) {
   const uint32_t sel = (i & 1) ? 0x0C030C02u : 0x0C010C00u;
   return as_s2(__builtin_amdgcn_perm(w[i >> 1], w[i >> 1], sel));
+}
+template <bool WIDE>
+__device__ __forceinline__ s2 prow_w(const uint32_t* w, int i) {
+  return WIDE ? as_s2(w[i]) : prow(w, i);
 }
 
 __device__ __forceinline__ uint32_t shr1z(uint32_t v) {
