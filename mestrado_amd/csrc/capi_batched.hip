@@ -361,7 +361,10 @@ int upload_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P) {
     CK(hipMemcpy(P.d_items6, P.items6.data(), sizeof(Item3) * P.items6.size(), hipMemcpyHostToDevice));
     P.grid6 = (int)std::max<int64_t>(
         1, std::min<int64_t>((int64_t)ctx->num_cus * score6_groups_per_cu(), (int64_t)P.items6.size()));
-    CK(hipMalloc(&P.d_carry, sizeof(int2) * (size_t)P.grid6 * std::max<int64_t>(P.carry_cols, 1)));
+    // a row buffer per workgroup and per multi-sweep class: the classes may
+    // run at the same time on parallel branches of the graph
+    CK(hipMalloc(&P.d_carry, sizeof(int2) * (size_t)P.grid6 * std::max<int64_t>(P.carry_cols, 1) *
+                                 P.c6.size()));
   }
   if (!P.c5.empty()) {
     CK(hipMalloc(&P.d_items5, sizeof(Item3) * P.items5.size()));
@@ -503,6 +506,7 @@ int enqueue_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P, int32
     hipStream_t st = br[where[k]];
     Score5Args a = args5(c, ls[k].six ? P.d_items6 : P.d_items5);
     a.counter = P.d_counters + ls[k].ctr;
+    if (ls[k].six) a.carry += (int64_t)(&c - P.c6.data()) * P.grid6 * a.carry_cap;
     if (timed) CK(hipEventRecord(ctx->launch_events[ev++], st));
     if (ls[k].six) {
       launch_score6(c.K, a, P.grid6, st);

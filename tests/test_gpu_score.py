@@ -339,6 +339,29 @@ def test_config2_subset_all_pairs_batched(ctx):
     st.free()
 
 
+def test_config2_full_matrix_forms_agree(ctx):
+    # the whole 4,000-sequence matrix (1.8e12 cells, every packed class, the
+    # multi-sweep kernel at both K and the int32 pairs): the batched form equals
+    # the plain int32 form entry for entry, both are symmetric with the self
+    # scores on the diagonal, and a random sample of entries equals the oracle
+    seqs = M.synth(2, 0)
+    st = ctx.upload(seqs)
+    gb = st.all_pairs(M.FORM_BATCHED)
+    gp = st.all_pairs(M.FORM_PLAIN)
+    bad = np.argwhere(gb != gp)
+    assert len(bad) == 0, [(len(seqs[i]), len(seqs[j]), gb[i, j], gp[i, j]) for i, j in bad[:8]]
+    assert (gb == gb.T).all()
+    kinds = {r[0] for r in st.launch_stats(M.FORM_BATCHED)}
+    assert {-2003, -2004, -1105, -1208} <= kinds and any(k > 0 for k in kinds)
+    rng = np.random.default_rng(33)
+    pi = rng.integers(0, len(seqs), 1500)
+    pj = rng.integers(0, len(seqs), 1500)
+    codes, offs = O.pack_seqs(seqs)
+    o, _ = O.pair_list(codes, offs, pi, pj, S)
+    assert (gb[pi, pj] == o).all()
+    st.free()
+
+
 def test_batched_launch_stats_cover_all_cells(ctx):
     seqs = M.synth(0, 1)
     st = ctx.upload(seqs)
