@@ -33,10 +33,11 @@ struct Pipes {
   static constexpr int P = W == 1 ? 4 : (W == 2 ? 2 : 1);
 };
 
-// One-wave pipelines of up to 8 rows per lane keep the profile in the wide
-// form (a dword per row, no unpacking per cell): its LDS, up to 49 KB for the
-// four pipelines' shared profile, still leaves three workgroups per CU.
-constexpr int kWideMaxK = 8;
+// One-wave pipelines of up to 12 rows per lane keep the profile in the wide
+// form (a dword per row, no unpacking per cell): its LDS, the four pipelines'
+// shared profile, is 49 KB up to 8 rows per lane (three workgroups per CU) and
+// 74 KB for 9..12 (12 dwords per lane, two workgroups per CU).
+constexpr int kWideMaxK = 12;
 template <int K, int W>
 struct Wide {
   static constexpr bool v = W == 1 && K <= kWideMaxK;
@@ -134,7 +135,7 @@ int score5_block_threads(int W) { return MST_WAVE * W * pipes(W); }
 int score5_groups_per_cu(int K, int W) {
   const int P = pipes(W);
   const bool wide = W == 1 && K <= kWideMaxK;
-  const int D = wide ? (K <= 1 ? 1 : (K <= 2 ? 2 : (K <= 4 ? 4 : (K <= 8 ? 8 : 16))))
+  const int D = wide ? (K <= 1 ? 1 : (K <= 2 ? 2 : (K <= 4 ? 4 : (K <= 8 ? 8 : (K <= 12 ? 12 : 16)))))
                      : (K <= 2 ? 1 : (K <= 4 ? 2 : (K <= 8 ? 4 : 8)));
   const int nr = W > 1 ? P * (W - 1) : 1;
   const int lds = W * MST_LETTERS * MST_WAVE * D * 4 + nr * RING5 * 8 + 8 * W * P + 4 +

@@ -24,7 +24,7 @@ struct KD {
 template <int K, bool WIDE>
 struct KDW {
   static constexpr int v =
-      !WIDE ? KD<K>::v : (K <= 1 ? 1 : (K <= 2 ? 2 : (K <= 4 ? 4 : (K <= 8 ? 8 : 16))));
+      !WIDE ? KD<K>::v : (K <= 1 ? 1 : (K <= 2 ? 2 : (K <= 4 ? 4 : (K <= 8 ? 8 : (K <= 12 ? 12 : 16)))));
 };
 
 template <int N>
