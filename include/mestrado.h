@@ -102,7 +102,9 @@ int mst_last_launch_stats(mst_ctx* ctx, int* rows_per_lane, int64_t* pairs, int6
  * pipeline of 4 rows per lane for rows 1025..256 * wmax; 4: every row pair
  * above 1024 rows takes the multi-sweep kernel, the default), "k6" (rows per
  * lane of the multi-sweep kernel: 3, 4, or 0 = per row pair, the one whose
- * four-strip sweeps leave fewer idle rows; default 0), "wg5", "wg2" and
+ * four-strip sweeps leave fewer idle rows; default 0), "chunk6" (column
+ * sequences per work item of the multi-sweep kernel, 0 = "chunk"; default
+ * 0), "wg5", "wg2" and
  * "wg3.<K>" (workgroups per CU, 0 = default). Setting a value drops the cached
  * batched plans. Unknown keys or values out of range return MST_E_ARG. */
 int mst_tune_set(mst_ctx* ctx, const char* key, int value);

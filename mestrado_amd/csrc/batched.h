@@ -31,6 +31,7 @@ struct Tune {
   int wmax = 4;              // most waves of a one-sweep pipeline of 4 rows per lane (4..8);
                              // longer rows take the multi-sweep kernel
   int k6 = 0;                // rows per lane of the multi-sweep kernel: 3, 4, or 0 (per row pair)
+  int chunk6 = 0;            // column sequences per item of the multi-sweep kernel (0: chunk)
 };
 
 struct BatchedPlan {

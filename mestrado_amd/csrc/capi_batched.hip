@@ -203,7 +203,8 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
       }
       if (kind == 3 || kind == 5 || kind == 6) {
         // pipelines of three or four waves take their own item size
-        const int Cw = (kind == 5 && W >= 3 && ctx->tune.chunk_w3 > 0) ? ctx->tune.chunk_w3 : C;
+        int Cw = (kind == 5 && W >= 3 && ctx->tune.chunk_w3 > 0) ? ctx->tune.chunk_w3 : C;
+        if (kind == 6 && ctx->tune.chunk6 > 0) Cw = ctx->tune.chunk6;
         size_t c0 = 0;
         while (c0 < elig.size()) {
           // a chunk is a run of consecutive positions; score6 keeps its column
@@ -655,6 +656,7 @@ static int* tune_slot(mst_ctx* ctx, const char* key) {
   if (!strcmp(key, "branches")) return &ctx->tune.branches;
   if (!strcmp(key, "wmax")) return &ctx->tune.wmax;
   if (!strcmp(key, "k6")) return &ctx->tune.k6;
+  if (!strcmp(key, "chunk6")) return &ctx->tune.chunk6;
   if (!strcmp(key, "wg2")) return &ctx->tune.wg_per_cu2;
   if (!strncmp(key, "wg3.", 4)) {
     const int k = atoi(key + 4);
@@ -673,6 +675,8 @@ extern "C" int mst_tune_set(mst_ctx* ctx, const char* key, int value) {
     return ctx->fail(MST_E_ARG, "mst_tune_set", "shared3 is 0 or 1");
   if (!strcmp(key, "k5min") && (value < 1 || value > 3))
     return ctx->fail(MST_E_ARG, "mst_tune_set", "k5min out of range 1..3");
+  if (!strcmp(key, "chunk6") && (value < 0 || value > 4096))
+    return ctx->fail(MST_E_ARG, "mst_tune_set", "chunk6 out of range 0..4096");
   if (!strcmp(key, "chunk_w3") && (value < 0 || value > 4096))
     return ctx->fail(MST_E_ARG, "mst_tune_set", "chunk_w3 out of range 0..4096");
   if (!strcmp(key, "branches") && (value < 1 || value > 4))
@@ -685,7 +689,8 @@ extern "C" int mst_tune_set(mst_ctx* ctx, const char* key, int value) {
     return ctx->fail(MST_E_ARG, "mst_tune_set", "k2max out of range 4..8");
   if (!strcmp(key, "k1max") && (value < 5 || value > 16))
     return ctx->fail(MST_E_ARG, "mst_tune_set", "k1max out of range 5..16");
-  if (strcmp(key, "chunk") && strcmp(key, "shared3") && strcmp(key, "chunk_w3") && (value < 0 || value > 64))
+  if (strcmp(key, "chunk") && strcmp(key, "shared3") && strcmp(key, "chunk_w3") && strcmp(key, "chunk6") &&
+      (value < 0 || value > 64))
     return ctx->fail(MST_E_ARG, "mst_tune_set", "workgroups per CU out of range 0..64");
   if (*slot == value) return MST_OK;
   hipSetDevice(ctx->device);
