@@ -198,9 +198,9 @@ def test_batched_two_wave_large_k(ctx):
         st.free()
 
 
-@pytest.mark.parametrize("seed,n,lo,hi,k6", [(21, 9, 1025, 2100, 0), (22, 14, 800, 1600, 0),
-                                              (23, 9, 1025, 2300, 3), (24, 9, 1025, 2300, 4)])
-def test_batched_multi_sweep_rows(ctx, seed, n, lo, hi, k6):
+@pytest.mark.parametrize("seed,n,lo,hi,k6,chunk6", [(21, 9, 1025, 2100, 0, 0), (22, 14, 800, 1600, 0, 4),
+                                                     (23, 9, 1025, 2300, 3, 1), (24, 9, 1025, 2300, 4, 2)])
+def test_batched_multi_sweep_rows(ctx, seed, n, lo, hi, k6, chunk6):
     # rows beyond 1024: four waves sweep the column stream once per four
     # strips of 64*K rows (K = 3 or 4), the row between sweeps kept in HBM;
     # the longest pairs exceed the 16-bit bound and take the int32 kernel
@@ -208,9 +208,10 @@ def test_batched_multi_sweep_rows(ctx, seed, n, lo, hi, k6):
     lens = list(rng.integers(lo, hi + 1, n)) + [1025, 1280, 1281, 2048]
     seqs = [_rand(rng, int(L)) for L in lens]
     st = ctx.upload(seqs)
-    saved = ctx.tune_get("k6")
+    saved = ctx.tune_get("k6"), ctx.tune_get("chunk6")
     try:
         ctx.tune_set("k6", k6)
+        ctx.tune_set("chunk6", chunk6)
         g = st.all_pairs(M.FORM_BATCHED)
         o, _ = O.all_pairs(seqs, S)
         bad = np.argwhere(g != o)
@@ -221,7 +222,8 @@ def test_batched_multi_sweep_rows(ctx, seed, n, lo, hi, k6):
         else:
             assert {-2003, -2004} <= kinds  # rows 1025..1536 take K = 3, 1537..2048 K = 4
     finally:
-        ctx.tune_set("k6", saved)
+        ctx.tune_set("k6", saved[0])
+        ctx.tune_set("chunk6", saved[1])
         st.free()
 
 
