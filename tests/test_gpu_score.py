@@ -216,6 +216,8 @@ def test_batched_multi_sweep_rows(ctx, seed, n, lo, hi, k6, chunk6):
         o, _ = O.all_pairs(seqs, S)
         bad = np.argwhere(g != o)
         assert len(bad) == 0, [(lens[i], lens[j], g[i, j], o[i, j]) for i, j in bad[:10]]
+        for _ in range(2):  # graph replays (the classes on parallel branches)
+            assert (st.all_pairs(M.FORM_BATCHED) == o).all()
         kinds = {r[0] for r in st.launch_stats(M.FORM_BATCHED)}
         if k6:
             assert -2000 - k6 in kinds and -2007 + k6 not in kinds
