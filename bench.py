@@ -391,6 +391,8 @@ def main():
     gcups_p = cells / sec_p / 1e9 if sec_p else None
 
     def kernel_of(cls):
+        if cls <= -3000:
+            return "score5_kernel<%d, 1, stacked>" % (-3000 - cls), OPS_PER_CELL, PEAK_INT16_TOPS, "int16x2"
         if cls <= -2000:
             return "score6_kernel<%d>" % (-2000 - cls), OPS_PER_CELL, PEAK_INT16_TOPS, "int16x2"
         if cls <= -1000:

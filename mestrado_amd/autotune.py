@@ -27,7 +27,7 @@ import sys
 import numpy as np
 
 TUNING_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning")
-KEYS = ("chunk", "shared3", "wg5", "k5min", "chunk_w3", "k1max", "k2max", "branches", "wmax", "k6", "chunk6")
+KEYS = ("chunk", "shared3", "wg5", "k5min", "chunk_w3", "k1max", "k2max", "branches", "wmax", "k6", "chunk6", "stack")
 BUCKETS = ((256, "<=256"), (1024, "<=1024"), (None, ">1024"))
 
 

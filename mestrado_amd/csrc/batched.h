@@ -14,6 +14,7 @@ struct BClass {
   int64_t cells = 0;  // useful DP cells
   int64_t pairs = 0;
   int W = 1;          // waves per pipeline (pipelined packed kernel)
+  bool stacked = false;  // two row pairs per strip (one-wave kernel)
 };
 // Tunables of the batched form (set through mst_tune_set; the autotuner in
 // mestrado_amd/autotune.py searches them and persists the best per device).
@@ -32,6 +33,7 @@ struct Tune {
                              // longer rows take the multi-sweep kernel
   int k6 = 0;                // rows per lane of the multi-sweep kernel: 3, 4, or 0 (per row pair)
   int chunk6 = 0;            // column sequences per item of the multi-sweep kernel (0: chunk)
+  int stack = 0;             // 1: two row pairs of K = 2, 3 rows per lane share a strip of 2K
 };
 
 struct BatchedPlan {
@@ -42,6 +44,8 @@ struct BatchedPlan {
   std::vector<mst::Pair2> items2;
   std::vector<BClass> c2;
   std::vector<mst::Item3> items5;  // pipelined packed kernel (score5)
+  std::vector<int2> pair2_5;       // their second row pair (stacked items; else -1)
+  int2* d_pair2 = nullptr;
   std::vector<BClass> c5;
   std::vector<mst::Item3> items6;  // multi-sweep packed kernel (score6)
   std::vector<BClass> c6;

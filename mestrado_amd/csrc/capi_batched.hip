@@ -113,6 +113,7 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
     Item3 it;
     int K, W;
     int64_t cost, cells, pairs;
+    int2 p2 = make_int2(-1, -1);  // stacked items: the second row pair
   };
   struct It2 {
     Pair2 it;
@@ -250,6 +251,72 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
         }
       }
     };
+    // Stacking: this row pair and the next, both of the one-wave classes of
+    // K = 2 or 3 rows per lane and of the same K, share a strip of 2K rows
+    // per lane. With a symmetric matrix the stacked items take every column
+    // from this pair's ya on: the few pairs both row pairs then meet (each
+    // other's rows) are scored twice, to the same value; otherwise over their
+    // common columns only (from the next pair's yb on), the rest as before.
+    const bool stack = ctx->tune.stack && !ctx->tune.shared3 && kind == 5 && W == 1 && K >= 2 && K <= 3 &&
+                       !single &&
+                       len(ya) > 0 && 2 * k + 3 < N &&
+                       std::max(ctx->tune.k5min, (len(order[2 * k + 3]) + 63) / 64) == K &&
+                       score5_stacked_supported(2 * K);
+    if (stack) {
+      int from = 2 * k;
+      if (!ctx->symmetric) {
+        add_cols(2 * k, 2 * k, 2 * k, -1);                 // (ya, ya)
+        add_cols(2 * k + 1, 2 * k + 2, 2 * k, 2 * k + 1);  // both rows, columns before the next pair's yb
+        add_cols(2 * k + 2, 2 * k + 2, 2 * k + 2, -1);     // the next pair's (ya, ya)
+        from = 2 * k + 3;
+      }
+      // row position 2k + r against column position t counts (and, past the
+      // 16-bit bound, goes to the int32 kernel) when t >= 2k + r: the other
+      // (row, column) meetings are the duplicates
+      std::vector<int> elig;
+      for (int t = from; t < N; ++t) {
+        const int x = order[t];
+        const int n = len(x);
+        if (n == 0 || pk16_bound(ctx, R, n) > 32767) {
+          for (int r = 0; r < 4 && 2 * k + r <= t; ++r) {
+            fb_i.push_back(x);
+            fb_j.push_back(order[2 * k + r]);
+          }
+        } else {
+          elig.push_back(t);
+        }
+      }
+      auto rows_at = [&](int t, int64_t* nr) {
+        int64_t s = 0;
+        int c = 0;
+        for (int r = 0; r < 4 && 2 * k + r <= t; ++r, ++c) s += len(order[2 * k + r]);
+        *nr = c;
+        return s;
+      };
+      for (size_t c0 = 0; c0 < elig.size();) {
+        size_t c1 = c0 + 1;
+        while (c1 < elig.size() && (int)(c1 - c0) < C && elig[c1] == elig[c1 - 1] + 1) ++c1;
+        Item3 it;
+        it.ya = 2 * k;
+        it.yb = 2 * k + 1;
+        it.xbegin = elig[c0];
+        it.xcount = (int)(c1 - c0);
+        int64_t cost = 0, cells = 0, npairs = 0;
+        for (size_t q = c0; q < c1; ++q) {
+          int64_t nr;
+          const int64_t rl = rows_at(elig[q], &nr);
+          cost += (int64_t)(len(order[elig[q]]) + 63) * 64 * 2 * K;
+          cells += (int64_t)len(order[elig[q]]) * rl;
+          npairs += nr;
+        }
+        It3 e{it, 2 * K, 1, cost, cells, npairs};
+        e.p2 = make_int2(2 * k + 2, 2 * k + 3);
+        v5.push_back(e);
+        c0 = c1;
+      }
+      ++k;  // the next pair is done
+      continue;
+    }
     if (single) {
       add_cols(2 * k, N - 1, 2 * k, -1);
     } else if (len(ya) == 0) {
@@ -275,6 +342,7 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
   by_class(v2);
   std::stable_sort(v5.begin(), v5.end(), [](const It3& a, const It3& b) {
     if (a.W != b.W) return a.W < b.W;
+    if ((a.p2.x >= 0) != (b.p2.x >= 0)) return a.p2.x < 0;
     if (a.K != b.K) return a.K < b.K;
     return a.cost > b.cost;
   });
@@ -295,9 +363,12 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
   }
   for (size_t q = 0; q < v5.size(); ++q) {
     P.items5.push_back(v5[q].it);
-    if (P.c5.empty() || P.c5.back().K != v5[q].K || P.c5.back().W != v5[q].W) {
+    P.pair2_5.push_back(v5[q].p2);
+    const bool st = v5[q].p2.x >= 0;
+    if (P.c5.empty() || P.c5.back().K != v5[q].K || P.c5.back().W != v5[q].W || P.c5.back().stacked != st) {
       BClass c{v5[q].K, (int64_t)q, (int64_t)q + 1};
       c.W = v5[q].W;
+      c.stacked = st;
       P.c5.push_back(c);
     } else {
       P.c5.back().end = q + 1;
@@ -370,6 +441,8 @@ int upload_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P) {
   if (!P.c5.empty()) {
     CK(hipMalloc(&P.d_items5, sizeof(Item3) * P.items5.size()));
     CK(hipMemcpy(P.d_items5, P.items5.data(), sizeof(Item3) * P.items5.size(), hipMemcpyHostToDevice));
+    CK(hipMalloc(&P.d_pair2, sizeof(int2) * P.pair2_5.size()));
+    CK(hipMemcpy(P.d_pair2, P.pair2_5.data(), sizeof(int2) * P.pair2_5.size(), hipMemcpyHostToDevice));
   }
   if (streamed) {
     const int N = s.n;
@@ -514,7 +587,8 @@ int enqueue_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P, int32
     } else {
       const int per = ctx->tune.wg5 > 0 ? ctx->tune.wg5 : score5_groups_per_cu(c.K, c.W);
       const int64_t g = std::min<int64_t>((int64_t)ctx->num_cus * per, c.end - c.begin);
-      launch_score5(c.K, c.W, a, (int)std::max<int64_t>(g, 1), st);
+      if (c.stacked) a.pair2 = P.d_pair2;
+      launch_score5(c.K, c.W, a, (int)std::max<int64_t>(g, 1), st, c.stacked);
     }
     CK(hipGetLastError());
     if (timed) CK(hipEventRecord(ctx->launch_events[ev++], st));
@@ -599,7 +673,7 @@ int run_batched(mst_ctx* ctx, int id, int32_t* d_out, bool timed) {
     for (const BClass& c : P.c6)
       if ((rc = rec(-2000 - c.K, c.pairs, c.cells))) return rc;
     for (const BClass& c : P.c5)
-      if ((rc = rec(-(1000 + 100 * c.W + c.K), c.pairs, c.cells))) return rc;
+      if ((rc = rec(c.stacked ? -(3000 + c.K) : -(1000 + 100 * c.W + c.K), c.pairs, c.cells))) return rc;
     for (const BClass& c : P.c2)
       if ((rc = rec(-100 - c.K, c.pairs, c.cells))) return rc;
     for (const Bucket& b : ctx->fbplans[id].buckets)
@@ -637,7 +711,7 @@ int run_batched(mst_ctx* ctx, int id, int32_t* d_out, bool timed) {
 void free_batched(BatchedPlan& P) {
   if (P.graph) hipGraphExecDestroy(P.graph);
   void* ps[] = {P.d_items3, P.d_items2, P.d_xlist, P.d_counters, P.d_items5,
-                P.d_scodes, P.d_soff, P.d_items6, P.d_carry};
+                P.d_scodes, P.d_soff, P.d_items6, P.d_carry, P.d_pair2};
   for (void* p : ps)
     if (p) hipFree(p);
   P = BatchedPlan();
@@ -657,6 +731,7 @@ static int* tune_slot(mst_ctx* ctx, const char* key) {
   if (!strcmp(key, "wmax")) return &ctx->tune.wmax;
   if (!strcmp(key, "k6")) return &ctx->tune.k6;
   if (!strcmp(key, "chunk6")) return &ctx->tune.chunk6;
+  if (!strcmp(key, "stack")) return &ctx->tune.stack;
   if (!strcmp(key, "wg2")) return &ctx->tune.wg_per_cu2;
   if (!strncmp(key, "wg3.", 4)) {
     const int k = atoi(key + 4);
@@ -675,6 +750,8 @@ extern "C" int mst_tune_set(mst_ctx* ctx, const char* key, int value) {
     return ctx->fail(MST_E_ARG, "mst_tune_set", "shared3 is 0 or 1");
   if (!strcmp(key, "k5min") && (value < 1 || value > 3))
     return ctx->fail(MST_E_ARG, "mst_tune_set", "k5min out of range 1..3");
+  if (!strcmp(key, "stack") && value != 0 && value != 1)
+    return ctx->fail(MST_E_ARG, "mst_tune_set", "stack is 0 or 1");
   if (!strcmp(key, "chunk6") && (value < 0 || value > 4096))
     return ctx->fail(MST_E_ARG, "mst_tune_set", "chunk6 out of range 0..4096");
   if (!strcmp(key, "chunk_w3") && (value < 0 || value > 4096))

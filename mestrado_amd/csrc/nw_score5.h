@@ -27,6 +27,9 @@ struct Score5Args {
   int go, ge;
   int2* carry;             // score6: per-workgroup row buffer between sweeps
   int64_t carry_cap;       // its columns per workgroup
+  // stacked launches: the second row pair (positions) of each item, on lanes
+  // 32..63 of the strip (the first on lanes 0..31); null otherwise
+  const int2* pair2 = nullptr;
 };
 
 // score5: rows up to 1024 in W strips of 64*K rows, one wave per strip.
@@ -34,7 +37,9 @@ struct Score5Args {
 bool score5_supported(int K, int W);
 int score5_block_threads(int W);
 int score5_groups_per_cu(int K, int W);
-void launch_score5(int K, int W, const Score5Args& a, int grid, hipStream_t st);
+void launch_score5(int K, int W, const Score5Args& a, int grid, hipStream_t st, bool stacked = false);
+// stacked one-wave launches exist for K = 2, 4, 6 (two row pairs of up to 32*K rows)
+bool score5_stacked_supported(int K);
 
 // score6: any number of strips of 64*K rows (K = 3 or 4), four waves sweeping
 // the stream once per four strips; the last strip of a sweep hands its row to

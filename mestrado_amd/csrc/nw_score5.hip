@@ -43,8 +43,12 @@ struct Wide {
   static constexpr bool v = W == 1 && K <= kWideMaxK;
 };
 
-template <int K, int W>
+// ST (stacked, one wave only): an item holds two row pairs of up to 32*K rows
+// each, the first on lanes 0..31, the second (a.pair2) on lanes 32..63; lane 32
+// starts from the row -1 boundary instead of lane 31's last row.
+template <int K, int W, bool ST = false>
 __global__ void __launch_bounds__(MST_WAVE * W * Pipes<W>::P) score5_kernel(Score5Args a) {
+  static_assert(!ST || W == 1, "stacked row pairs take one wave");
   constexpr int P = Pipes<W>::P;
   constexpr int NW = W * P;
   constexpr bool WIDE = Wide<K, W>::v;
@@ -73,9 +77,26 @@ __global__ void __launch_bounds__(MST_WAVE * W * Pipes<W>::P) score5_kernel(Scor
     if (it >= a.last) break;
     const Item3 item = a.items[it];
     Stream S = item_stream(item, it, a);
+    int rs = 0;  // first row of the lane's row pair in the strip
+    if (ST) {
+      Item3 i2 = item;
+      const int2 q = a.pair2[it];
+      i2.ya = q.x;
+      i2.yb = q.y;
+      const Stream S2 = item_stream(i2, it, a);
+      if (lane >= MST_WAVE / 2) {
+        S.ya = S2.ya;
+        S.ma = S2.ma;
+        S.yb = S2.yb;
+        S.mb = S2.mb;
+        S.oya = S2.oya;
+        S.oyb = S2.oyb;
+        rs = (MST_WAVE / 2) * K;
+      }
+    }
     // profile of every strip: wave wv fills residues wv, wv+NW, ...
     for (int js = 0; js < W; ++js)
-      build_strip_profile<K, WIDE>(prof + js * STRIP, S_lds, S, a, js * MST_WAVE * K, wv, NW, bonus);
+      build_strip_profile<K, WIDE>(prof + js * STRIP, S_lds, S, a, js * MST_WAVE * K - rs, wv, NW, bonus);
     if (threadIdx.x < NW) {
       s_prog[threadIdx.x] = 0;
       s_cons[threadIdx.x] = 0;
@@ -98,12 +119,12 @@ __global__ void __launch_bounds__(MST_WAVE * W * Pipes<W>::P) score5_kernel(Scor
     if (kb < ke) {
       S.base = a.soff[kb];
       S.ncol = (int)(a.soff[ke] - S.base);
-      const int r0 = j * MST_WAVE * K;
+      const int r0 = j * MST_WAVE * K - rs;
       const Link in{ring_in, 0, RING5 - 1}, out{ring_out, 0, RING5 - 1};
       int* prog = s_prog + p * W;
       int* cons = s_cons + p * W;
       if (W == 1) {
-        sweep5<K, true, false, WIDE>(kb, r0, S, prof_lane, G2, in, 0, nullptr, nullptr, out, 0, nullptr,
+        sweep5<K, true, false, WIDE, ST>(kb, r0, S, prof_lane, G2, in, 0, nullptr, nullptr, out, 0, nullptr,
                                nullptr, 0, 1, a);
       } else if (j == 0) {
         sweep5<K, true, true, WIDE>(kb, r0, S, prof_lane, G2, in, 0, nullptr, nullptr, out, 0, prog + j,
@@ -146,7 +167,19 @@ int score5_groups_per_cu(int K, int W) {
   return g < 1 ? 1 : g;
 }
 
-void launch_score5(int K, int W, const Score5Args& a, int grid, hipStream_t st) {
+bool score5_stacked_supported(int K) { return K == 2 || K == 4 || K == 6; }
+
+void launch_score5(int K, int W, const Score5Args& a, int grid, hipStream_t st, bool stacked) {
+  if (stacked) {
+    if (W != 1) return;
+    if (K == 2)
+      hipLaunchKernelGGL((score5_kernel<2, 1, true>), dim3(grid), dim3(score5_block_threads(1)), 0, st, a);
+    else if (K == 4)
+      hipLaunchKernelGGL((score5_kernel<4, 1, true>), dim3(grid), dim3(score5_block_threads(1)), 0, st, a);
+    else if (K == 6)
+      hipLaunchKernelGGL((score5_kernel<6, 1, true>), dim3(grid), dim3(score5_block_threads(1)), 0, st, a);
+    return;
+  }
 #define MST_CASE(k, w)                                                                          \
   if (K == k && W == w) {                                                                       \
     hipLaunchKernelGGL((score5_kernel<k, w>), dim3(grid), dim3(score5_block_threads(w)), 0, st, \

@@ -87,7 +87,7 @@ __device__ __forceinline__ void emit(const Lane2<K>& L, const Own& ow, int kx, c
 // One step t of the wave: lane l computes column c = t - l of its K rows.
 // w: the lane's profile dwords for the column's residue; bv: for lane 0, F and
 // H of the row above at column t (the other lanes take them from lane l-1).
-template <int K, bool PRED, bool CHECK, bool FIRSTW, bool STORE, bool WIDE>
+template <int K, bool PRED, bool CHECK, bool FIRSTW, bool STORE, bool WIDE, bool STACK>
 __device__ __forceinline__ void step5(Lane2<K>& L, Cursor& cu, const Own& ow, int t,
                                       const uint32_t* w, int2 bv, const Stream& S, s2 G2,
                                       const Link& out, const Score5Args& a) {
@@ -99,6 +99,11 @@ __device__ __forceinline__ void step5(Lane2<K>& L, Cursor& cu, const Own& ow, in
   } else {
     fin = shr1o((uint32_t)bv.x, L.Fout);
     hup = shr1o((uint32_t)bv.y, L.Hout);
+  }
+  if (STACK && lane == MST_WAVE / 2) {
+    // first row of the second row pair: the row -1 boundary (0 when biased)
+    fin = 0;
+    hup = 0;
   }
   const int c = t - lane;
   const bool act = !PRED || (c >= 0 && c < S.ncol);
@@ -178,7 +183,7 @@ __device__ __forceinline__ int code_at(uint2 v, int j) {
 // 8-byte load per lane), reads the 8 steps' profile dwords from LDS before the
 // first step, and reloads each step's `in` word for the next block right after
 // the step has used it, so no register moves between blocks.
-template <int K, bool FIRSTW, bool STORE, bool WIDE = false>
+template <int K, bool FIRSTW, bool STORE, bool WIDE = false, bool STACK = false>
 __device__ __forceinline__ void sweep5(int kb, int r0, const Stream& S, const uint8_t* prof_lane,
                                        s2 G2, const Link& in, int gin, const int* prog_in,
                                        int* cons_out, const Link& out, int gout, int* prog_out,
@@ -236,7 +241,7 @@ __device__ __forceinline__ void sweep5(int kb, int r0, const Stream& S, const ui
 #define MST_STEPS(PRED, CHECK)                                                            \
   _Pragma("unroll") for (int j = 0; j < U5; ++j) {                                        \
     if (!PRED || tb + j < T)                                                              \
-      step5<K, PRED, CHECK, FIRSTW, STORE, WIDE>(L, cu, ow, tb + j, w[j], bv[j], S, G2, out, a); \
+      step5<K, PRED, CHECK, FIRSTW, STORE, WIDE, STACK>(L, cu, ow, tb + j, w[j], bv[j], S, G2, out, a); \
     if (!FIRSTW) bv[j] = in.p[(in.base + min(tb + U5 + j, ncol - 1)) & in.mask];           \
   }
     if (full && !cross) {

@@ -136,21 +136,25 @@ def test_batched_tunables_keep_results(ctx, shared3, chunk, chunk_w3, branches):
 
 @pytest.mark.parametrize("k5min", [1, 2])
 def test_batched_one_wave_small_k(ctx, k5min):
-    # short row pairs with 1 or 2 rows per lane (64 or 128 rows per wave)
+    # short row pairs with 1 or 2 rows per lane (64 or 128 rows per wave),
+    # unstacked and stacked
     rng = np.random.default_rng(30 + k5min)
     lens = list(rng.integers(1, 200, 60)) + [1, 2, 63, 64, 65, 127, 128, 129, 300, 700]
     seqs = [_rand(rng, int(L)) for L in lens]
     o, _ = O.all_pairs(seqs, S)
     st = ctx.upload(seqs)
-    saved = ctx.tune_get("k5min")
+    saved = ctx.tune_get("k5min"), ctx.tune_get("stack")
     try:
         ctx.tune_set("k5min", k5min)
-        g = st.all_pairs(M.FORM_BATCHED)
-        assert (g == o).all()
-        classes = {k for k, _, _, _ in st.launch_stats(M.FORM_BATCHED)}
-        assert -(1000 + 100 + k5min) in classes
+        for stack in (0, 1):
+            ctx.tune_set("stack", stack)
+            g = st.all_pairs(M.FORM_BATCHED)
+            assert (g == o).all()
+            classes = {k for k, _, _, _ in st.launch_stats(M.FORM_BATCHED)}
+            assert (-3004 if stack and k5min == 2 else -(1000 + 100 + k5min)) in classes
     finally:
-        ctx.tune_set("k5min", saved)
+        ctx.tune_set("k5min", saved[0])
+        ctx.tune_set("stack", saved[1])
         st.free()
     with pytest.raises(M.MestradoError):
         ctx.tune_set("k5min", 0)
@@ -226,6 +230,34 @@ def test_batched_multi_sweep_rows(ctx, seed, n, lo, hi, k6, chunk6):
     finally:
         ctx.tune_set("k6", saved[0])
         ctx.tune_set("chunk6", saved[1])
+        st.free()
+
+
+@pytest.mark.parametrize("stack", [0, 1])
+def test_batched_stacked_row_pairs(ctx, stack):
+    # row pairs of 65..192 rows: consecutive pairs of the same class share a
+    # strip (lanes 0..31 and 32..63), with a symmetric matrix over every
+    # column from the first pair's ya on (the pairs the two row pairs meet
+    # twice get the same score twice); the same matrix with stacking off
+    rng = np.random.default_rng(70)
+    lens = list(rng.integers(1, 200, 90)) + [1, 2, 31, 32, 33, 63, 64, 65, 96, 97, 128, 129, 191, 192, 193, 400]
+    seqs = [_rand(rng, int(L)) for L in lens]
+    o, _ = O.all_pairs(seqs, S)
+    st = ctx.upload(seqs)
+    saved = ctx.tune_get("stack")
+    try:
+        ctx.tune_set("stack", stack)
+        g = st.all_pairs(M.FORM_BATCHED)
+        bad = np.argwhere(g != o)
+        assert len(bad) == 0, [(lens[i], lens[j], g[i, j], o[i, j]) for i, j in bad[:10]]
+        assert (st.all_pairs(M.FORM_BATCHED) == o).all()  # graph replay
+        kinds = {r[0] for r in st.launch_stats(M.FORM_BATCHED)}
+        stacked = {k for k in kinds if k <= -3000}
+        assert stacked == ({-3004, -3006} if stack else set())
+        L = np.array(lens, dtype=np.int64)
+        assert sum(r[2] for r in st.launch_stats(M.FORM_BATCHED)) == (L.sum() ** 2 + (L * L).sum()) // 2
+    finally:
+        ctx.tune_set("stack", saved)
         st.free()
 
 
