@@ -83,9 +83,10 @@ def apply_for(ctx, lengths):
 
 def candidates():
     # shared3 = 1 and k5min = 3 lost by 10-20 and 5-8 ms on the 4,000-protein
-    # set in a wider search; the search keeps the knobs that stayed close
-    for chunk, chunk_w3, k1max in itertools.product((16, 32), (0, 8), (8, 10, 12, 13)):
-        yield {"shared3": 0, "chunk": chunk, "chunk_w3": chunk_w3, "wg5": 0, "k5min": 1, "k1max": k1max}
+    # set in a wider search, k1max 14-16 and items of 72 or more sequences by
+    # 3-70 ms; the search keeps the knobs that stayed close
+    for chunk, chunk6, k1max in itertools.product((32, 48, 64), (2, 4), (12, 13)):
+        yield {"shared3": 0, "chunk": chunk, "chunk6": chunk6, "wg5": 0, "k5min": 1, "k1max": k1max}
 
 
 def tune(ctx, seqs, reps=2, log=None):

@@ -33,6 +33,8 @@ struct Tune {
                              // longer rows take the multi-sweep kernel
   int k6 = 0;                // rows per lane of the multi-sweep kernel: 3, 4, or 0 (per row pair)
   int chunk6 = 0;            // column sequences per item of the multi-sweep kernel (0: chunk)
+  int chunk_k9 = 0;          // column sequences per item of one- and two-wave classes of 9+ rows
+                             // per lane in all (0: chunk)
   int stack = 0;             // 1: two row pairs of K = 2, 3 rows per lane share a strip of 2K
 };
 

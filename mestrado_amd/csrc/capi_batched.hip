@@ -206,6 +206,8 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
         // pipelines of three or four waves take their own item size
         int Cw = (kind == 5 && W >= 3 && ctx->tune.chunk_w3 > 0) ? ctx->tune.chunk_w3 : C;
         if (kind == 6 && ctx->tune.chunk6 > 0) Cw = ctx->tune.chunk6;
+        // the classes of many rows per lane have few row pairs: smaller items
+        if (kind == 5 && W <= 2 && W * K >= 9 && ctx->tune.chunk_k9 > 0) Cw = ctx->tune.chunk_k9;
         size_t c0 = 0;
         while (c0 < elig.size()) {
           // a chunk is a run of consecutive positions; score6 keeps its column
@@ -732,6 +734,7 @@ static int* tune_slot(mst_ctx* ctx, const char* key) {
   if (!strcmp(key, "k6")) return &ctx->tune.k6;
   if (!strcmp(key, "chunk6")) return &ctx->tune.chunk6;
   if (!strcmp(key, "stack")) return &ctx->tune.stack;
+  if (!strcmp(key, "chunk_k9")) return &ctx->tune.chunk_k9;
   if (!strcmp(key, "wg2")) return &ctx->tune.wg_per_cu2;
   if (!strncmp(key, "wg3.", 4)) {
     const int k = atoi(key + 4);
@@ -752,6 +755,8 @@ extern "C" int mst_tune_set(mst_ctx* ctx, const char* key, int value) {
     return ctx->fail(MST_E_ARG, "mst_tune_set", "k5min out of range 1..3");
   if (!strcmp(key, "stack") && value != 0 && value != 1)
     return ctx->fail(MST_E_ARG, "mst_tune_set", "stack is 0 or 1");
+  if (!strcmp(key, "chunk_k9") && (value < 0 || value > 4096))
+    return ctx->fail(MST_E_ARG, "mst_tune_set", "chunk_k9 out of range 0..4096");
   if (!strcmp(key, "chunk6") && (value < 0 || value > 4096))
     return ctx->fail(MST_E_ARG, "mst_tune_set", "chunk6 out of range 0..4096");
   if (!strcmp(key, "chunk_w3") && (value < 0 || value > 4096))
@@ -767,7 +772,7 @@ extern "C" int mst_tune_set(mst_ctx* ctx, const char* key, int value) {
   if (!strcmp(key, "k1max") && (value < 5 || value > 16))
     return ctx->fail(MST_E_ARG, "mst_tune_set", "k1max out of range 5..16");
   if (strcmp(key, "chunk") && strcmp(key, "shared3") && strcmp(key, "chunk_w3") && strcmp(key, "chunk6") &&
-      (value < 0 || value > 64))
+      strcmp(key, "chunk_k9") && (value < 0 || value > 64))
     return ctx->fail(MST_E_ARG, "mst_tune_set", "workgroups per CU out of range 0..64");
   if (*slot == value) return MST_OK;
   hipSetDevice(ctx->device);
