@@ -210,7 +210,7 @@ struct mst_server {
   double slack = 2.0;          // deadline = submission + slack x estimated run time
   double deadline_add = 0.0;   //   + this many seconds
   double t_retire = 0;         // when the last round was seen finished
-  int chunk = 8;
+  int chunk = 12;
   int64_t done_count = 0, cells_done = 0;
   std::vector<int64_t> completed;  // finished since the last mst_server_completed
   // NJ: jobs whose scores are done; batches in flight (one per slot)
