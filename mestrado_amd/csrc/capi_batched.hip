@@ -4,13 +4,15 @@
 // sequences are paired consecutively (Ya = s[2k], Yb = s[2k+1]) and run against
 // every column sequence at or after s[2k+1] in that order, so each pair i<=j is
 // scored exactly once with the shorter sequence in the rows. A row pair's
-// column list is cut into chunks: one chunk is one work item of the shared-
-// profile kernel (score3), in the rows-per-lane class K = ceil(rows/64), for
-// the classes enabled below. Pairs whose 16-bit bound fails, and rows outside
-// the enabled classes, go to the int32 kernel (the per-wave multi-strip 16-bit
-// kernel, score2, is switched off). Items are ordered longest first inside every launch; the
-// packed launches are captured once into a hipGraph and replayed, and the
-// int32 launches follow the replay on the same stream.
+// column list is cut into chunks of consecutive positions: one chunk is one
+// work item, swept as one column stream by the kernel of the row pair's class
+// (the routes are listed below: one-wave, two-wave and wider pipelines of the
+// streamed kernel, the multi-sweep kernel, optionally stacked row pairs).
+// Pairs whose 16-bit bound fails, and rows outside the enabled classes, go to
+// the int32 kernel (the per-wave multi-strip 16-bit kernel, score2, is
+// switched off). Items are ordered longest first inside every launch; the
+// packed launches are captured once into a hipGraph (in parallel branches)
+// and replayed, and the int32 launches follow the replay on the same stream.
 #include <algorithm>
 #include <cstring>
 #include <numeric>
