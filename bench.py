@@ -307,7 +307,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the all-pairs path runs on a single MI355X "
-              "(how it would shard across GPUs is described in DESIGN.md); use --gpus 1",
+              "(sharding across GPUs is a design note, not built); use --gpus 1",
               file=sys.stderr)
         return 2
 

@@ -51,7 +51,8 @@ def test_synth_is_deterministic_and_shaped():
 
 def test_create_without_device_fails_cleanly():
     # In a container without a GPU the context cannot be created; the call must
-    # return an error code, not crash. On a GPU box this test is skipped.
+    # return an error code, not crash. With a device present the call succeeds
+    # and the test is skipped.
     L = M.lib()
     h = C.c_void_p()
     rc = L.mst_create(10, 2, None, 0, C.byref(h))
