@@ -79,12 +79,13 @@ double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// Classes of the streamed kernels: 0 = score6; then score5<K, 1> for K =
+// Classes of the streamed kernels: 0 = score6<4>; then score5<K, 1> for K =
 // 1..16, score5<K, 2> for K = 3..8, score5<K, 3> and score5<K, 4> for K = 3..4
-// (the routes of the batched form's planner).
-constexpr int kClasses = 1 + 16 + 6 + 2 + 2;
+// (the routes of the batched form's planner); last, score6<3>.
+constexpr int kClasses = 1 + 16 + 6 + 2 + 2 + 1;
+constexpr int kClassScore6K3 = kClasses - 1;
 int class_of(int kind, int K, int W) {
-  if (kind == 6) return 0;
+  if (kind == 6) return K == 3 ? kClassScore6K3 : 0;
   if (W == 1) return K;
   if (W == 2) return 17 + (K - 3);
   return 23 + (W - 3) * 2 + (K - 3);
@@ -211,6 +212,7 @@ struct mst_server {
   double deadline_add = 0.0;   //   + this many seconds
   double t_retire = 0;         // when the last round was seen finished
   int chunk = 12;
+  int chunk6 = 0;  // items of the multi-sweep kernel (0: chunk)
   int64_t done_count = 0, cells_done = 0;
   std::vector<int64_t> completed;  // finished since the last mst_server_completed
   // NJ: jobs whose scores are done; batches in flight (one per slot)
@@ -303,7 +305,7 @@ int64_t job_footprint(const Job& J) { return job_layout(J).total; }
 // Streamed plan of one job whose sequences sit at positions pos0 .. pos0+n-1
 // in length order (lens ascending): the row pairs (2k, 2k+1) against every
 // later position, cut into runs of `chunk` column sequences.
-void plan_job(const mst_ctx* ctx, Job& J, const std::vector<int>& lens, int chunk) {
+void plan_job(const mst_ctx* ctx, Job& J, const std::vector<int>& lens, int chunk, int chunk6) {
   const int N = J.n;
   const int p0 = (int)J.pos0;
   auto add = [&](int kind, int K, int W, int64_t R, int pa, int pb, int first, int last) {
@@ -322,7 +324,7 @@ void plan_job(const mst_ctx* ctx, Job& J, const std::vector<int>& lens, int chun
     while (c0 < elig.size()) {
       size_t c1 = c0 + 1;
       int64_t cols = lens[elig[c0]];
-      while (c1 < elig.size() && (int)(c1 - c0) < chunk && elig[c1] == elig[c1 - 1] + 1 &&
+      while (c1 < elig.size() && (int)(c1 - c0) < (kind == 6 ? chunk6 : chunk) && elig[c1] == elig[c1 - 1] + 1 &&
              (kind != 6 || cols + lens[elig[c1]] <= 16384)) {
         cols += lens[elig[c1]];
         ++c1;
@@ -358,10 +360,16 @@ void plan_job(const mst_ctx* ctx, Job& J, const std::vector<int>& lens, int chun
       K = (m + 64 * W - 1) / (64 * W);
       kind = 5;
     } else {
+      // the multi-sweep kernel in strips of 3 or 4 rows per lane, chosen as
+      // the batched form's planner does
       kind = 6;
-      K = kScore6K;
+      K = ctx->tune.k6;
+      if (K == 0) {
+        auto cap = [&](int k) { return (double)((m + 256 * k - 1) / (256 * k)) * 256 * k; };
+        K = cap(3) / 0.89 < cap(4) ? 3 : 4;
+      }
     }
-    const int64_t R = kind == 6 ? (int64_t)(m + 255) / 256 * 256 : (int64_t)W * 64 * K;
+    const int64_t R = kind == 6 ? (int64_t)(m + 64 * K - 1) / (64 * K) * 64 * K : (int64_t)W * 64 * K;
     if (single) {
       add(kind, K, W, R, pa, -1, pa, N - 1);
     } else if (lens[pa] == 0) {
@@ -463,7 +471,7 @@ int admit(mst_server* S, Job& J) {
   launch_encode((const char*)(S->d_arena + o_ascii), (const int64_t*)(S->d_arena + o_roff), N,
                 (int8_t*)(S->d_arena + o_codes), ctx->stream);
   CK(hipGetLastError());
-  plan_job(ctx, J, lens, S->chunk);
+  plan_job(ctx, J, lens, S->chunk, S->chunk6 > 0 ? S->chunk6 : S->chunk);
   J.roff.assign(roff.begin(), roff.end());
   J.where.assign(N, 0);
   for (int i = 0; i < N; ++i) J.where[order[i]] = i;
@@ -604,7 +612,10 @@ int launch_round(mst_server* S, int* launched) {
   int nside = 0;
   for (int c = 0; c < kClasses; ++c) {
     if (ranges[c].second == ranges[c].first) continue;
-    const int si = nside++ % 4;
+    // the multi-sweep classes keep a side stream each (0 for K = 4, 1 for
+    // K = 3): their row buffers are per class, so launches of one class, of
+    // this round and the rounds still in flight, must not overlap
+    const int si = c == 0 ? 0 : (c == kClassScore6K3 ? 1 : nside++ % 4);
     hipStream_t st = S->side[si];
     if (!used[si]) {
       CK(hipStreamWaitEvent(st, B.ev_go, 0));
@@ -626,11 +637,11 @@ int launch_round(mst_server* S, int* launched) {
     a.subst = ctx->d_S;
     a.go = ctx->go;
     a.ge = ctx->ge;
-    a.carry = S->d_carry;
+    a.carry = S->d_carry + (c == kClassScore6K3 ? (int64_t)S->grid6 * 16384 : 0);
     a.carry_cap = 16384;
     const int64_t nit = ranges[c].second - ranges[c].first;
-    if (c == 0) {
-      launch_score6(kScore6K, a, (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)(S->grid6 * share), nit)), st);
+    if (c == 0 || c == kClassScore6K3) {
+      launch_score6(c == 0 ? 4 : 3, a, (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)(S->grid6 * share), nit)), st);
     } else {
       int W, K;
       class_shape(c, &K, &W);
@@ -1089,7 +1100,7 @@ extern "C" int mst_server_create(mst_ctx* ctx, int64_t budget_bytes, mst_server*
   if ((e = hipMalloc(&S->d_job_out, sizeof(int64_t) * max_slots)) != hipSuccess) return fail("jobs", e);
   if ((e = hipMalloc(&S->d_job_ld, sizeof(int) * max_slots)) != hipSuccess) return fail("jobs", e);
   S->grid6 = ctx->num_cus * score6_groups_per_cu();
-  if ((e = hipMalloc(&S->d_carry, sizeof(int2) * (size_t)S->grid6 * 16384)) != hipSuccess)
+  if ((e = hipMalloc(&S->d_carry, sizeof(int2) * (size_t)S->grid6 * 16384 * 2)) != hipSuccess)
     return fail("carry", e);
   S->grid32 = ctx->num_cus * 4;
   S->bufs_cap_cols = 1 << 15;  // int32 kernel: pairs up to 32768 residues
@@ -1378,6 +1389,10 @@ extern "C" int mst_server_set(mst_server* S, const char* key, double value) {
   if (!S || !key) return MST_E_ARG;
   if (!strcmp(key, "round_cells") && value >= 1e6 && value <= 1e14) {
     S->round_cells = value;
+    return MST_OK;
+  }
+  if (!strcmp(key, "chunk6") && value >= 0 && value <= 4096) {
+    S->chunk6 = (int)value;
     return MST_OK;
   }
   if (!strcmp(key, "chunk") && value >= 1 && value <= 4096) {
