@@ -212,7 +212,7 @@ struct mst_server {
   double deadline_add = 0.0;   //   + this many seconds
   double t_retire = 0;         // when the last round was seen finished
   int chunk = 12;
-  int chunk6 = 0;  // items of the multi-sweep kernel (0: chunk)
+  int chunk6 = 8;  // items of the multi-sweep kernel (0: chunk)
   int64_t done_count = 0, cells_done = 0;
   std::vector<int64_t> completed;  // finished since the last mst_server_completed
   // NJ: jobs whose scores are done; batches in flight (one per slot)
