@@ -106,7 +106,8 @@ int mst_last_launch_stats(mst_ctx* ctx, int* rows_per_lane, int64_t* pairs, int6
  * sequences per work item of the multi-sweep kernel, 0 = "chunk"; default
  * 0), "chunk_k9" (column sequences per work item of the one- and two-wave
  * classes of 9 or more rows per lane in all, 0 = "chunk"; default 0),
- * "stack" (1: consecutive row pairs of the one-wave classes of 2 and 3
+ * "s6p2" (1: the multi-sweep classes as two pipelines per workgroup sharing
+ * the strip profiles, in step per sweep; default 0), "stack" (1: consecutive row pairs of the one-wave classes of 2 and 3
  * rows per lane share a strip of 4 or 6 rows per lane, one pair on each half
  * of the wave; default 0), "wg5", "wg2" and
  * "wg3.<K>" (workgroups per CU, 0 = default). Setting a value drops the cached

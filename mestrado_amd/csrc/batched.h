@@ -35,6 +35,7 @@ struct Tune {
   int chunk6 = 0;            // column sequences per item of the multi-sweep kernel (0: chunk)
   int chunk_k9 = 0;          // column sequences per item of one- and two-wave classes of 9+ rows
                              // per lane in all (0: chunk)
+  int s6p2 = 0;              // 1: the multi-sweep classes as two pipelines per workgroup (score7)
   int stack = 0;             // 1: two row pairs of K = 2, 3 rows per lane share a strip of 2K
 };
 

@@ -435,12 +435,14 @@ int upload_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P) {
   if (!P.c6.empty()) {
     CK(hipMalloc(&P.d_items6, sizeof(Item3) * P.items6.size()));
     CK(hipMemcpy(P.d_items6, P.items6.data(), sizeof(Item3) * P.items6.size(), hipMemcpyHostToDevice));
+    const bool two = ctx->tune.s6p2 != 0;
     P.grid6 = (int)std::max<int64_t>(
-        1, std::min<int64_t>((int64_t)ctx->num_cus * score6_groups_per_cu(), (int64_t)P.items6.size()));
-    // a row buffer per workgroup and per multi-sweep class: the classes may
-    // run at the same time on parallel branches of the graph
-    CK(hipMalloc(&P.d_carry, sizeof(int2) * (size_t)P.grid6 * std::max<int64_t>(P.carry_cols, 1) *
-                                 P.c6.size()));
+        1, std::min<int64_t>((int64_t)ctx->num_cus * (two ? score7_groups_per_cu() : score6_groups_per_cu()),
+                             (int64_t)P.items6.size()));
+    // a row buffer per pipeline, workgroup and multi-sweep class: the classes
+    // may run at the same time on parallel branches of the graph
+    CK(hipMalloc(&P.d_carry, sizeof(int2) * (size_t)P.grid6 * (two ? 2 : 1) *
+                                 std::max<int64_t>(P.carry_cols, 1) * P.c6.size()));
   }
   if (!P.c5.empty()) {
     CK(hipMalloc(&P.d_items5, sizeof(Item3) * P.items5.size()));
@@ -584,10 +586,14 @@ int enqueue_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P, int32
     hipStream_t st = br[where[k]];
     Score5Args a = args5(c, ls[k].six ? P.d_items6 : P.d_items5);
     a.counter = P.d_counters + ls[k].ctr;
-    if (ls[k].six) a.carry += (int64_t)(&c - P.c6.data()) * P.grid6 * a.carry_cap;
+    if (ls[k].six)
+      a.carry += (int64_t)(&c - P.c6.data()) * P.grid6 * (ctx->tune.s6p2 ? 2 : 1) * a.carry_cap;
     if (timed) CK(hipEventRecord(ctx->launch_events[ev++], st));
     if (ls[k].six) {
-      launch_score6(c.K, a, P.grid6, st);
+      if (ctx->tune.s6p2)
+        launch_score7(c.K, a, P.grid6, st);
+      else
+        launch_score6(c.K, a, P.grid6, st);
     } else {
       const int per = ctx->tune.wg5 > 0 ? ctx->tune.wg5 : score5_groups_per_cu(c.K, c.W);
       const int64_t g = std::min<int64_t>((int64_t)ctx->num_cus * per, c.end - c.begin);
@@ -737,6 +743,7 @@ static int* tune_slot(mst_ctx* ctx, const char* key) {
   if (!strcmp(key, "chunk6")) return &ctx->tune.chunk6;
   if (!strcmp(key, "stack")) return &ctx->tune.stack;
   if (!strcmp(key, "chunk_k9")) return &ctx->tune.chunk_k9;
+  if (!strcmp(key, "s6p2")) return &ctx->tune.s6p2;
   if (!strcmp(key, "wg2")) return &ctx->tune.wg_per_cu2;
   if (!strncmp(key, "wg3.", 4)) {
     const int k = atoi(key + 4);
@@ -757,6 +764,8 @@ extern "C" int mst_tune_set(mst_ctx* ctx, const char* key, int value) {
     return ctx->fail(MST_E_ARG, "mst_tune_set", "k5min out of range 1..3");
   if (!strcmp(key, "stack") && value != 0 && value != 1)
     return ctx->fail(MST_E_ARG, "mst_tune_set", "stack is 0 or 1");
+  if (!strcmp(key, "s6p2") && value != 0 && value != 1)
+    return ctx->fail(MST_E_ARG, "mst_tune_set", "s6p2 is 0 or 1");
   if (!strcmp(key, "chunk_k9") && (value < 0 || value > 4096))
     return ctx->fail(MST_E_ARG, "mst_tune_set", "chunk_k9 out of range 0..4096");
   if (!strcmp(key, "chunk6") && (value < 0 || value > 4096))

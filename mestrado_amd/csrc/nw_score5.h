@@ -48,5 +48,10 @@ constexpr int kScore6K = 4;
 int score6_block_threads();
 int score6_groups_per_cu();
 void launch_score6(int K, const Score5Args& a, int grid, hipStream_t st);
+// score7: the multi-sweep kernel as two pipelines per workgroup sharing the
+// strips' profiles, in step per sweep; row buffers: two per workgroup.
+int score7_block_threads();
+int score7_groups_per_cu();
+void launch_score7(int K, const Score5Args& a, int grid, hipStream_t st);
 
 }  // namespace mst

@@ -141,4 +141,146 @@ void launch_score6(int K, const Score5Args& a, int grid, hipStream_t st) {
     hipLaunchKernelGGL(score6_kernel<4>, dim3(grid), dim3(score6_block_threads()), 0, st, a);
 }
 
+// Two pipelines of four waves per workgroup sharing the strips' profiles, in
+// step per sweep: every sweep starts and ends with a workgroup barrier, the
+// four waves of each pipeline sweep their part of the item's columns (split
+// by columns as in score5) like a one-sweep pipeline of four strips, and the
+// last strip of a sweep leaves its row in the pipeline's row buffer (`carry`,
+// two per workgroup), which the next sweep's first strip reads after the
+// barrier. Inside a sweep a wave waits only on its own pipeline's rings, as
+// in score5; waves without a strip in the last sweep still take its barriers.
+// LDS: the four profile slots serve eight waves.
+constexpr int P7 = 2;
+
+template <int K>
+__global__ void __launch_bounds__(MST_WAVE * W6 * P7) score7_kernel(Score5Args a) {
+  constexpr int D = KD<K>::v;
+  constexpr int STRIP = MST_LETTERS * MST_WAVE * D * 4;
+  constexpr int ROWS = MST_WAVE * K;
+  constexpr int NW = W6 * P7;
+  __shared__ int8_t S_lds[MST_LETTERS * MST_LETTERS];
+  __shared__ __attribute__((aligned(16))) uint8_t prof[W6 * STRIP];
+  __shared__ int2 ring[P7 * (W6 - 1) * RING5];
+  __shared__ int s_prog[NW], s_cons[NW];
+  __shared__ int s_full, s_sink;
+  __shared__ int s_item;
+  const int lane = lane_id();
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int p = wv / W6, j = wv % W6;
+  for (int i = threadIdx.x; i < MST_LETTERS * MST_LETTERS; i += MST_WAVE * NW) S_lds[i] = a.subst[i];
+  if (threadIdx.x == 0) {
+    s_full = 1 << 30;  // the row buffer read after a barrier: always complete
+    s_sink = 0;
+  }
+  const int G = a.go - a.ge;
+  const s2 G2 = {(short)G, (short)G};
+  const int bonus = a.go + a.ge;
+  uint8_t* slot = prof + j * STRIP;
+  const uint8_t* prof_lane = slot + lane * D * 4;
+  int2* carry = a.carry + ((int64_t)blockIdx.x * P7 + p) * a.carry_cap;
+  int2* ring_in = j > 0 ? ring + (p * (W6 - 1) + j - 1) * RING5 : nullptr;
+  int2* ring_out = j < W6 - 1 ? ring + (p * (W6 - 1) + j) * RING5 : nullptr;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = (int)atomicAdd(a.counter, 1u);
+    __syncthreads();
+    const int64_t it = (int64_t)s_item + a.first;
+    if (it >= a.last) break;
+    const Item3 item = a.items[it];
+    Stream S = item_stream(item, it, a);
+    const int m = S.ma > S.mb ? S.ma : S.mb;
+    const int ns = (m + ROWS - 1) / ROWS;
+    const int nsweeps = (ns + W6 - 1) / W6;
+    // pipeline p's part of the item: positions [kb, ke), balanced by columns
+    const int xb0 = item.xbegin, xe0 = item.xbegin + item.xcount;
+    const int64_t c0 = a.soff[xb0], total = a.soff[xe0] - c0;
+    int kb = xe0, ke = xe0;
+    {
+      const int64_t lo = total * p / P7, hi = total * (p + 1) / P7;
+      for (int k = xb0; k < xe0; ++k) {
+        const int64_t sk = a.soff[k] - c0;
+        if (kb == xe0 && sk >= lo) kb = k;
+        if (ke == xe0 && sk >= hi && p + 1 < P7) ke = k;
+      }
+    }
+    for (int q = 0; q < nsweeps; ++q) {
+      const int s = j + q * W6;
+      // slot j: strip s, its residues split between the two pipelines' wave j
+      if (s < ns) build_strip_profile<K>(slot, S_lds, S, a, s * ROWS, p, P7, bonus);
+      if (threadIdx.x < NW) {
+        s_prog[threadIdx.x] = 0;
+        s_cons[threadIdx.x] = 0;
+      }
+      __syncthreads();
+      if (s < ns && kb < ke) {
+        S.base = a.soff[kb];
+        S.ncol = (int)(a.soff[ke] - S.base);
+        const bool first = (s == 0), last = (s == ns - 1);
+        Link in, out;
+        const int* prog_in;
+        int* cons_out;
+        if (j > 0) {
+          in = Link{ring_in, 0, RING5 - 1};
+          prog_in = s_prog + p * W6 + j - 1;
+          cons_out = s_cons + p * W6 + j;
+        } else {
+          in = Link{carry, 0, -1};
+          prog_in = &s_full;
+          cons_out = &s_sink;
+        }
+        int* prog_out;
+        const int* cons_in;
+        int span, pub;
+        if (j < W6 - 1) {
+          out = Link{ring_out, 0, RING5 - 1};
+          prog_out = s_prog + p * W6 + j;
+          cons_in = s_cons + p * W6 + j + 1;
+          span = RING5;
+          pub = 1;
+        } else {
+          out = Link{carry, 0, -1};
+          prog_out = &s_sink;
+          cons_in = nullptr;
+          span = 0;
+          pub = 16;
+        }
+        const int r0 = s * ROWS;
+        if (first) {
+          if (last)
+            sweep5<K, true, false>(kb, r0, S, prof_lane, G2, in, 0, prog_in, cons_out, out, 0, prog_out,
+                                   cons_in, span, pub, a);
+          else
+            sweep5<K, true, true>(kb, r0, S, prof_lane, G2, in, 0, prog_in, cons_out, out, 0, prog_out,
+                                  cons_in, span, pub, a);
+        } else {
+          if (last)
+            sweep5<K, false, false>(kb, r0, S, prof_lane, G2, in, 0, prog_in, cons_out, out, 0, prog_out,
+                                    cons_in, span, pub, a);
+          else
+            sweep5<K, false, true>(kb, r0, S, prof_lane, G2, in, 0, prog_in, cons_out, out, 0, prog_out,
+                                   cons_in, span, pub, a);
+        }
+      }
+      __syncthreads();  // slots, rings and row buffers are reused by the next sweep
+    }
+  }
+}
+
+int score7_block_threads() { return MST_WAVE * W6 * P7; }
+
+int score7_groups_per_cu() {
+  const int D = KD<kScore6K>::v;
+  const int lds = W6 * MST_LETTERS * MST_WAVE * D * 4 + P7 * (W6 - 1) * RING5 * 8 + 8 * W6 * P7 + 12 +
+                  MST_LETTERS * MST_LETTERS;
+  int g = (160 * 1024) / (lds + 512);
+  if (g * W6 * P7 > 32) g = 32 / (W6 * P7);
+  return g < 1 ? 1 : g;
+}
+
+void launch_score7(int K, const Score5Args& a, int grid, hipStream_t st) {
+  if (K == 3)
+    hipLaunchKernelGGL(score7_kernel<3>, dim3(grid), dim3(score7_block_threads()), 0, st, a);
+  else
+    hipLaunchKernelGGL(score7_kernel<4>, dim3(grid), dim3(score7_block_threads()), 0, st, a);
+}
+
 }  // namespace mst

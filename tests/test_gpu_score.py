@@ -202,20 +202,25 @@ def test_batched_two_wave_large_k(ctx):
         st.free()
 
 
-@pytest.mark.parametrize("seed,n,lo,hi,k6,chunk6", [(21, 9, 1025, 2100, 0, 0), (22, 14, 800, 1600, 0, 4),
-                                                     (23, 9, 1025, 2300, 3, 1), (24, 9, 1025, 2300, 4, 2)])
-def test_batched_multi_sweep_rows(ctx, seed, n, lo, hi, k6, chunk6):
+@pytest.mark.parametrize("seed,n,lo,hi,k6,chunk6,s6p2",
+                         [(21, 9, 1025, 2100, 0, 0, 0), (22, 14, 800, 1600, 0, 4, 1),
+                          (23, 9, 1025, 2300, 3, 1, 1), (24, 9, 1025, 2300, 4, 2, 0),
+                          (25, 9, 1025, 2300, 0, 8, 1)])
+def test_batched_multi_sweep_rows(ctx, seed, n, lo, hi, k6, chunk6, s6p2):
     # rows beyond 1024: four waves sweep the column stream once per four
-    # strips of 64*K rows (K = 3 or 4), the row between sweeps kept in HBM;
-    # the longest pairs exceed the 16-bit bound and take the int32 kernel
+    # strips of 64*K rows (K = 3 or 4), the row between sweeps kept in HBM
+    # (s6p2: two such pipelines per workgroup on halves of the columns, in
+    # step per sweep); the longest pairs exceed the 16-bit bound and take the
+    # int32 kernel
     rng = np.random.default_rng(seed)
     lens = list(rng.integers(lo, hi + 1, n)) + [1025, 1280, 1281, 2048]
     seqs = [_rand(rng, int(L)) for L in lens]
     st = ctx.upload(seqs)
-    saved = ctx.tune_get("k6"), ctx.tune_get("chunk6")
+    saved = ctx.tune_get("k6"), ctx.tune_get("chunk6"), ctx.tune_get("s6p2")
     try:
         ctx.tune_set("k6", k6)
         ctx.tune_set("chunk6", chunk6)
+        ctx.tune_set("s6p2", s6p2)
         g = st.all_pairs(M.FORM_BATCHED)
         o, _ = O.all_pairs(seqs, S)
         bad = np.argwhere(g != o)
@@ -230,6 +235,7 @@ def test_batched_multi_sweep_rows(ctx, seed, n, lo, hi, k6, chunk6):
     finally:
         ctx.tune_set("k6", saved[0])
         ctx.tune_set("chunk6", saved[1])
+        ctx.tune_set("s6p2", saved[2])
         st.free()
 
 
