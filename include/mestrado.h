@@ -189,7 +189,9 @@ int mst_server_stats(mst_server* srv, int* waiting, int* running, int* inflight,
                      int64_t* arena_used, double* cells_per_s);
 /* "round_cells" (DP cells per round, default 2e10, about 4 ms of the device),
  * "chunk" (column sequences per item, default 12), "chunk6" (per item of the
- * multi-sweep kernel, rows above 1024; 0 = "chunk"; default 8), "slack" (a
+ * multi-sweep kernel, rows above 1024; 0 = "chunk"; default 8), "s6p2" (1:
+ * the multi-sweep kernel as two pipelines per workgroup, the default; 0: one
+ * pipeline), "slack" (a
  * job's deadline
  * is its submission plus slack times its estimated run time, default 2;
  * jobs go earliest deadline first), "rate" (initial throughput estimate in

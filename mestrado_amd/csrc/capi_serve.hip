@@ -191,6 +191,9 @@ struct mst_server {
   int* d_job_ld = nullptr;
   int2* d_carry = nullptr;
   int grid6 = 1;
+  int grid7 = 1;        // workgroups of the two-pipeline multi-sweep kernel
+  int64_t carry_rows = 0;  // row buffers (of 16384 columns) per multi-sweep class
+  int s6p2 = 1;         // 1: the multi-sweep classes as two pipelines per workgroup
   int2* d_bufs = nullptr;  // int32 kernel strip buffers
   int64_t bufs_cap_cols = 0;
   int grid32 = 1;
@@ -637,11 +640,14 @@ int launch_round(mst_server* S, int* launched) {
     a.subst = ctx->d_S;
     a.go = ctx->go;
     a.ge = ctx->ge;
-    a.carry = S->d_carry + (c == kClassScore6K3 ? (int64_t)S->grid6 * 16384 : 0);
+    a.carry = S->d_carry + (c == kClassScore6K3 ? S->carry_rows * 16384 : 0);
     a.carry_cap = 16384;
     const int64_t nit = ranges[c].second - ranges[c].first;
     if (c == 0 || c == kClassScore6K3) {
-      launch_score6(c == 0 ? 4 : 3, a, (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)(S->grid6 * share), nit)), st);
+      if (S->s6p2)
+        launch_score7(c == 0 ? 4 : 3, a, (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)(S->grid7 * share), nit)), st);
+      else
+        launch_score6(c == 0 ? 4 : 3, a, (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)(S->grid6 * share), nit)), st);
     } else {
       int W, K;
       class_shape(c, &K, &W);
@@ -1100,7 +1106,11 @@ extern "C" int mst_server_create(mst_ctx* ctx, int64_t budget_bytes, mst_server*
   if ((e = hipMalloc(&S->d_job_out, sizeof(int64_t) * max_slots)) != hipSuccess) return fail("jobs", e);
   if ((e = hipMalloc(&S->d_job_ld, sizeof(int) * max_slots)) != hipSuccess) return fail("jobs", e);
   S->grid6 = ctx->num_cus * score6_groups_per_cu();
-  if ((e = hipMalloc(&S->d_carry, sizeof(int2) * (size_t)S->grid6 * 16384 * 2)) != hipSuccess)
+  S->grid7 = ctx->num_cus * score7_groups_per_cu();
+  // a row buffer per workgroup (two per workgroup of the two-pipeline form)
+  // and per multi-sweep class
+  S->carry_rows = std::max<int64_t>(S->grid6, 2 * (int64_t)S->grid7);
+  if ((e = hipMalloc(&S->d_carry, sizeof(int2) * (size_t)S->carry_rows * 16384 * 2)) != hipSuccess)
     return fail("carry", e);
   S->grid32 = ctx->num_cus * 4;
   S->bufs_cap_cols = 1 << 15;  // int32 kernel: pairs up to 32768 residues
@@ -1389,6 +1399,10 @@ extern "C" int mst_server_set(mst_server* S, const char* key, double value) {
   if (!S || !key) return MST_E_ARG;
   if (!strcmp(key, "round_cells") && value >= 1e6 && value <= 1e14) {
     S->round_cells = value;
+    return MST_OK;
+  }
+  if (!strcmp(key, "s6p2") && (value == 0 || value == 1)) {
+    S->s6p2 = (int)value;
     return MST_OK;
   }
   if (!strcmp(key, "chunk6") && value >= 0 && value <= 4096) {
