@@ -23,6 +23,11 @@
  *   orc_profile_*        build-defined profile-profile DP; pinned by reduction to
  *                        orc_nw_* on one-hot profiles and brute force on tiny
  *                        profiles.
+ *   orc_progressive_msa  the merges of tests/oracle_bind.progressive_msa (profile
+ *                        counts, orc_profile_scores, orc_nw_flags_sc, orc_nw_path,
+ *                        gap insertion) in C++ for large sets; pinned by equality
+ *                        with that Python restatement on small sets, and through
+ *                        it by the pins of orc_profile_* and orc_nw_*.
  *   orc_nj_*             SpeciesBlock.cpp:426-453,456-674,710-842,844-1343 (R5);
  *                        pinned by additive-tree recovery, hand-worked 4-taxon
  *                        cases, a textbook O(n^3) NJ, and the reference's own
@@ -95,6 +100,18 @@ int32_t orc_nw_flags_sc(const int32_t* sc, int n, int m, int go, int ge,
 /* Traceback producing an edit path: ops[k] = 'D' (both), 'L' (X column vs gap
  * column), 'U' (gap vs Y column), forward order; returns the length. */
 int orc_nw_path(const uint8_t* nib, int n, int m, char* ops);
+
+/* Progressive alignment of n sequences (ASCII, offsets[n+1]) along njoins
+ * joins given as (node, left, right) triples: each join aligns the left
+ * group (columns) against the right group (rows) and inserts the path's gap
+ * columns into every member. rows_out (may be NULL) receives row i at
+ * i*stride, NUL-terminated; *msa_len the number of columns. Returns 0, -1
+ * for joins that do not form one tree, or -2 (rows not written, *msa_len
+ * set) for a stride below *msa_len + 1. */
+int orc_progressive_msa(const char* ascii, const int64_t* offsets, int n,
+                        const int32_t* joins3, int njoins, const int8_t* S576,
+                        int go, int ge, char* rows_out, int64_t stride,
+                        int* msa_len, int nthreads);
 
 /* ---- neighbour joining (R5/R6) ---- */
 typedef struct {

@@ -65,6 +65,9 @@ def lib():
                                        C.c_float, C.c_int, C.c_int, C.POINTER(JoinT),
                                        C.POINTER(C.c_int)]
         L.orc_nj_fullscan.argtypes = [C.POINTER(C.c_float), C.c_int, C.c_float, C.POINTER(JoinT)]
+        L.orc_progressive_msa.argtypes = [C.c_char_p, C.POINTER(C.c_int64), C.c_int,
+                                          C.POINTER(C.c_int32), C.c_int, P8, C.c_int, C.c_int,
+                                          C.c_char_p, C.c_int64, C.POINTER(C.c_int), C.c_int]
         L.orc_tree_text.argtypes = [C.POINTER(JoinT), C.c_int, C.POINTER(C.c_char_p),
                                     C.c_char_p, C.c_int]
         _lib = L
@@ -253,3 +256,30 @@ def progressive_msa(seqs, joins, S, go=10, ge=2):
     for i, r in zip(members, rows):
         out[i] = r
     return out
+
+
+def progressive_msa_c(seqs, joins, S, go=10, ge=2, threads=0):
+    """progressive_msa in C++ (oracle/msa.cpp), for large sets: the same
+    merges, rows in input order."""
+    n = len(seqs)
+    off = np.zeros(n + 1, dtype=np.int64)
+    for i, s in enumerate(seqs):
+        off[i + 1] = off[i] + len(s)
+    j3 = np.array([(J.node, J.left, J.right) for J in joins], dtype=np.int32).reshape(-1)
+    if j3.size == 0:
+        j3 = np.zeros(3, dtype=np.int32)
+    L = C.c_int(0)
+    asc = "".join(seqs).encode()
+    stride = 2 * max([len(s) for s in seqs] + [0]) + 64  # room for the usual gap columns
+    for _ in range(2):
+        buf = C.create_string_buffer(n * stride)
+        rc = lib().orc_progressive_msa(asc, off.ctypes.data_as(C.POINTER(C.c_int64)), n,
+                                       j3.ctypes.data_as(C.POINTER(C.c_int32)), len(joins),
+                                       _i8(S), go, ge, buf, stride, C.byref(L), threads)
+        if rc != -2:
+            break
+        stride = L.value + 1
+    if rc != 0:
+        raise ValueError("orc_progressive_msa: joins do not form one tree")
+    raw = buf.raw
+    return [raw[r * stride: r * stride + L.value].decode() for r in range(n)]

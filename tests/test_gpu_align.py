@@ -26,6 +26,23 @@ def test_hand_tie_cases(ctx):
     _check(ctx, seqs, [0, 1, 2, 4, 6, 7], [1, 0, 3, 5, 7, 6])
 
 
+def test_continuation_tie_cases(ctx):
+    # hand-worked cases where extending and opening a gap tie (the reference
+    # extends, NeedlemanWunsch.cu:315-316,336): the expected strings are literal
+    from test_oracle_nw import CONTINUATION_TIES
+    seqs = []
+    for x, y, _, _, _ in CONTINUATION_TIES:
+        seqs += [x, y]
+    st = ctx.upload(seqs)
+    k = len(CONTINUATION_TIES)
+    sc, gx, gy = st.align_pairs(np.arange(0, 2 * k, 2), np.arange(1, 2 * k, 2),
+                                [len(s) for s in seqs])
+    for q, (x, y, ext, _, score) in enumerate(CONTINUATION_TIES):
+        assert sc[q] == score
+        assert (gx[q], gy[q]) == ext, (x, y)
+    st.free()
+
+
 def test_random_small_alphabet_ties(ctx):
     rng = np.random.default_rng(3)
     seqs = ["".join(rng.choice(list("AWCG"), int(n))) for n in rng.integers(1, 40, 60)]

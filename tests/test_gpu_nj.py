@@ -33,7 +33,9 @@ def test_nj_config0_scores(ctx):
     _same(g, O.nj_reference(D)[0], exact=False)
 
 
-@pytest.mark.parametrize("n,seed", [(2, 0), (3, 1), (57, 2), (450, 3), (700, 4)])
+# 230: one workgroup past the first T rebuild (after 200 joins); 450, 700: the
+# launches per step
+@pytest.mark.parametrize("n,seed", [(2, 0), (3, 1), (57, 2), (230, 5), (450, 3), (700, 4)])
 def test_nj_random_matrices(ctx, n, seed):
     rng = np.random.default_rng(seed)
     A = rng.integers(-300, 1500, (n, n))

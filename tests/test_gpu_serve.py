@@ -210,3 +210,32 @@ def test_msa_job_misuse(ctx):
         srv.drain()
         assert srv.job(a)["state"] == M.JOB_CANCELLED
         assert srv.stats()["arena_used"] == 0
+
+
+def test_large_msa_jobs_match_oracle(ctx):
+    """Jobs of 210-512 sequences (lengths 100-300) in one mix: their NJ runs
+    past the first T rebuild (after max(n/2, 200) joins) in the served
+    one-workgroup-per-job NJ launch. Scores, trees and rows against the
+    oracle: the full-scan NJ bit-exact, the reference restatement's topology
+    (its queue-size trigger not firing), the progressive alignment's rows."""
+    rng = np.random.default_rng(21)
+    jobs = [_job(rng, n, 100, 300) for n in (210, 347, 512)]
+    with M.Server(ctx, 4 << 30) as srv:
+        ids = [srv.submit(j, M.SERVE_MSA) for j in jobs]
+        srv.drain()
+        for jid, seqs in zip(ids, jobs):
+            assert srv.job(jid)["state"] == M.JOB_DONE
+            D, _ = O.all_pairs(seqs, S)
+            Df = D.astype(np.float32)
+            full, rebuilds = O.nj_fullscan(Df)
+            assert rebuilds >= 1
+            ref, _, trig = O.nj_reference(Df)
+            assert trig == 0
+            joins = srv.tree(jid)
+            for a, o, r in zip(joins, full, ref):
+                assert (a.node, a.left, a.right, a.n) == (o.node, o.left, o.right, o.n)
+                assert (a.d, a.Ti, a.Tj, a.q, a.len_left, a.len_right) == \
+                    (o.d, o.Ti, o.Tj, o.q, o.lenL, o.lenR)
+                assert (a.left, a.right) == (r.left, r.right)
+            assert srv.msa(jid) == O.progressive_msa_c(seqs, joins, S)
+            assert (srv.result(jid) == D).all()

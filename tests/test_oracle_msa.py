@@ -104,3 +104,20 @@ def test_progressive_msa_keeps_sequences():
     for s, r in zip(seqs, rows):
         assert r.replace("-", "") == s
     assert not any(all(r[t] == "-" for r in rows) for t in range(len(rows[0])))
+
+
+def test_c_restatement_equals_python():
+    # the C++ progressive alignment (used for large sets) equals the Python
+    # restatement merge for merge: empty sequences, ties, unknown letters,
+    # and profiles longer than its 64-row blocks of column scores
+    rng = np.random.default_rng(5)
+    cases = [
+        ["".join(rng.choice(list("AWC"), int(n))) for n in rng.integers(0, 25, 14)],
+        ["".join(rng.choice(list(O.STANDARD20), int(n))) for n in rng.integers(50, 200, 9)],
+        ["MKJQ", "mkoq", "", "AUWX*", "W"],
+        ["".join(rng.choice(list(O.STANDARD20), 150))] * 3 + ["A"],
+    ]
+    for seqs in cases:
+        D, _ = O.all_pairs(seqs, S)
+        joins, _ = O.nj_fullscan(D.astype(np.float32))
+        assert O.progressive_msa_c(seqs, joins, S) == O.progressive_msa(seqs, joins, S)

@@ -136,6 +136,41 @@ def test_traceback_tie_priorities_hand_cases():
     assert O.nw_align("WCG", "CW", S)[1:3] == ("-WCG", "CW--")
 
 
+# Continuation ties: extending a gap and opening one give the same value, so
+# F(r-1,c) - ge == H(r-1,c) - go (or the E analogue). The reference sets the
+# continuation bit on that tie (F_up = (F == F_prev - ge), NeedlemanWunsch.cu:
+# 315-316; the E bit at :336), and the traceback then keeps the gap going
+# (:430-433). Each case below has two optimal alignments (both rescore to the
+# brute-force maximum); "open on tie" would give the second string.
+CONTINUATION_TIES = [
+    # W vs AWW, worked by hand. DP on the reversed inputs: X = [W], Y = [W, W, A].
+    # r=0: H = 11 (diagonal). r=1: F = max(-12 - 2, 11 - 10) = 1, H = max(1, -2 + 11) = 9
+    # (diagonal). r=2 (A): F = max(F(1) - 2, H(1) - 10) = max(-1, -1) -> tie, contUp set;
+    # H = max(F = -1, E = -16, -4 + S[A][W] = -7) = -1 (up). Traceback from r=2: up, then
+    # prev was up with contUp -> up again (not r=1's diagonal), then the diagonal W/W.
+    #   extend: --W / AWW = -(10 + 2) + 11 = -1;  open: -W- / AWW = -10 + 11 - 2 = -1.
+    ("W", "AWW", ("--W", "AWW"), ("-W-", "AWW"), -1),
+    # the E analogue (X and Y swapped): left moves instead of up moves
+    ("AWW", "W", ("AWW", "--W"), ("AWW", "-W-"), -1),
+    # interior ties, both alternatives found by brute force over every alignment
+    ("WAGC", "GACGWC", ("WA--GC", "GACGWC"), ("WA-G-C", "GACGWC"), -3),
+    ("CGWCWW", "GCWW", ("CGWCWW", "--GCWW"), ("CGWCWW", "-G-CWW"), 17),
+    ("CCGW", "WACCW", ("--CCGW", "WACC-W"), ("-CCGW", "WACCW"), 7),
+]
+
+
+@pytest.mark.parametrize("x,y,ext,opn,score", CONTINUATION_TIES)
+def test_traceback_continuation_ties_extend(x, y, ext, opn, score):
+    assert O.nw_score(x, y, S) == score
+    assert O.bruteforce(x, y, S) == score
+    # both alternatives are optimal alignments of x and y ...
+    for gx, gy in (ext, opn):
+        assert gx.replace("-", "") == x and gy.replace("-", "") == y
+        assert O.rescore(gx, gy, S) == score
+    # ... and the oracle takes the one that extends on the tie
+    assert O.nw_align(x, y, S)[1:3] == ext
+
+
 def test_nibbles_of_reference_layout():
     # First cell of a 1x1 alignment of equal residues: H == diag only, so the
     # direction is CROSSWISE (12) and the continuation bits compare against -inf.
