@@ -398,10 +398,6 @@ def main():
         if cls <= -1000:
             w, k = divmod(-cls - 1000, 100)
             return "score5_kernel<%d, %d>" % (k, w), OPS_PER_CELL, PEAK_INT16_TOPS, "int16x2"
-        if cls <= -100:
-            return "score2_kernel<%d>" % (-100 - cls), OPS_PER_CELL, PEAK_INT16_TOPS, "int16x2"
-        if cls < 0:
-            return "score3_kernel<%d>" % (-cls), OPS_PER_CELL, PEAK_INT16_TOPS, "int16x2"
         return "score_pairs_kernel<%d>" % cls, OPS_PER_CELL, PEAK_INT32_TOPS, "int32"
 
     peak_meas = {"int32": round(ctx.valu_peak(False), 2), "int16x2": round(ctx.valu_peak(True), 2)}

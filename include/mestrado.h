@@ -77,8 +77,7 @@ int mst_all_pairs_resident(mst_ctx* ctx, int set_id, int form, float* kernel_ms)
  * form's launches are not graph-replayed here), for per-launch timing. */
 int mst_all_pairs_launches(mst_ctx* ctx, int set_id, int form);
 /* Per-launch record of the last mst_all_pairs_launches call: rows per lane of
- * the kernel instance (negative: -K for the shared-profile 16-bit kernel,
- * -100-K for the multi-strip 16-bit kernel, -(1000+100*W+K) for the pipelined
+ * the kernel instance (negative: -(1000+100*W+K) for the pipelined
  * 16-bit kernel with W waves of K rows per lane, -2000-K for the multi-sweep
  * 16-bit kernel), pairs, DP cells and device
  * milliseconds of each launch (arrays of cap entries; *count receives the
@@ -87,9 +86,7 @@ int mst_last_launch_stats(mst_ctx* ctx, int* rows_per_lane, int64_t* pairs, int6
                           float* ms, int cap, int* count);
 /* Tunables of the batched form (no counterpart in the reference; the
  * reference's window size wSize becomes such a hint): "chunk" (column
- * sequences per work item, 1..4096; default 32), "shared3" (rows 129..320
- * by the shared-profile kernel 1, or by the streamed kernel 0, the default),
- * "k5min" (least rows per lane of the one-wave streamed kernel, 1..3: short
+ * sequences per work item, 1..4096; default 32), "k5min" (least rows per lane of the one-wave streamed kernel, 1..3: short
  * row pairs of m rows take ceil(m/64) rows per lane when that is larger;
  * default 1), "chunk_w3" (column sequences per work item of the pipelines
  * of three and four waves, 0 = "chunk"; default 0), "k1max" (most rows per
@@ -109,8 +106,8 @@ int mst_last_launch_stats(mst_ctx* ctx, int* rows_per_lane, int64_t* pairs, int6
  * "s6p2" (1: the multi-sweep classes as two pipelines per workgroup sharing
  * the strip profiles, in step per sweep; default 0), "stack" (1: consecutive row pairs of the one-wave classes of 2 and 3
  * rows per lane share a strip of 4 or 6 rows per lane, one pair on each half
- * of the wave; default 0), "wg5", "wg2" and
- * "wg3.<K>" (workgroups per CU, 0 = default). Setting a value drops the cached
+ * of the wave; default 0), "wg5" (workgroups per CU of the streamed
+ * kernel, 0 = default). Setting a value drops the cached
  * batched plans. Unknown keys or values out of range return MST_E_ARG. */
 int mst_tune_set(mst_ctx* ctx, const char* key, int value);
 int mst_tune_get(mst_ctx* ctx, const char* key, int* value);

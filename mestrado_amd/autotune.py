@@ -1,8 +1,7 @@
 """Autotuner of the batched all-pairs form.
 
 Searches the tunables of include/mestrado.h (mst_tune_set: "chunk" column
-sequences per work item, "shared3" routing of rows 129..320, "wg5"
-workgroups per CU of the streamed kernel, "k5min" least rows per lane of its
+sequences per work item, "wg5" workgroups per CU of the streamed kernel, "k5min" least rows per lane of its
 one-wave classes, "k1max" most rows per lane of a one-wave pipeline,
 "chunk_w3" item size of its 3- and 4-wave pipelines) on a workload, timing each
 candidate with the resident all-pairs pass (device time, plan and graph built
@@ -27,7 +26,8 @@ import sys
 import numpy as np
 
 TUNING_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning")
-KEYS = ("chunk", "shared3", "wg5", "k5min", "chunk_w3", "k1max", "k2max", "branches", "wmax", "k6", "chunk6", "stack")
+KEYS = ("chunk", "wg5", "k5min", "chunk_w3", "k1max", "k2max", "branches", "wmax", "k6", "chunk6",
+        "chunk_k9", "s6p2", "stack")
 BUCKETS = ((256, "<=256"), (1024, "<=1024"), (None, ">1024"))
 
 
@@ -82,11 +82,10 @@ def apply_for(ctx, lengths):
 
 
 def candidates():
-    # shared3 = 1 and k5min = 3 lost by 10-20 and 5-8 ms on the 4,000-protein
-    # set in a wider search, k1max 14-16 and items of 72 or more sequences by
+    # k5min = 3 lost by 5-8 ms on the 4,000-protein set in a wider search, k1max 14-16 and items of 72 or more sequences by
     # 3-70 ms; the search keeps the knobs that stayed close
     for chunk, chunk6, k1max in itertools.product((32, 48, 64), (2, 4), (12, 13)):
-        yield {"shared3": 0, "chunk": chunk, "chunk6": chunk6, "wg5": 0, "k5min": 1, "k1max": k1max}
+        yield {"chunk": chunk, "chunk6": chunk6, "wg5": 0, "k5min": 1, "k1max": k1max}
 
 
 def tune(ctx, seqs, reps=2, log=None):

@@ -5,7 +5,7 @@
 
 #include <vector>
 
-#include "nw_score2.h"
+#include "nw_score5.h"
 
 struct BClass {
   int K;
@@ -20,9 +20,6 @@ struct BClass {
 // mestrado_amd/autotune.py searches them and persists the best per device).
 struct Tune {
   int chunk = 32;            // column sequences per work item
-  int shared3 = 0;           // rows 129..320: 1 shared-profile kernel, 0 streamed kernel
-  int wg_per_cu3[17] = {0};  // score3 workgroups per CU by rows-per-lane K (0: default)
-  int wg_per_cu2 = 0;        // score2 workgroups per CU (0: default)
   int wg5 = 0;               // score5 workgroups per CU (0: as many as LDS allows)
   int k5min = 1;             // least rows per lane of the one-wave streamed kernel (1..3)
   int chunk_w3 = 0;          // column sequences per item of 3- and 4-wave pipelines (0: chunk)
@@ -42,10 +39,6 @@ struct Tune {
 struct BatchedPlan {
   bool built = false;
   std::vector<int32_t> xlist;  // sequence ids sorted by length
-  std::vector<mst::Item3> items3;
-  std::vector<BClass> c3;
-  std::vector<mst::Pair2> items2;
-  std::vector<BClass> c2;
   std::vector<mst::Item3> items5;  // pipelined packed kernel (score5)
   std::vector<int2> pair2_5;       // their second row pair (stacked items; else -1)
   int2* d_pair2 = nullptr;
@@ -56,8 +49,6 @@ struct BatchedPlan {
   std::vector<int32_t> fb_i, fb_j;  // int32 fallback pairs (column, row)
   // device copies
   bool uploaded = false;
-  mst::Item3* d_items3 = nullptr;
-  mst::Pair2* d_items2 = nullptr;
   int32_t* d_xlist = nullptr;
   mst::Item3* d_items5 = nullptr;
   mst::Item3* d_items6 = nullptr;

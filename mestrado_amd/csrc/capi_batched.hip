@@ -8,9 +8,7 @@
 // work item, swept as one column stream by the kernel of the row pair's class
 // (the routes are listed below: one-wave, two-wave and wider pipelines of the
 // streamed kernel, the multi-sweep kernel, optionally stacked row pairs).
-// Pairs whose 16-bit bound fails, and rows outside the enabled classes, go to
-// the int32 kernel (the per-wave multi-strip 16-bit kernel, score2, is
-// switched off). Items are ordered longest first inside every launch; the
+// Pairs whose 16-bit bound fails go to the int32 kernel. Items are ordered longest first inside every launch; the
 // packed launches are captured once into a hipGraph (in parallel branches)
 // and replayed, and the int32 launches follow the replay on the same stream.
 #include <algorithm>
@@ -22,7 +20,6 @@
 #include "batched.h"
 #include "ctx.h"
 #include "nw_score.h"
-#include "nw_score2.h"
 #include "nw_score5.h"
 
 using namespace mst;
@@ -74,23 +71,14 @@ bool pk16_usable(const mst_ctx* ctx) {
 //  * m <= 1024: the pipelined packed kernel (score5): one wave of K =
 //    max(k5min, ceil(m/64)) rows per lane for m <= 64 * k1max (K up to 16),
 //    else W = ceil(m/256) waves of 64*K rows;
-//  * with tune.shared3 = 1, rows 129..320 take the shared-profile kernel
-//    (score3) instead, in its classes K = 3..5;
 //  * m > 1024: the multi-sweep packed kernel (score6), up to the 16-bit bound,
 //    in strips of 64*K rows, K = 3 or 4 (tune.k6, 0: per row pair, the K
 //    whose four-strip sweeps cover m with fewer idle rows, K = 3 counted at
 //    the 0.89 of K = 4's rate per row that one-wave launches measure);
 //  * every pair beyond the bound: the int32 kernel.
-// score3's other classes and the multi-strip packed kernel (score2) are
-// switched off: a run that used them (lengths 1..3100, classes K = 1, 2, 8, 9,
-// 12, 16 and the multi-strip kernel) did not finish on the device, with no
-// cause found in the code.
-constexpr int kPackedMinK = 3, kPackedMaxK = 5;
-constexpr bool kMultiStripPacked = false;
-constexpr bool kPipelined = true;
-// Rows beyond 1024 (up to the 16-bit bound) go to the multi-sweep kernel
-// (score6); a work item's column stream stays within kScore6MaxCols.
-constexpr bool kSweeps = true;
+// A work item's column stream of the multi-sweep kernel stays within
+// kScore6MaxCols (the per-workgroup row buffer).
+constexpr int kOneWaveShortK = 5;
 constexpr int64_t kScore6MaxCols = 16384;
 
 static int rows_per_lane(int m, int* nstrips) {
@@ -117,13 +105,7 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
     int64_t cost, cells, pairs;
     int2 p2 = make_int2(-1, -1);  // stacked items: the second row pair
   };
-  struct It2 {
-    Pair2 it;
-    int K;
-    int64_t cost, cells, pairs;
-  };
-  std::vector<It3> v3, v5, v6;
-  std::vector<It2> v2;
+  std::vector<It3> v5, v6;
   for (int k = 0; 2 * k < N; ++k) {
     const int ya = order[2 * k];
     const bool single = (2 * k + 1 >= N);
@@ -143,32 +125,26 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
     int nstr;
     int K = rows_per_lane(m, &nstr);
     int W = 1;   // waves per pipeline (score5)
-    int kind;    // 3: score3, 5: score5, 6: score6, 2: score2, 0: int32 kernel
-    if (nstr == 1 && m <= 64 * kPackedMaxK) {
-      if (ctx->tune.shared3 && K >= kPackedMinK) {
-        kind = 3;
-      } else if (kPipelined) {
-        K = std::max(K, ctx->tune.k5min);  // one wave, columns streamed
-        kind = 5;
-      } else {
-        kind = 0;
-      }
-    } else if (nstr == 1 && kPipelined && m <= 64 * ctx->tune.k1max) {
+    int kind;    // 5: score5 (one sweep of W waves), 6: the multi-sweep kernel
+    if (nstr == 1 && m <= 64 * kOneWaveShortK) {
+      K = std::max(K, ctx->tune.k5min);  // one wave, columns streamed
+      kind = 5;
+    } else if (nstr == 1 && m <= 64 * ctx->tune.k1max) {
       K = (m + 63) / 64;  // one wave of up to k1max rows per lane
       kind = 5;
-    } else if (nstr == 1 && kPipelined && (m + 127) / 128 <= ctx->tune.k2max) {
+    } else if (nstr == 1 && (m + 127) / 128 <= ctx->tune.k2max) {
       W = 2;  // two waves of up to k2max rows per lane
       K = (m + 127) / 128;
       kind = 5;
-    } else if (nstr == 1 && kPipelined) {
+    } else if (nstr == 1) {
       W = (m + 255) / 256;
       K = (m + 64 * W - 1) / (64 * W);
       kind = 5;
-    } else if (nstr > 1 && kPipelined && (m + 255) / 256 <= ctx->tune.wmax) {
+    } else if ((m + 255) / 256 <= ctx->tune.wmax) {
       W = (m + 255) / 256;  // one sweep of W waves, 4 rows per lane (m > 1024: K = 4)
       K = (m + 64 * W - 1) / (64 * W);
       kind = 5;
-    } else if (nstr > 1 && kSweeps) {
+    } else {
       K = ctx->tune.k6;
       if (K == 0) {
         // row capacity of whole four-strip sweeps, over the per-row rate
@@ -176,15 +152,9 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
         K = cap(3) / 0.89 < cap(4) ? 3 : 4;
       }
       kind = 6;
-    } else if (nstr > 1 && kMultiStripPacked) {
-      kind = 2;
-    } else {
-      kind = 0;
     }
-    int64_t R = (int64_t)nstr * 64 * K;  // padded rows (bound of the 16-bit values)
-    if (kind == 5) R = (int64_t)W * 64 * K;
-    if (kind == 6) R = (int64_t)(m + 64 * K - 1) / (64 * K) * 64 * K;
-    const bool packed = kind != 0;
+    // padded rows (bound of the 16-bit values)
+    const int64_t R = kind == 5 ? (int64_t)W * 64 * K : (int64_t)(m + 64 * K - 1) / (64 * K) * 64 * K;
     // rows (ya, yb) against columns s[first..last]; yb < 0: ya alone. The
     // streamed kernels name rows by position (pa, pb), the others by id.
     auto add_cols = [&](int first, int last, int pa, int pb) {
@@ -193,7 +163,7 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
       for (int t = first; t <= last; ++t) {
         const int x = order[t];
         const int n = len(x);
-        if (!packed || n == 0 || pk16_bound(ctx, R, n) > 32767) {
+        if (n == 0 || pk16_bound(ctx, R, n) > 32767) {
           fb_i.push_back(x);
           fb_j.push_back(ya_);
           if (yb_ >= 0) {
@@ -204,7 +174,7 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
           elig.push_back(t);
         }
       }
-      if (kind == 3 || kind == 5 || kind == 6) {
+      {
         // pipelines of three or four waves take their own item size
         int Cw = (kind == 5 && W >= 3 && ctx->tune.chunk_w3 > 0) ? ctx->tune.chunk_w3 : C;
         if (kind == 6 && ctx->tune.chunk6 > 0) Cw = ctx->tune.chunk6;
@@ -222,8 +192,8 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
             ++c1;
           }
           Item3 it;
-          it.ya = kind == 3 ? ya_ : pa;
-          it.yb = kind == 3 ? yb_ : pb;
+          it.ya = pa;
+          it.yb = pb;
           it.xbegin = elig[c0];
           it.xcount = (int)(c1 - c0);
           int64_t cost = 0, cells = 0;
@@ -233,25 +203,11 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
             cells += (int64_t)len(order[elig[q]]) * rows;
           }
           const int64_t npairs = (int64_t)(c1 - c0) * (yb_ >= 0 ? 2 : 1);
-          if (kind == 3)
-            v3.push_back(It3{it, K, 1, cost, cells, npairs});
-          else if (kind == 5)
+          if (kind == 5)
             v5.push_back(It3{it, K, W, cost * W, cells, npairs});
           else
             v6.push_back(It3{it, K, (int)(R / 256), cost * (R / (64 * K)), cells, npairs});
           c0 = c1;
-        }
-      } else {
-        for (int t : elig) {
-          Pair2 it;
-          it.x = order[t];
-          it.ya = ya_;
-          it.yb = yb_;
-          it.pad = 0;
-          it.oa = it.ob = -1;
-          const int64_t rows = len(ya_) + (yb_ >= 0 ? len(yb_) : 0);
-          v2.push_back(It2{it, K, (int64_t)(len(order[t]) + 63) * R, (int64_t)len(order[t]) * rows,
-                           yb_ >= 0 ? 2 : 1});
         }
       }
     };
@@ -261,7 +217,7 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
     // from this pair's ya on: the few pairs both row pairs then meet (each
     // other's rows) are scored twice, to the same value; otherwise over their
     // common columns only (from the next pair's yb on), the rest as before.
-    const bool stack = ctx->tune.stack && !ctx->tune.shared3 && kind == 5 && W == 1 && K >= 2 && K <= 3 &&
+    const bool stack = ctx->tune.stack && kind == 5 && W == 1 && K >= 2 && K <= 3 &&
                        !single &&
                        len(ya) > 0 && 2 * k + 3 < N &&
                        std::max(ctx->tune.k5min, (len(order[2 * k + 3]) + 63) / 64) == K &&
@@ -336,14 +292,6 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
       add_cols(2 * k + 1, N - 1, 2 * k, 2 * k + 1);  // every x at or after yb, both rows
     }
   }
-  auto by_class = [](auto& v) {
-    std::stable_sort(v.begin(), v.end(), [](const auto& a, const auto& b) {
-      if (a.K != b.K) return a.K < b.K;
-      return a.cost > b.cost;
-    });
-  };
-  by_class(v3);
-  by_class(v2);
   std::stable_sort(v5.begin(), v5.end(), [](const It3& a, const It3& b) {
     if (a.W != b.W) return a.W < b.W;
     if ((a.p2.x >= 0) != (b.p2.x >= 0)) return a.p2.x < 0;
@@ -381,26 +329,6 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
     P.c5.back().cells += v5[q].cells;
     P.c5.back().pairs += v5[q].pairs;
   }
-  for (size_t q = 0; q < v3.size(); ++q) {
-    P.items3.push_back(v3[q].it);
-    if (P.c3.empty() || P.c3.back().K != v3[q].K)
-      P.c3.push_back(BClass{v3[q].K, (int64_t)q, (int64_t)q + 1, 0});
-    else
-      P.c3.back().end = q + 1;
-    P.c3.back().work += v3[q].cost;
-    P.c3.back().cells += v3[q].cells;
-    P.c3.back().pairs += v3[q].pairs;
-  }
-  for (size_t q = 0; q < v2.size(); ++q) {
-    P.items2.push_back(v2[q].it);
-    if (P.c2.empty() || P.c2.back().K != v2[q].K)
-      P.c2.push_back(BClass{v2[q].K, (int64_t)q, (int64_t)q + 1, 0});
-    else
-      P.c2.back().end = q + 1;
-    P.c2.back().work += v2[q].cost;
-    P.c2.back().cells += v2[q].cells;
-    P.c2.back().pairs += v2[q].pairs;
-  }
   P.fb_i = fb_i;
   P.fb_j = fb_j;
   P.built = true;
@@ -414,23 +342,10 @@ int run_plan_on(mst_ctx* ctx, const SeqSet& s, const PairPlan& plan, const int2*
                 int32_t* d_out_pair, int32_t* d_out_mat, bool timed, int event_base,
                 hipStream_t st = nullptr);
 
-static int grid3(mst_ctx* ctx, int K, int64_t items) {
-  int per = ctx->tune.wg_per_cu3[K < 17 ? K : 16];
-  if (per <= 0) per = K <= 4 ? 8 : (K <= 8 ? 6 : 3);
-  return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ctx->num_cus * per, items));
-}
-
-static int grid2(mst_ctx* ctx, int64_t items) {
-  int per = ctx->tune.wg_per_cu2 > 0 ? ctx->tune.wg_per_cu2 : 8;
-  return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ctx->num_cus * per, items));
-}
-
 int upload_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P) {
   if (P.uploaded) return MST_OK;
-  CK(hipMalloc(&P.d_items3, sizeof(Item3) * std::max<size_t>(P.items3.size(), 1)));
-  CK(hipMalloc(&P.d_items2, sizeof(Pair2) * std::max<size_t>(P.items2.size(), 1)));
   CK(hipMalloc(&P.d_xlist, sizeof(int32_t) * std::max<size_t>(P.xlist.size(), 1)));
-  P.ncounters = (int)(P.c3.size() + P.c2.size() + P.c5.size() + P.c6.size()) + 1;
+  P.ncounters = (int)(P.c5.size() + P.c6.size()) + 1;
   const bool streamed = !P.c5.empty() || !P.c6.empty();
   if (!P.c6.empty()) {
     CK(hipMalloc(&P.d_items6, sizeof(Item3) * P.items6.size()));
@@ -460,10 +375,6 @@ int upload_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P) {
     CK(hipMalloc(&P.d_scodes, std::max<int64_t>(soff[N], 1)));
   }
   CK(hipMalloc(&P.d_counters, sizeof(unsigned int) * P.ncounters));
-  if (!P.items3.empty())
-    CK(hipMemcpy(P.d_items3, P.items3.data(), sizeof(Item3) * P.items3.size(), hipMemcpyHostToDevice));
-  if (!P.items2.empty())
-    CK(hipMemcpy(P.d_items2, P.items2.data(), sizeof(Pair2) * P.items2.size(), hipMemcpyHostToDevice));
   if (!P.xlist.empty())
     CK(hipMemcpy(P.d_xlist, P.xlist.data(), sizeof(int32_t) * P.xlist.size(), hipMemcpyHostToDevice));
   if (streamed) {
@@ -502,25 +413,6 @@ int enqueue_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P, int32
                      P.ncounters);
   CK(hipGetLastError());
   int ev = 0, ctr = 0;
-  for (const BClass& c : P.c3) {
-    Score3Args a;
-    a.seqs = s.d_codes;
-    a.off = s.d_off;
-    a.items = P.d_items3;
-    a.first = c.begin;
-    a.last = c.end;
-    a.counter = P.d_counters + ctr++;
-    a.xlist = P.d_xlist;
-    a.out_mat = d_out;
-    a.nseq = s.n;
-    a.subst = ctx->d_S;
-    a.go = ctx->go;
-    a.ge = ctx->ge;
-    if (timed) CK(hipEventRecord(ctx->launch_events[ev++], ctx->stream));
-    launch_score3(c.K, a, grid3(ctx, c.K, c.end - c.begin), ctx->stream);
-    CK(hipGetLastError());
-    if (timed) CK(hipEventRecord(ctx->launch_events[ev++], ctx->stream));
-  }
   auto args5 = [&](const BClass& c, const Item3* items) {
     Score5Args a;
     a.scodes = P.d_scodes;
@@ -609,40 +501,13 @@ int enqueue_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P, int32
       CK(hipStreamWaitEvent(ctx->stream, ctx->bevent[b], 0));
     }
   }
-  if (!P.c2.empty()) {
-    int64_t bufcap = std::max(s.maxlen, 1);
-    int g = grid2(ctx, (int64_t)P.items2.size());
-    int rc = ctx->ensure((void**)&ctx->d_bufs, &ctx->bufs_cap, sizeof(int2) * 2 * bufcap * (size_t)g);
-    if (rc) return rc;
-    for (const BClass& c : P.c2) {
-      Score2Args a;
-      a.seqs = s.d_codes;
-      a.off = s.d_off;
-      a.items = P.d_items2;
-      a.first = c.begin;
-      a.last = c.end;
-      a.counter = P.d_counters + ctr++;
-      a.out_mat = d_out;
-      a.nseq = s.n;
-      a.out_pair = nullptr;
-      a.bufs = ctx->d_bufs;
-      a.bufcap = bufcap;
-      a.subst = ctx->d_S;
-      a.go = ctx->go;
-      a.ge = ctx->ge;
-      if (timed) CK(hipEventRecord(ctx->launch_events[ev++], ctx->stream));
-      launch_score2(c.K, a, std::min(g, grid2(ctx, c.end - c.begin)), ctx->stream);
-      CK(hipGetLastError());
-      if (timed) CK(hipEventRecord(ctx->launch_events[ev++], ctx->stream));
-    }
-  }
   if (with_fallback) return enqueue_fallback(ctx, s, id, d_out, timed, ev);
   return MST_OK;
 }
 
 int batched_launch_count(mst_ctx* ctx, int id) {
   const BatchedPlan& P = ctx->bplans[id];
-  return (int)(P.c3.size() + P.c6.size() + P.c5.size() + P.c2.size() +
+  return (int)(P.c6.size() + P.c5.size() +
                ctx->fbplans[id].buckets.size());
 }
 
@@ -678,14 +543,10 @@ int run_batched(mst_ctx* ctx, int id, int32_t* d_out, bool timed) {
       ctx->launch_stats.push_back(LaunchStat{K, pairs, cells, t});
       return MST_OK;
     };
-    for (const BClass& c : P.c3)
-      if ((rc = rec(-c.K, c.pairs, c.cells))) return rc;
     for (const BClass& c : P.c6)
       if ((rc = rec(-2000 - c.K, c.pairs, c.cells))) return rc;
     for (const BClass& c : P.c5)
       if ((rc = rec(c.stacked ? -(3000 + c.K) : -(1000 + 100 * c.W + c.K), c.pairs, c.cells))) return rc;
-    for (const BClass& c : P.c2)
-      if ((rc = rec(-100 - c.K, c.pairs, c.cells))) return rc;
     for (const Bucket& b : ctx->fbplans[id].buckets)
       if ((rc = rec(b.K, b.end - b.begin, b.cells))) return rc;
     return MST_OK;
@@ -720,7 +581,7 @@ int run_batched(mst_ctx* ctx, int id, int32_t* d_out, bool timed) {
 
 void free_batched(BatchedPlan& P) {
   if (P.graph) hipGraphExecDestroy(P.graph);
-  void* ps[] = {P.d_items3, P.d_items2, P.d_xlist, P.d_counters, P.d_items5,
+  void* ps[] = {P.d_xlist, P.d_counters, P.d_items5,
                 P.d_scodes, P.d_soff, P.d_items6, P.d_carry, P.d_pair2};
   for (void* p : ps)
     if (p) hipFree(p);
@@ -731,7 +592,6 @@ void free_batched(BatchedPlan& P) {
 // plan (and its graph), so the next call plans with the new values.
 static int* tune_slot(mst_ctx* ctx, const char* key) {
   if (!strcmp(key, "chunk")) return &ctx->tune.chunk;
-  if (!strcmp(key, "shared3")) return &ctx->tune.shared3;
   if (!strcmp(key, "wg5")) return &ctx->tune.wg5;
   if (!strcmp(key, "k5min")) return &ctx->tune.k5min;
   if (!strcmp(key, "chunk_w3")) return &ctx->tune.chunk_w3;
@@ -744,11 +604,6 @@ static int* tune_slot(mst_ctx* ctx, const char* key) {
   if (!strcmp(key, "stack")) return &ctx->tune.stack;
   if (!strcmp(key, "chunk_k9")) return &ctx->tune.chunk_k9;
   if (!strcmp(key, "s6p2")) return &ctx->tune.s6p2;
-  if (!strcmp(key, "wg2")) return &ctx->tune.wg_per_cu2;
-  if (!strncmp(key, "wg3.", 4)) {
-    const int k = atoi(key + 4);
-    if (k >= 1 && k <= 16) return &ctx->tune.wg_per_cu3[k];
-  }
   return nullptr;
 }
 
@@ -758,8 +613,6 @@ extern "C" int mst_tune_set(mst_ctx* ctx, const char* key, int value) {
   if (!slot) return ctx->fail(MST_E_ARG, "mst_tune_set", "unknown key");
   if (!strcmp(key, "chunk") && (value < 1 || value > 4096))
     return ctx->fail(MST_E_ARG, "mst_tune_set", "chunk out of range 1..4096");
-  if (!strcmp(key, "shared3") && value != 0 && value != 1)
-    return ctx->fail(MST_E_ARG, "mst_tune_set", "shared3 is 0 or 1");
   if (!strcmp(key, "k5min") && (value < 1 || value > 3))
     return ctx->fail(MST_E_ARG, "mst_tune_set", "k5min out of range 1..3");
   if (!strcmp(key, "stack") && value != 0 && value != 1)
@@ -782,7 +635,7 @@ extern "C" int mst_tune_set(mst_ctx* ctx, const char* key, int value) {
     return ctx->fail(MST_E_ARG, "mst_tune_set", "k2max out of range 4..8");
   if (!strcmp(key, "k1max") && (value < 5 || value > 16))
     return ctx->fail(MST_E_ARG, "mst_tune_set", "k1max out of range 5..16");
-  if (strcmp(key, "chunk") && strcmp(key, "shared3") && strcmp(key, "chunk_w3") && strcmp(key, "chunk6") &&
+  if (strcmp(key, "chunk") && strcmp(key, "chunk_w3") && strcmp(key, "chunk6") &&
       strcmp(key, "chunk_k9") && (value < 0 || value > 64))
     return ctx->fail(MST_E_ARG, "mst_tune_set", "workgroups per CU out of range 0..64");
   if (*slot == value) return MST_OK;

@@ -2,9 +2,14 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "nw_score2.h"
-
 namespace mst {
+
+// Work item of the streamed kernels: a row pair (ya, yb) against the column
+// sequences at positions xbegin .. xbegin+xcount-1.
+struct Item3 {
+  int ya, yb;
+  int xbegin, xcount;
+};
 
 // Streamed packed forms (nw_score5.hip, nw_score6.hip). Rows and columns are
 // positions in a length-ordered copy of the codes; a work item (Item3) is a

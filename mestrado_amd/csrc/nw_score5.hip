@@ -1,7 +1,7 @@
 // Batched all-pairs scores (R1) in packed 16-bit arithmetic, pipelined over
 // the waves of a workgroup.
 //
-// The cell update is the packed kernel's (nw_score2.hip): pair (X, Ya) in the
+// The cell update: pair (X, Ya) in the
 // low and (X, Yb) in the high half of every VGPR, 6 packed ops per two cells
 // plus one v_perm_b32 to widen the profile bytes. What differs is how the rows
 // and columns are laid out (device parts in nw_stream.h):

@@ -14,7 +14,7 @@ def test_buckets_by_median_length():
 
 def test_save_and_load_round_trip(tmp_path, monkeypatch):
     monkeypatch.setattr(T, "TUNING_DIR", str(tmp_path))
-    best = {"chunk": 16, "shared3": 0, "wg5": 0, "ms": 1.5}
+    best = {"chunk": 16, "s6p2": 0, "wg5": 0, "ms": 1.5}
     p = T.save("gfx999:x", "<=1024", best, "w")
     t = T.load("gfx999")
     assert t["default"] == "<=1024"

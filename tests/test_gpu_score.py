@@ -29,8 +29,9 @@ def test_config0_all_pairs_bit_exact(ctx, form):
     M.FORM_PLAIN,
     pytest.param(M.FORM_BATCHED, marks=pytest.mark.skip(
         reason="the batched form did not finish on this case on the device (packed classes "
-               "K=1,2,8,9,12,16 and the multi-strip packed kernel, since switched off); no cause "
-               "found in the code or on the CPU emulation, so the case stays off the device")),
+               "K=1,2,8,9,12,16 and the multi-strip packed kernel, since deleted from the build); "
+               "no cause found in the code or on the CPU emulation, so the case stays off the "
+               "device")),
 ])
 def test_edge_lengths(ctx, form):
     # wave and strip boundaries: 64 lanes, K rows per lane (256/512/768/1024)
@@ -103,18 +104,16 @@ def test_batched_pipelined_rows(ctx, seed, n, lo, hi):
     st.free()
 
 
-@pytest.mark.parametrize("shared3,chunk,chunk_w3,branches",
-                         [(1, 32, 0, 1), (1, 1, 0, 1), (0, 5, 0, 2), (0, 32, 3, 3)])
-def test_batched_tunables_keep_results(ctx, shared3, chunk, chunk_w3, branches):
+@pytest.mark.parametrize("chunk,chunk_w3,branches", [(32, 0, 1), (1, 0, 1), (5, 0, 2), (32, 3, 3)])
+def test_batched_tunables_keep_results(ctx, chunk, chunk_w3, branches):
     # every routing / item size of the batched planner gives the same matrix
     rng = np.random.default_rng(11)
     lens = list(rng.integers(1, 900, 45)) + [128, 129, 320, 321, 1024]
     seqs = [_rand(rng, int(L)) for L in lens]
     o, _ = O.all_pairs(seqs, S)
     st = ctx.upload(seqs)
-    saved = {k: ctx.tune_get(k) for k in ("shared3", "chunk", "chunk_w3", "branches")}
+    saved = {k: ctx.tune_get(k) for k in ("chunk", "chunk_w3", "branches")}
     try:
-        ctx.tune_set("shared3", shared3)
         ctx.tune_set("chunk", chunk)
         ctx.tune_set("chunk_w3", chunk_w3)
         ctx.tune_set("branches", branches)
