@@ -69,6 +69,18 @@ int mst_all_pairs(mst_ctx* ctx, int set_id, int form, int32_t* scores_nxn,
 int mst_score_pairs(mst_ctx* ctx, int set_id, const int32_t* pi,
                     const int32_t* pj, int64_t npairs, int form, int32_t* out);
 
+/* No counterpart in the reference: mst_score_pairs (plain form) for callers
+ * that cannot allocate device memory (code under stream capture, tight
+ * serving loops, harnesses that forbid hipMalloc). Pair indices, scores and
+ * the kernel's row buffers live in a 64 MiB device array of the library,
+ * one per device, that holds 12 B per pair plus 16 B per residue of the
+ * set's longest sequence for each wave of the launch; larger requests return
+ * MST_E_ARG. No device memory is allocated or freed. Every context on a
+ * device shares that device's array: calls that overlap on one device race.
+ * Results equal mst_score_pairs's. */
+int mst_score_pairs_noalloc(mst_ctx* ctx, int set_id, const int32_t* pi,
+                            const int32_t* pj, int64_t npairs, int32_t* out);
+
 /* Benchmark hook: all pairs of a resident set with the result left on the
  * device; *kernel_ms receives the device time of the score launches (HIP
  * events on the context's stream). */

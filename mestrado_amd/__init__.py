@@ -42,6 +42,7 @@ def lib():
         L.mst_seqset_codes.argtypes = [P, C.c_int, P, C.c_int64]
         L.mst_all_pairs.argtypes = [P, C.c_int, C.c_int, P, P]
         L.mst_score_pairs.argtypes = [P, C.c_int, P, P, C.c_int64, C.c_int, P]
+        L.mst_score_pairs_noalloc.argtypes = [P, C.c_int, P, P, C.c_int64, P]
         L.mst_all_pairs_resident.argtypes = [P, C.c_int, C.c_int, C.POINTER(C.c_float)]
         L.mst_all_pairs_launches.argtypes = [P, C.c_int, C.c_int]
         L.mst_last_launch_stats.argtypes = [P, P, P, P, P, C.c_int, C.POINTER(C.c_int)]
@@ -266,6 +267,16 @@ class SeqSet:
         out = np.zeros(len(pi), dtype=np.int32)
         self.ctx._ck(self.ctx._L.mst_score_pairs(self.ctx.h, self.id, _p(pi), _p(pj), len(pi), form,
                                                  _p(out)), "mst_score_pairs")
+        return out
+
+    def score_pairs_noalloc(self, pi, pj):
+        """score_pairs (plain form) through the entry point that allocates no
+        device memory (mst_score_pairs_noalloc)."""
+        pi = np.ascontiguousarray(pi, dtype=np.int32)
+        pj = np.ascontiguousarray(pj, dtype=np.int32)
+        out = np.zeros(len(pi), dtype=np.int32)
+        self.ctx._ck(self.ctx._L.mst_score_pairs_noalloc(self.ctx.h, self.id, _p(pi), _p(pj), len(pi),
+                                                         _p(out)), "mst_score_pairs_noalloc")
         return out
 
     def align_pairs(self, pi, pj, lengths):

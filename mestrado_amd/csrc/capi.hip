@@ -16,6 +16,8 @@
 
 using namespace mst;
 
+int noalloc_init_device(int device, void** ptr);  // capi_noalloc.hip
+
 namespace {
 
 __global__ void fill_i32(int32_t* p, int64_t n, int32_t v) {
@@ -114,6 +116,10 @@ extern "C" int mst_create(int gap_open, int gap_ext, const int8_t* subst, int de
   CK(hipMalloc(&ctx->d_S, 576));
   CK(hipMemcpy(ctx->d_S, ctx->S, 576, hipMemcpyHostToDevice));
   CK(init_encode_table());
+  {
+    const int rc = noalloc_init_device(device, &ctx->d_scratch);
+    if (rc) return ctx->fail(rc, "hipGetSymbolAddress", "no-allocation scratch array");
+  }
   *out = c.release();
   return MST_OK;
 }

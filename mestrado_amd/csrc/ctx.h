@@ -60,6 +60,7 @@ struct mst_ctx {
   bool symmetric = true;
   int smax = 0;  // largest entry of S
   int8_t* d_S = nullptr;
+  void* d_scratch = nullptr;  // this device's address of the no-allocation scratch array
   std::string err;
   std::map<int, SeqSet> sets;
   std::map<int, PairPlan> plans;  // all-pairs plan per set (plain form)

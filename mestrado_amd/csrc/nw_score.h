@@ -8,7 +8,9 @@ namespace mst {
 struct ScoreArgs {
   const int8_t* seqs;    // reversed residue codes, concatenated
   const int64_t* off;    // nseq+1 offsets into seqs
-  const int2* pairs;     // (x sequence, y sequence) per pair
+  const int2* pairs;     // (x sequence, y sequence) per pair (or null: pi, pj)
+  const int32_t* pi = nullptr;  // x and y sequence per pair when pairs is null
+  const int32_t* pj = nullptr;
   int64_t first, last;   // pair index range [first, last)
   int32_t* out_pair;     // score per pair index (or null)
   int32_t* out_mat;      // nseq*nseq matrix, written at (x,y) and (y,x) (or null)

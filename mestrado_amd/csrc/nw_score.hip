@@ -264,7 +264,7 @@ __global__ void __launch_bounds__(MST_WAVE) score_pairs_kernel(ScoreArgs a) {
   __syncthreads();
   int2* buf = a.bufs + (int64_t)blockIdx.x * 2 * a.bufcap;
   for (int64_t p = (int64_t)blockIdx.x + a.first; p < a.last; p += gridDim.x) {
-    const int2 pr = a.pairs[p];
+    const int2 pr = a.pairs ? a.pairs[p] : make_int2(a.pi[p], a.pj[p]);
     int xs = pr.x, ys = pr.y;
     int64_t xo = a.off[xs], yo = a.off[ys];
     int n = (int)(a.off[xs + 1] - xo), m = (int)(a.off[ys + 1] - yo);
