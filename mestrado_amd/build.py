@@ -79,7 +79,7 @@ def build_cli():
          ["-L" + PKG, "-lmestrado", "-Wl,-rpath,$ORIGIN/.."])
     # device-free driver of the host classes, for the CPU tests
     plain = [os.path.join(HOST, f) for f in ("FastaFile.cpp", "CmdLineOptions.cpp", "TreeNode.cpp",
-                                             "MatchesManager.cpp")]
+                                             "MatchesManager.cpp", "MatrixFile.cpp")]
     _run(["g++", "-O2", "-std=c++17"] + inc + ["-o", os.path.join(BIN, "host_tool"),
           os.path.join(HOST, "tools", "host_tool.cpp")] + plain)
     return out

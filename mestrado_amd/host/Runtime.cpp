@@ -3,9 +3,6 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <fstream>
-#include <sstream>
-#include <vector>
 
 namespace mstrt {
 namespace {
@@ -23,28 +20,6 @@ void configure(const int8_t* subst, int device, int f) {
   }
   g_device = device;
   g_form = f;
-}
-
-bool loadMatrix(const std::string& path, int8_t out[576]) {
-  std::ifstream in(path);
-  if (!in) return false;
-  std::vector<int> v;
-  std::string line;
-  while (std::getline(in, line)) {
-    std::stringstream ss(line);
-    std::string tok;
-    while (std::getline(ss, tok, ',')) {
-      size_t a = tok.find_first_not_of(" \t\r");
-      if (a == std::string::npos) continue;
-      v.push_back(atoi(tok.c_str() + a));
-    }
-  }
-  if (v.size() != 576) return false;
-  for (int k = 0; k < 576; ++k) {
-    if (v[k] < -128 || v[k] > 127) return false;
-    out[k] = (int8_t)v[k];
-  }
-  return true;
 }
 
 mst_ctx* context(int go, int ge) {

@@ -15,7 +15,8 @@ namespace mstrt {
 // Settings for the context (call before the first AlignmentScore/-Match).
 void configure(const int8_t* subst576, int device, int form);
 // Loads a matrix file in the blosum64.txt format (24 rows of 24 comma-separated
-// integers, letter order ARNDCQEGHILKMFPSTWYVBZX*). Returns false on error.
+// integers, letter order ARNDCQEGHILKMFPSTWYVBZX*; MatrixFile.cpp). Returns
+// false on a missing file, a count other than 576 or a value outside int8.
 bool loadMatrix(const std::string& path, int8_t out[576]);
 // The context for gap penalties (go, ge); exits with a message on failure,
 // like the reference's gpuErrchk (AlignmentScore.cu:3-8).

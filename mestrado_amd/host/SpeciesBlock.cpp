@@ -27,8 +27,52 @@ SpeciesBlock::SpeciesBlock(const CmdLineOptions& cmd) : m_cmd(cmd), m_n0(0), max
 
 SpeciesBlock::~SpeciesBlock() {}
 
+// -i m: the matrix a run with -o m wrote (first line n, then per row the
+// sequence name and its n values); the tree is built from it, and there are no
+// sequences to align.
+void SpeciesBlock::readMatrix() {
+  std::ifstream in(m_cmd.inputFile.c_str());
+  std::string line;
+  int n = -1;
+  if (!std::getline(in, line) || sscanf(line.c_str(), "%d", &n) != 1 || n < 0) {
+    std::cerr << "Bad matrix file: " << m_cmd.inputFile << std::endl;
+    exit(1);
+  }
+  m_distances.assign(n, std::vector<float>(n, 0.0f));
+  for (int i = 0; i < n; ++i) {
+    if (!std::getline(in, line)) {
+      std::cerr << "Bad matrix file: " << m_cmd.inputFile << " (row " << i << " missing)" << std::endl;
+      exit(1);
+    }
+    // the n values are the last n fields; the name (which may hold spaces) precedes them
+    size_t end = line.size();
+    for (int j = n - 1; j >= 0; --j) {
+      while (end > 0 && line[end - 1] == ' ') --end;
+      size_t b = line.rfind(' ', end == 0 ? 0 : end - 1);
+      b = (b == std::string::npos) ? 0 : b + 1;
+      char* stop = nullptr;
+      m_distances[i][j] = strtof(line.c_str() + b, &stop);
+      if (b >= end || stop != line.c_str() + end) {
+        std::cerr << "Bad matrix file: " << m_cmd.inputFile << " (row " << i << ")" << std::endl;
+        exit(1);
+      }
+      end = b;
+    }
+    while (end > 0 && line[end - 1] == ' ') --end;
+    seqsName.push_back(line.substr(0, end));
+  }
+  m_n0 = n;
+  m_nodes.assign(m_n0 > 0 ? 2 * m_n0 - 1 : 0, TreeNode());
+  for (int i = 0; i < m_n0; ++i) m_nodes[i].init(i, seqsName[i].c_str());
+}
+
 void SpeciesBlock::readFasta() {
   m_Timer[tBuildMtx].start();
+  if (m_cmd.inputFormat == "m") {
+    readMatrix();
+    m_Timer[tBuildMtx].end();
+    return;
+  }
   FastaFile file(m_cmd.inputFile.c_str());
   int len = 0;
   std::string seq = file.nextSeq(&len);
@@ -66,9 +110,13 @@ void SpeciesBlock::readFasta() {
   if (m_cmd.outputFormat == "m") {  // the score matrix the tree is built from
     std::ofstream o((m_cmd.baseName + ".mtx").c_str());
     o << m_n0 << "\n";
+    char v[32];
     for (int i = 0; i < m_n0; ++i) {
       o << seqsName[i];
-      for (int j = 0; j < m_n0; ++j) o << " " << m_distances[i][j];
+      for (int j = 0; j < m_n0; ++j) {
+        snprintf(v, sizeof v, " %.9g", m_distances[i][j]);  // every float round-trips
+        o << v;
+      }
       o << "\n";
     }
   }
@@ -102,7 +150,7 @@ void SpeciesBlock::freeMemory() {
 
 void SpeciesBlock::buildAlign() {
   m_Timer[tBuildAlign].start();
-  if (m_n0 > 1) {
+  if (m_n0 > 1 && (int)seqs.size() == m_n0) {
     AlignmentMatch align(m_cmd.gapOpen, m_cmd.gapExt, 16, maxSeqLen);
     align.prepare(seqs, 0);
     std::vector<int> nodes;
@@ -120,7 +168,7 @@ void SpeciesBlock::buildAlign() {
     for (int i = m_n0; i < 2 * m_n0 - 2; ++i) m_nodes[i].update();
   }
   m_Timer[tBuildAlign].end();
-  if (m_cmd.writeMsa && m_n0 > 0) {
+  if (m_cmd.writeMsa && m_n0 > 0 && (int)seqs.size() == m_n0) {
     std::ofstream o((m_cmd.baseName + ".msa").c_str());
     for (int i = 0; i < m_n0; ++i) o << seqsName[i] << "\n" << seqs[i] << "\n";
   }

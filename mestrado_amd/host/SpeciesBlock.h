@@ -4,7 +4,9 @@
 // joining and writes <basename>.tree (:844-1343), buildAlign runs the
 // progressive alignment over every merge of the guide tree (:1379-1591),
 // reportTimers prints "BuildMtx,BuildTree,BuildAlign,Total" seconds to stderr
-// (:104-116), measured as wall time.
+// (:104-116), measured as wall time. With -i m (CmdLineOptions.cpp:41,88-98
+// accepts it; the reference reads nothing) the matrix a -o m run wrote is read
+// instead of a FASTA file, and only the tree is built.
 #ifndef MST_SPECIESBLOCK_H
 #define MST_SPECIESBLOCK_H
 
@@ -44,6 +46,7 @@ class SpeciesBlock {
   };
 
   CmdLineOptions m_cmd;
+  void readMatrix();
   int m_n0;
   int maxSeqLen;
   std::vector<int> seqsLen;

@@ -3,6 +3,11 @@
 //   host_tool cmd ARGS...    the parsed option fields (CmdLineOptions)
 //   host_tool tree < joins   tree text (TreeNode); input format: n0, n0 names,
 //                            then n0-1 lines "node left right d Ti Tj n"
+//   host_tool matrix FILE    the 576 values of a matrix file (-b), one row of
+//                            24 per line, or exit status 1
+//   host_tool matches W < ops  MatchesManager with window size W: lines
+//                            "put x y gappedX gappedY", "seq x y", "win wx wy";
+//                            seq/win print the two strings stored or "null"
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -13,6 +18,8 @@
 
 #include "CmdLineOptions.h"
 #include "FastaFile.h"
+#include "MatchesManager.h"
+#include "Runtime.h"
 #include "TreeNode.h"
 
 static int do_tree() {
@@ -64,10 +71,49 @@ static int do_cmd(int argc, char** argv) {
   return 0;
 }
 
+static int do_matrix(const char* path) {
+  int8_t m[576];
+  if (!mstrt::loadMatrix(path, m)) return 1;
+  for (int r = 0; r < 24; ++r)
+    for (int c = 0; c < 24; ++c) std::cout << (int)m[24 * r + c] << (c == 23 ? "\n" : " ");
+  return 0;
+}
+
+static void print_pair(const char* p) {
+  if (!p) {
+    std::cout << "null\n";
+    return;
+  }
+  std::cout << p << " " << p + strlen(p) + 1 << "\n";
+}
+
+static int do_matches(int window) {
+  MatchesManager mm(1 << 20, window, 0);
+  std::string op;
+  while (std::cin >> op) {
+    int a, b;
+    if (!(std::cin >> a >> b)) return 2;
+    if (op == "put") {
+      std::string gx, gy;
+      if (!(std::cin >> gx >> gy)) return 2;
+      mm.put(a, b, gx, gy);
+    } else if (op == "seq") {
+      print_pair(mm.getSequence(a, b));
+    } else if (op == "win") {
+      print_pair(mm.getWindow(a, b));
+    } else {
+      return 2;
+    }
+  }
+  return 0;
+}
+
 int main(int argc, char** argv) {
   if (argc < 2) return 2;
   if (!strcmp(argv[1], "tree")) return do_tree();
   if (!strcmp(argv[1], "fasta") && argc == 3) return do_fasta(argv[2]);
   if (!strcmp(argv[1], "cmd")) return do_cmd(argc - 1, argv + 1);
+  if (!strcmp(argv[1], "matrix") && argc == 3) return do_matrix(argv[2]);
+  if (!strcmp(argv[1], "matches") && argc == 3) return do_matches(atoi(argv[2]));
   return 2;
 }

@@ -69,3 +69,61 @@ def test_clustal_plain_form_and_matrix_output(tmp_path):
 def test_clustal_bad_arguments():
     assert subprocess.run([CLUSTAL, "-i", "z", "x"], capture_output=True).returncode == 1
     assert subprocess.run([CLUSTAL], capture_output=True).returncode == 0  # help state
+
+
+def test_clustal_matrix_input_round_trip(tmp_path):
+    # -o m writes the score matrix; -i m reads it back and builds the same
+    # tree (byte for byte) without the sequences (names with spaces included)
+    seqs = M.synth(0, 5)[:30]
+    names = [">gi|%d| protein %d of set five" % (i, i) for i in range(30)]
+    a, b = tmp_path / "a", tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    _write_fasta(a / "x.fasta", names, seqs)
+    p = subprocess.run([CLUSTAL, "-o", "m", "x.fasta"], cwd=a, capture_output=True, text=True,
+                       timeout=120)
+    assert p.returncode == 0, p.stderr
+    (b / "x.mtx").write_bytes((a / "x.mtx").read_bytes())
+    p = subprocess.run([CLUSTAL, "-i", "m", "-M", "x.mtx"], cwd=b, capture_output=True, text=True,
+                       timeout=120)
+    assert p.returncode == 0, p.stderr
+    assert (b / "x.tree").read_text() == (a / "x.tree").read_text()
+    assert not (b / "x.msa").exists()  # no sequences to align
+    D, _ = O.all_pairs(seqs, S)
+    joins, _, _ = O.nj_reference(D.astype(np.float32))
+    assert (b / "x.tree").read_text() == O.tree_text(joins, names)
+    (b / "bad.mtx").write_text("3\n>a 1 2 3\n>b 1 2\n")
+    p = subprocess.run([CLUSTAL, "-i", "m", "bad.mtx"], cwd=b, capture_output=True, text=True,
+                       timeout=60)
+    assert p.returncode == 1 and "Bad matrix file" in p.stderr
+
+
+def test_clustal_matrix_file_option(tmp_path):
+    # -b FILE: the blosum64.txt format; the BLOSUM62 fixture gives the built-in
+    # matrix's tree, a changed matrix the oracle's tree under that matrix
+    seqs = M.synth(0, 6)[:24]
+    names = [">s%d" % i for i in range(24)]
+    _write_fasta(tmp_path / "x.fasta", names, seqs)
+    fixture = os.path.join(ROOT, "tests", "golden", "blosum62_24.txt")
+    trees = []
+    for args in ([], ["-b", fixture]):
+        p = subprocess.run([CLUSTAL] + args + ["x.fasta"], cwd=tmp_path, capture_output=True,
+                           text=True, timeout=120)
+        assert p.returncode == 0, p.stderr
+        trees.append((tmp_path / "x.tree").read_text())
+    assert trees[0] == trees[1]
+    S2 = S.copy()
+    L = O.LETTERS
+    for a_, b_, v in (("W", "W", 15), ("A", "S", 3), ("S", "A", 3), ("L", "I", 4), ("I", "L", 4)):
+        S2[L.index(a_), L.index(b_)] = v
+    (tmp_path / "m2.txt").write_text("\n".join(", ".join(str(int(v)) for v in r) for r in S2) + "\n")
+    p = subprocess.run([CLUSTAL, "-b", "m2.txt", "x.fasta"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    D2, _ = O.all_pairs(seqs, S2)
+    joins, _, _ = O.nj_reference(D2.astype(np.float32))
+    assert (tmp_path / "x.tree").read_text() == O.tree_text(joins, names)
+    (tmp_path / "bad.txt").write_text("1, 2, 3\n")
+    p = subprocess.run([CLUSTAL, "-b", "bad.txt", "x.fasta"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=60)
+    assert p.returncode == 1 and "substitution matrix" in p.stderr

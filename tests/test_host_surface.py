@@ -92,3 +92,58 @@ def test_tree_text_matches_reference(n):
                                                 float(J.Ti).hex(), float(J.Tj).hex(), J.n))
     text = "\n".join(lines) + "\n"
     assert run(HOST_TOOL, "tree", stdin=text) == run(O.REF_TOOL, "tree", stdin=text)
+
+
+def _matrix_rows(text):
+    return [[int(v) for v in line.split()] for line in text.splitlines()]
+
+
+def test_matrix_file_reader(tmp_path):
+    # the -b reader on the blosum64.txt format (the reference ships that file,
+    # blosum64.txt:1-24, and reads it nowhere): the fixture parses to BLOSUM62
+    path = os.path.join(ROOT, "tests", "golden", "blosum62_24.txt")
+    rc, out = run(HOST_TOOL, "matrix", path)
+    assert rc == 0 and (np.array(_matrix_rows(out)) == O.blosum62()).all()
+    if os.path.exists(os.path.join(REF_CASES, "blosum64.txt")):
+        assert run(HOST_TOOL, "matrix", os.path.join(REF_CASES, "blosum64.txt")) == (rc, out)
+    rows = open(path).read().splitlines()
+    bad = {
+        "short.txt": "\n".join(rows[:23]) + "\n",                 # 552 values
+        "word.txt": "\n".join(rows[:23] + [rows[23].replace("1", "x")]) + "\n",
+        "range.txt": "\n".join(rows[:23] + [rows[23].replace(" 1", " 300")]) + "\n",
+    }
+    for name, text in bad.items():
+        (tmp_path / name).write_text(text)
+        assert run(HOST_TOOL, "matrix", str(tmp_path / name))[0] == 1, name
+    assert run(HOST_TOOL, "matrix", str(tmp_path / "missing.txt"))[0] == 1
+    # whitespace and a trailing comma per row, as blosum64.txt has, are accepted
+    (tmp_path / "spaced.txt").write_text("\n".join(" " + r + ", " for r in rows) + "\n")
+    assert run(HOST_TOOL, "matrix", str(tmp_path / "spaced.txt")) == (rc, out)
+
+
+def test_matches_manager_store():
+    # getSequence(x, y) is symmetric: the reference swaps so that x >= y
+    # (MatchesManager.cpp:37-45); getWindow(wx, wy) addresses the pairs of the
+    # window (x / W, y / W). The store returns the two gapped strings as put
+    # for the pair, in (larger index, smaller index) order.
+    ops = "\n".join([
+        "put 5 2 AC-G A-TG",   # x > y: kept as given
+        "put 1 7 WW- -WW",     # x < y: kept as (7, 1), strings swapped
+        "seq 5 2", "seq 2 5",
+        "seq 7 1", "seq 1 7",
+        "seq 3 3",
+        "win 1 0",             # W = 4: pair (5, 2) lies in window (1, 0)
+        "win 0 0", "win 1 1",  # (7, 1) is in window (1, 0) too; nothing in these
+        "put 9 8 A A",
+        "win 2 2",
+    ]) + "\n"
+    rc, out = run(HOST_TOOL, "matches", "4", stdin=ops)
+    assert rc == 0
+    assert out.splitlines() == [
+        "AC-G A-TG", "AC-G A-TG",
+        "-WW WW-", "-WW WW-",
+        "null",
+        "AC-G A-TG",
+        "null", "null",
+        "A A",
+    ]
