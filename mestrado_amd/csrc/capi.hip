@@ -134,7 +134,7 @@ mst_ctx::~mst_ctx() {
   }
   void* ps[] = {d_S,    d_pairs,    d_bufs,  d_mat, d_pscore, d_nj,    d_nj_T,
                 d_nj_act, d_nj_joins, d_flags, d_ops, d_jobs,   d_jscore, d_acounter,
-                d_tiles,  d_trows,    d_tstate, d_nj_state, d_nj_parts};
+                d_tiles,  d_trows,    d_tstate, d_nj_parts};
   for (void* p : ps)
     if (p) hipFree(p);
   for (hipEvent_t e : launch_events) hipEventDestroy(e);

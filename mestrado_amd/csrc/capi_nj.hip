@@ -53,7 +53,7 @@ constexpr int kNjStepsMinN = 256;
 static int run_nj(mst_ctx* ctx, int n, float ratio, mst_join* joins) {
   std::vector<int> init(2 * (size_t)n, 0);
   for (int i = 0; i < n; ++i) init[i] = i;
-  int rc = ctx->ensure((void**)&ctx->d_nj_T, &ctx->nj_T_cap, sizeof(double) * n);
+  int rc = ctx->ensure((void**)&ctx->d_nj_T, &ctx->nj_T_cap, sizeof(double) * 3 * (size_t)n);
   if (rc) return rc;
   rc = ctx->ensure((void**)&ctx->d_nj_act, &ctx->nj_act_cap, sizeof(int) * 4 * (size_t)n);
   if (rc) return rc;
@@ -67,6 +67,7 @@ static int run_nj(mst_ctx* ctx, int n, float ratio, mst_join* joins) {
   a.ld = n;
   a.n0 = n;
   a.T = ctx->d_nj_T;
+  a.Tpos = ctx->d_nj_T + n;
   a.act = ctx->d_nj_act;
   a.ids = ctx->d_nj_act + 2 * n;
   a.joins = (NjJoin*)ctx->d_nj_joins;
@@ -74,11 +75,9 @@ static int run_nj(mst_ctx* ctx, int n, float ratio, mst_join* joins) {
   if (n >= kNjStepsMinN) {
     // large N: one launch per step, the scan over the whole device
     const int nparts = ctx->num_cus * 2;
-    rc = ctx->ensure(&ctx->d_nj_state, &ctx->nj_state_cap, nj_state_bytes());
-    if (rc) return rc;
     rc = ctx->ensure(&ctx->d_nj_parts, &ctx->nj_parts_cap, nj_part_bytes(nparts));
     if (rc) return rc;
-    CK(run_nj_steps(a, ctx->d_nj_state, ctx->d_nj_parts, nparts, ctx->stream));
+    CK(run_nj_steps(a, ctx->d_nj_parts, nparts, ctx->stream));
   } else {
     CK(launch_nj(a, ctx->stream));
   }

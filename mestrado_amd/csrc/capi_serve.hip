@@ -286,7 +286,7 @@ Layout job_layout(const Job& J) {
     l.D = o;
     o += r256(N * N * 4);
     l.T = o;
-    o += r256(N * 8);
+    o += r256(N * 8 * 3);  // T by storage row, then by position (two halves)
     l.act = o;
     o += r256(N * 16);
     l.joins = o;
@@ -858,6 +858,7 @@ int launch_nj_jobs(mst_server* S) {
     a.ld = n;
     a.n0 = n;
     a.T = (double*)(S->d_arena + J.o_T);
+    a.Tpos = a.T + n;
     a.act = act;
     a.ids = act + 2 * n;
     a.joins = (NjJoin*)(S->d_arena + J.o_joins);

@@ -89,8 +89,6 @@ struct mst_ctx {
   size_t nj_act_cap = 0;
   void* d_nj_joins = nullptr;
   size_t nj_joins_cap = 0;
-  void* d_nj_state = nullptr;  // NJ as launches per step (large N)
-  size_t nj_state_cap = 0;
   void* d_nj_parts = nullptr;
   size_t nj_parts_cap = 0;
 

@@ -35,8 +35,11 @@ constexpr int NJ_THREADS = 1024;
 
 struct Best {
   double q;
-  int J, I;  // node ids, I < J
-  int pa, pb;  // positions in the active array
+  double Ta, Tb;  // T of the pair's storage rows
+  float d;        // D of the pair
+  int J, I;       // node ids, I < J
+  int pa, pb;     // positions in the active array, pa < pb
+  int ra, rb;     // storage rows
 };
 
 __device__ __forceinline__ bool better(const Best& a, const Best& b) {
@@ -46,50 +49,112 @@ __device__ __forceinline__ bool better(const Best& a, const Best& b) {
   return a.I > b.I;
 }
 
+// b = better(c, b) ? c : b, field by field (a whole-struct select or a Best
+// assigned in a loop is kept in scratch memory by the compiler)
+__device__ __forceinline__ void keep_better(Best& b, const Best& c) {
+  const bool t = better(c, b);
+  b.q = t ? c.q : b.q;
+  b.Ta = t ? c.Ta : b.Ta;
+  b.Tb = t ? c.Tb : b.Tb;
+  b.d = t ? c.d : b.d;
+  b.J = t ? c.J : b.J;
+  b.I = t ? c.I : b.I;
+  b.pa = t ? c.pa : b.pa;
+  b.pb = t ? c.pb : b.pb;
+  b.ra = t ? c.ra : b.ra;
+  b.rb = t ? c.rb : b.rb;
+}
+
+__device__ __forceinline__ Best no_best() {
+  Best b;
+  b.q = DBL_MAX;
+  b.Ta = b.Tb = 0.0;
+  b.d = 0.0f;
+  b.J = b.I = -1;
+  b.pa = b.pb = b.ra = b.rb = -1;
+  return b;
+}
+
+__device__ __forceinline__ Best wave_best(Best best) {
+  for (int off = 32; off > 0; off >>= 1) {
+    Best o;
+    o.q = __shfl_down(best.q, off);
+    o.Ta = __shfl_down(best.Ta, off);
+    o.Tb = __shfl_down(best.Tb, off);
+    o.d = __shfl_down(best.d, off);
+    o.J = __shfl_down(best.J, off);
+    o.I = __shfl_down(best.I, off);
+    o.pa = __shfl_down(best.pa, off);
+    o.pb = __shfl_down(best.pb, off);
+    o.ra = __shfl_down(best.ra, off);
+    o.rb = __shfl_down(best.rb, off);
+    keep_better(best, o);
+  }
+  return best;
+}
+
 // q of the pairs (pa, pb), pb > pa, of row pa for the lane's columns
-// pb = pa + 1 + lane + 64k, folded into best. Four columns per lane are loaded
-// before any is used (act/ids, then D and T through them), so the two
-// dependent loads of a column overlap those of the next three.
+// pb = pa + 1 + lane + 64k, folded into best. T is read by position (Tp, in
+// the active order), so only D goes through the storage row act[pb]. Four
+// columns per lane are loaded before any is used, so the dependent loads of a
+// column overlap those of the next three.
 __device__ __forceinline__ void scan_row(Best& best, int pa, int m, int R, double nm2, const int* act,
-                                         const int* ids, const float* D, const double* T, int ld,
+                                         const int* ids, const float* D, const double* Tp, int ld,
                                          int lane) {
   const int ra = act[pa], ia = ids[pa];
-  const double Ta = T[ra];
+  const double Ta = Tp[pa];
   const float* Drow = D + (int64_t)ra * ld;
+  // the row's best in scalars (a Best assigned per candidate stays in memory)
+  double bq = DBL_MAX, bTb = 0.0;
+  float bd = 0.0f;
+  int bJ = -1, bpb = -1, brb = -1;
   auto fold = [&](int pb, int rb, int ib, float dv, double Tb) {
     const double dq = (double)dv * nm2;
     // pair created by an insertion after the last rebuild: storage order
     const bool swap = (ib >= R) && (rb < ra);
     const double q = swap ? (dq - Tb) - Ta : (dq - Ta) - Tb;
-    Best c;
-    c.q = q;
-    c.I = ia;
-    c.J = ib;
-    c.pa = pa;
-    c.pb = pb;
-    if (better(c, best)) best = c;
+    // within the row I = ia is fixed: the larger J wins a tie
+    const bool take = q < bq || (q == bq && ib > bJ);
+    bq = take ? q : bq;
+    bJ = take ? ib : bJ;
+    bpb = take ? pb : bpb;
+    brb = take ? rb : brb;
+    bd = take ? dv : bd;
+    bTb = take ? Tb : bTb;
   };
   int pb = pa + 1 + lane;
   for (; pb + 3 * 64 < m; pb += 4 * 64) {
     int rb[4], ib[4];
+    double tb[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       rb[k] = act[pb + 64 * k];
       ib[k] = ids[pb + 64 * k];
+      tb[k] = Tp[pb + 64 * k];
     }
     float dv[4];
-    double tb[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      dv[k] = Drow[rb[k]];
-      tb[k] = T[rb[k]];
-    }
+    for (int k = 0; k < 4; ++k) dv[k] = Drow[rb[k]];
 #pragma unroll
     for (int k = 0; k < 4; ++k) fold(pb + 64 * k, rb[k], ib[k], dv[k], tb[k]);
   }
   for (; pb < m; pb += 64) {
     const int rb = act[pb];
-    fold(pb, rb, ids[pb], Drow[rb], T[rb]);
+    fold(pb, rb, ids[pb], Drow[rb], Tp[pb]);
+  }
+  if (bJ >= 0) {
+    Best c;
+    c.q = bq;
+    c.I = ia;
+    c.J = bJ;
+    c.pa = pa;
+    c.pb = bpb;
+    c.ra = ra;
+    c.rb = brb;
+    c.d = bd;
+    c.Ta = Ta;
+    c.Tb = bTb;
+    keep_better(best, c);
   }
 }
 
@@ -146,11 +211,195 @@ __device__ __forceinline__ double new_row_sum(const float* rowv, const float* Dr
   return s;
 }
 
+// Exponent of the lowest set bit of a nonzero float: x is an odd integer
+// times 2^k.
+__device__ __forceinline__ int lsb_exp(float x) {
+  const unsigned int b = __float_as_uint(x);
+  const int e = (int)((b >> 23) & 0xFFu);
+  const unsigned int m = b & 0x7FFFFFu;
+  return e == 0 ? -149 + __builtin_ctz(m) : e - 150 + __builtin_ctz(m | 0x800000u);
+}
+
+struct SumRed {
+  double sum[NJ_THREADS / 64], mag[NJ_THREADS / 64];
+  int k[NJ_THREADS / 64];
+  double result;
+};
+
+// T of the new node by the whole workgroup: the fp64 sum of the new row in
+// position order, positions pa and pb left out. Each thread adds its own
+// positions' terms as it writes them (sum, sum of magnitudes, least exponent
+// of their lowest set bits); when every term is an integer multiple of
+// 2^kmin and the sum of magnitudes stays below 2^(52+kmin), every partial sum
+// of every order of additions is an exact multiple of 2^kmin below 2^53 of
+// them, so the threads' and the tree's sums equal the sum taken in position
+// order, bit for bit (an exact zero comes out +0 either way, every run
+// starting from +0). Otherwise one thread adds in position order. Returns the
+// value to every thread (ends after a barrier).
+struct RowSum {
+  double sum = 0.0, mag = 0.0;
+  int kmin = INT_MAX;
+  __device__ __forceinline__ void add(float x) {
+    if (x != 0.0f) {
+      const int k = lsb_exp(x);
+      kmin = k < kmin ? k : kmin;
+    }
+    sum += (double)x;
+    mag += fabs((double)x);
+  }
+};
+
+__device__ __forceinline__ double new_row_sum_finish(SumRed& r, RowSum t, const float* rowv,
+                                                     const float* Drow, const int* act, int m, int pa,
+                                                     int pb) {
+  const int tid = threadIdx.x;
+  for (int off = 32; off > 0; off >>= 1) {
+    t.sum += __shfl_down(t.sum, off);
+    t.mag += __shfl_down(t.mag, off);
+    const int ko = __shfl_down(t.kmin, off);
+    t.kmin = ko < t.kmin ? ko : t.kmin;
+  }
+  if ((tid & 63) == 0) {
+    r.sum[tid >> 6] = t.sum;
+    r.mag[tid >> 6] = t.mag;
+    r.k[tid >> 6] = t.kmin;
+  }
+  __syncthreads();  // (also orders the new row's LDS and global writes before the fallback)
+  if (tid < 64) {
+    const int nw = (int)(blockDim.x / 64);
+    double s = tid < nw ? r.sum[tid] : 0.0, g = tid < nw ? r.mag[tid] : 0.0;
+    int k = tid < nw ? r.k[tid] : INT_MAX;
+    for (int off = 32; off > 0; off >>= 1) {
+      s += __shfl_down(s, off);
+      g += __shfl_down(g, off);
+      const int ko = __shfl_down(k, off);
+      k = ko < k ? ko : k;
+    }
+    if (tid == 0) {
+      // (all terms zero: k stays INT_MAX and the sum is +0)
+      const bool exact = k == INT_MAX || g < ldexp(1.0, 52 + k);
+      r.result = exact ? s : new_row_sum(rowv, Drow, act, m, pa, pb);
+    }
+  }
+  __syncthreads();
+  return r.result;
+}
+
+// Shared memory of a workgroup's candidate reduction, and of one that runs joins.
+struct BestLds {
+  Best red[NJ_THREADS / 64];
+  Best win;
+};
+struct JoinLds {
+  float rowv[kNjLdsRow];
+  BestLds b;
+  SumRed sred;
+};
+
+// The workgroup's best of its waves' bests (every thread gets it; ends after
+// a barrier).
+__device__ __forceinline__ Best workgroup_best(BestLds& L, Best best) {
+  const int tid = threadIdx.x;
+  best = wave_best(best);
+  if ((tid & 63) == 0) L.red[tid >> 6] = best;
+  __syncthreads();
+  if (tid < 64) {
+    // the waves' bests reduced by the first wave
+    Best b = tid < (int)(blockDim.x / 64) ? L.red[tid] : no_best();
+    b = wave_best(b);
+    if (tid == 0) L.win = b;
+  }
+  __syncthreads();
+  return L.win;
+}
+
+// The per-position values of a join (storage row, node id, T by position) of
+// one batch of positions, loaded before the winner is known.
+constexpr int kJoinU = 4;
+struct JoinBatch {
+  int rk[kJoinU], ik[kJoinU];
+  double tk[kJoinU];
+};
+
+__device__ __forceinline__ void join_load(JoinBatch& B, int p0, int m, const int* act, const int* ids,
+                                          const double* Tp) {
+#pragma unroll
+  for (int u = 0; u < kJoinU; ++u) {
+    const int p = p0 + u * NJ_THREADS;
+    const int pc = p < m ? p : m - 1;
+    B.rk[u] = act[pc];
+    B.ik[u] = ids[pc];
+    B.tk[u] = Tp[pc];
+  }
+}
+
+// One join (modifyMatrix, SpeciesBlock.cpp:710-842) of the winner w by the
+// whole workgroup (NJ_THREADS): the join record, the new row into storage
+// row w.ra (and its column), T of the other rows, T of the new node, and the
+// active arrays compacted into act2/ids2/Tp2 (pa and pb dropped, the new node
+// appended) in the same pass over the positions. B0: this thread's first
+// batch of positions (join_load at p0 = threadIdx.x), loaded by the caller.
+__device__ __forceinline__ void join_step(JoinLds& L, const Best& w, int m, int next, float* D, double* T,
+                                          int ld, const int* act, const int* ids, const double* Tp,
+                                          int* act2, int* ids2, double* Tp2, NjJoin* rec,
+                                          const JoinBatch& B0) {
+  const int tid = threadIdx.x;
+  const int ra = w.ra, rb = w.rb;
+  const float d = w.d;
+  if (tid == 0) {
+    rec->node = next;
+    rec->left = w.I;
+    rec->right = w.J;
+    rec->d = d;
+    rec->Ti = w.Ta;
+    rec->Tj = w.Tb;
+    rec->n = m;
+    rec->q = w.q;
+  }
+  const float* Da = D + (int64_t)ra * ld;
+  const float* Db = D + (int64_t)rb * ld;
+  JoinBatch B = B0;
+  RowSum rs;
+  for (int p0 = tid; p0 < m; p0 += kJoinU * NJ_THREADS) {
+    if (p0 != tid) join_load(B, p0, m, act, ids, Tp);
+    float dik[kJoinU], djk[kJoinU];
+#pragma unroll
+    for (int u = 0; u < kJoinU; ++u) {
+      dik[u] = Da[B.rk[u]];
+      djk[u] = Db[B.rk[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kJoinU; ++u) {
+      const int p = p0 + u * NJ_THREADS;
+      if (p >= m || p == w.pa || p == w.pb) continue;
+      const int np = p - (p > w.pa) - (p > w.pb);
+      const float tmp = ((dik[u] + djk[u]) - d) / 2;
+      const float dlt = tmp - (dik[u] + djk[u]);
+      const double t = B.tk[u] + (double)dlt;
+      T[B.rk[u]] = t;
+      act2[np] = B.rk[u];
+      ids2[np] = B.ik[u];
+      Tp2[np] = t;
+      D[(int64_t)ra * ld + B.rk[u]] = tmp;
+      D[(int64_t)B.rk[u] * ld + ra] = tmp;
+      if (m <= kNjLdsRow) L.rowv[p] = tmp;
+      rs.add(tmp);
+    }
+  }
+  if (tid == 0) {
+    act2[m - 2] = ra;
+    ids2[m - 2] = next;
+  }
+  const double tn = new_row_sum_finish(L.sred, rs, L.rowv, D + (int64_t)ra * ld, act, m, w.pa, w.pb);
+  if (tid == 0) {
+    T[ra] = tn;
+    Tp2[m - 2] = tn;
+  }
+}
+
 // The whole join loop of one matrix, run by one workgroup of NJ_THREADS.
 __device__ __forceinline__ void nj_run(const NjArgs a) {
-  __shared__ float rowv[kNjLdsRow];
-  __shared__ Best red[NJ_THREADS / 64];
-  __shared__ Best win;
+  __shared__ JoinLds L;
   __shared__ int s_m, s_R, s_steps, s_next;
   const int tid = threadIdx.x;
   const int n0 = a.n0;
@@ -160,6 +409,8 @@ __device__ __forceinline__ void nj_run(const NjArgs a) {
   int* ids = a.ids;   // node ids of the active nodes
   int* act2 = a.act + n0;  // ping-pong halves
   int* ids2 = a.ids + n0;
+  double* Tp = a.Tpos;     // T by active position, ping-pong halves
+  double* Tp2 = a.Tpos + n0;
   const int ld = a.ld;
 
   // initial row sums (buildGblRowSums): one thread per row, id order; D is
@@ -168,6 +419,7 @@ __device__ __forceinline__ void nj_run(const NjArgs a) {
     double s = 0.0;
     for (int j = 0; j < n0; ++j) s += (double)D[(int64_t)j * ld + r];
     T[r] = s;
+    Tp[r] = s;
   }
   if (tid == 0) {
     s_m = n0;
@@ -182,76 +434,15 @@ __device__ __forceinline__ void nj_run(const NjArgs a) {
     const int m = s_m, R = s_R;
     const double nm2 = (double)(m - 2);
     // ---- scan of the upper triangle (positions a < b) ----
-    Best best;
-    best.q = DBL_MAX;
-    best.J = -1;
-    best.I = -1;
-    best.pa = best.pb = -1;
+    Best best = no_best();
     // one wave per row pa (rows dealt round robin), lanes stride the columns
     const int wave = tid >> 6, lane = tid & 63;
     for (int pa = wave; pa < m - 1; pa += NJ_THREADS / 64)
-      scan_row(best, pa, m, R, nm2, act, ids, D, T, ld, lane);
-    // wave reduction then workgroup reduction
-    for (int off = 32; off > 0; off >>= 1) {
-      Best o;
-      o.q = __shfl_down(best.q, off);
-      o.J = __shfl_down(best.J, off);
-      o.I = __shfl_down(best.I, off);
-      o.pa = __shfl_down(best.pa, off);
-      o.pb = __shfl_down(best.pb, off);
-      if (better(o, best)) best = o;
-    }
-    if ((tid & 63) == 0) red[tid >> 6] = best;
-    __syncthreads();
-    if (tid == 0) {
-      Best b = red[0];
-      for (int w = 1; w < NJ_THREADS / 64; ++w)
-        if (better(red[w], b)) b = red[w];
-      win = b;
-    }
-    __syncthreads();
-    const Best w = win;
-    const int ra = act[w.pa], rb = act[w.pb];
-    const float d = D[(int64_t)ra * ld + rb];
-    if (tid == 0) {
-      NjJoin& J = a.joins[step];
-      J.node = s_next;
-      J.left = w.I;
-      J.right = w.J;
-      J.d = d;
-      J.Ti = T[ra];
-      J.Tj = T[rb];
-      J.n = m;
-      J.q = w.q;
-    }
-    __syncthreads();
-    // ---- new row (modifyMatrix) into storage row ra ----
-    for (int p = tid; p < m; p += NJ_THREADS) {
-      if (p == w.pa || p == w.pb) continue;
-      const int rk = act[p];
-      const float Dik = D[(int64_t)ra * ld + rk];
-      const float Djk = D[(int64_t)rb * ld + rk];
-      const float tmp = ((Dik + Djk) - d) / 2;
-      const float dlt = tmp - (Dik + Djk);
-      T[rk] += (double)dlt;
-      D[(int64_t)ra * ld + rk] = tmp;
-      D[(int64_t)rk * ld + ra] = tmp;
-      if (m <= kNjLdsRow) rowv[p] = tmp;
-    }
-    __syncthreads();
-    if (tid == 0) T[ra] = new_row_sum(rowv, D + (int64_t)ra * ld, act, m, w.pa, w.pb);
-    __syncthreads();
-    // ---- compact the active array: drop pa, pb, append the new node ----
-    for (int p = tid; p < m; p += NJ_THREADS) {
-      if (p == w.pa || p == w.pb) continue;
-      const int np = p - (p > w.pa) - (p > w.pb);
-      act2[np] = act[p];
-      ids2[np] = ids[p];
-    }
-    if (tid == 0) {
-      act2[m - 2] = ra;
-      ids2[m - 2] = s_next;
-    }
+      scan_row(best, pa, m, R, nm2, act, ids, D, Tp, ld, lane);
+    JoinBatch B0;
+    join_load(B0, tid, m, act, ids, Tp);
+    const Best w = workgroup_best(L.b, best);
+    join_step(L, w, m, s_next, D, T, ld, act, ids, Tp, act2, ids2, Tp2, &a.joins[step], B0);
     __syncthreads();
     {
       int* t = act;
@@ -260,6 +451,9 @@ __device__ __forceinline__ void nj_run(const NjArgs a) {
       t = ids;
       ids = ids2;
       ids2 = t;
+      double* u = Tp;
+      Tp = Tp2;
+      Tp2 = u;
     }
     // ---- step bookkeeping and the rebuild schedule ----
     const int m1 = m - 1;
@@ -270,6 +464,7 @@ __device__ __forceinline__ void nj_run(const NjArgs a) {
         double s = 0.0;
         for (int j = 0; j < m1; ++j) s += (double)D[(int64_t)act[j] * ld + r];
         T[r] = s;
+        Tp[p] = s;
       }
     }
     __syncthreads();
@@ -287,188 +482,101 @@ __device__ __forceinline__ void nj_run(const NjArgs a) {
   }
 }
 
-// ---- one launch per step, the scan over the whole device (large N) ----
+// ---- launches per step, the scan over the whole device (large N) ----
 //
 // The same steps as nj_run, split so that the O(m^2) scan of a step runs on
 // many workgroups: nj_scan_kernel (every wave takes rows pa round robin over
 // the grid, a best candidate per workgroup), then nj_join_kernel (one
-// workgroup: the winner from the partial bests, the join record, the new row,
-// T of the new node, the compacted active list, the bookkeeping), then, at
-// the steps the host's copy of the rebuild schedule names, nj_rebuild_kernel
-// (T of every active row, a thread per row). The partial bests reduce with
-// the same total order as the workgroup reduction, so the winner, and every
-// value after it, equals nj_run's.
-
-struct NjState {
-  int m, R, next, flip;  // active count, rebuild marker, next node id, active half
-};
-
-__device__ __forceinline__ Best wave_best(Best best) {
-  for (int off = 32; off > 0; off >>= 1) {
-    Best o;
-    o.q = __shfl_down(best.q, off);
-    o.J = __shfl_down(best.J, off);
-    o.I = __shfl_down(best.I, off);
-    o.pa = __shfl_down(best.pa, off);
-    o.pb = __shfl_down(best.pb, off);
-    if (better(o, best)) best = o;
-  }
-  return best;
-}
+// workgroup: the winner from the partial bests, reduced with the same total
+// order, so the winner, and every value after it, equals nj_run's; then the
+// join), then, at the steps of the rebuild schedule, which the host replays
+// from N and the ratio, nj_rebuild_kernel (T of every active row, a thread
+// per row). No workgroup waits on another; the stream orders the steps.
+// (One launch per step whose last workgroup ran the join, after an
+// agent-scope release by every workgroup, measured slower: 185 against 126
+// ms for 4,000 sequences.)
 
 // T[r] = sum over j of D(j, r) in fp64, j in storage order (all nodes active)
-__global__ void nj_rowsum_kernel(const float* __restrict__ D, int ld, int n0, double* __restrict__ T) {
+__global__ void nj_rowsum_kernel(const float* __restrict__ D, int ld, int n0, double* __restrict__ T,
+                                 double* __restrict__ Tp) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n0) return;
   double s = 0.0;
   for (int j = 0; j < n0; ++j) s += (double)D[(int64_t)j * ld + r];
   T[r] = s;
+  Tp[r] = s;
 }
 
 constexpr int NJ_SCAN_THREADS = 256;
 
-__global__ void __launch_bounds__(NJ_SCAN_THREADS) nj_scan_kernel(NjArgs a, const NjState* __restrict__ st,
+// The step's state comes from the host, which replays the schedule: m active
+// nodes, rebuild marker R, the active arrays' half `flip` (so no launch starts
+// with a dependent load of it).
+__global__ void __launch_bounds__(NJ_SCAN_THREADS) nj_scan_kernel(NjArgs a, int m, int R, int flip,
                                                                   Best* __restrict__ part) {
-  __shared__ Best red[NJ_SCAN_THREADS / 64];
-  const NjState S = *st;
-  const int m = S.m, R = S.R;
-  const int* act = a.act + (S.flip ? a.n0 : 0);
-  const int* ids = a.ids + (S.flip ? a.n0 : 0);
-  const float* D = a.D;
-  const double* T = a.T;
-  const int ld = a.ld;
+  __shared__ BestLds L;
+  const int* act = a.act + (flip ? a.n0 : 0);
+  const int* ids = a.ids + (flip ? a.n0 : 0);
+  const double* Tp = a.Tpos + (flip ? a.n0 : 0);
   const double nm2 = (double)(m - 2);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nwaves = gridDim.x * (NJ_SCAN_THREADS / 64);
-  Best best;
-  best.q = DBL_MAX;
-  best.J = -1;
-  best.I = -1;
-  best.pa = best.pb = -1;
+  Best best = no_best();
   for (int pa = blockIdx.x * (NJ_SCAN_THREADS / 64) + wave; pa < m - 1; pa += nwaves)
-    scan_row(best, pa, m, R, nm2, act, ids, D, T, ld, lane);
-  best = wave_best(best);
-  if (lane == 0) red[wave] = best;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    Best b = red[0];
-    for (int w = 1; w < NJ_SCAN_THREADS / 64; ++w)
-      if (better(red[w], b)) b = red[w];
-    part[blockIdx.x] = b;
-  }
+    scan_row(best, pa, m, R, nm2, act, ids, a.D, Tp, a.ld, lane);
+  best = workgroup_best(L, best);
+  if (threadIdx.x == 0) part[blockIdx.x] = best;
 }
 
-__global__ void __launch_bounds__(NJ_THREADS) nj_join_kernel(NjArgs a, NjState* __restrict__ st,
-                                                             const Best* __restrict__ part, int nparts,
-                                                             int step, int rebuild) {
-  __shared__ float rowv[kNjLdsRow];
-  __shared__ Best red[NJ_THREADS / 64];
-  __shared__ Best win;
+__global__ void __launch_bounds__(NJ_THREADS) nj_join_kernel(NjArgs a, const Best* __restrict__ part,
+                                                             int nparts, int step, int m, int flip) {
+  __shared__ JoinLds L;
   const int tid = threadIdx.x;
-  const NjState S = *st;
-  const int n0 = a.n0, m = S.m, ld = a.ld;
-  float* D = a.D;
-  double* T = a.T;
-  const int* act = a.act + (S.flip ? n0 : 0);
-  const int* ids = a.ids + (S.flip ? n0 : 0);
-  int* act2 = a.act + (S.flip ? 0 : n0);
-  int* ids2 = a.ids + (S.flip ? 0 : n0);
-  Best best;
-  best.q = DBL_MAX;
-  best.J = -1;
-  best.I = -1;
-  best.pa = best.pb = -1;
-  for (int k = tid; k < nparts; k += NJ_THREADS)
-    if (better(part[k], best)) best = part[k];
-  best = wave_best(best);
-  if ((tid & 63) == 0) red[tid >> 6] = best;
-  __syncthreads();
-  if (tid == 0) {
-    Best b = red[0];
-    for (int w = 1; w < NJ_THREADS / 64; ++w)
-      if (better(red[w], b)) b = red[w];
-    win = b;
-  }
-  __syncthreads();
-  const Best w = win;
-  const int ra = act[w.pa], rb = act[w.pb];
-  const float d = D[(int64_t)ra * ld + rb];
-  if (tid == 0) {
-    NjJoin& J = a.joins[step];
-    J.node = S.next;
-    J.left = w.I;
-    J.right = w.J;
-    J.d = d;
-    J.Ti = T[ra];
-    J.Tj = T[rb];
-    J.n = m;
-    J.q = w.q;
-  }
-  __syncthreads();
-  for (int p = tid; p < m; p += NJ_THREADS) {
-    if (p == w.pa || p == w.pb) continue;
-    const int rk = act[p];
-    const float Dik = D[(int64_t)ra * ld + rk];
-    const float Djk = D[(int64_t)rb * ld + rk];
-    const float tmp = ((Dik + Djk) - d) / 2;
-    const float dlt = tmp - (Dik + Djk);
-    T[rk] += (double)dlt;
-    D[(int64_t)ra * ld + rk] = tmp;
-    D[(int64_t)rk * ld + ra] = tmp;
-    if (m <= kNjLdsRow) rowv[p] = tmp;
-  }
-  __syncthreads();
-  if (tid == 0) T[ra] = new_row_sum(rowv, D + (int64_t)ra * ld, act, m, w.pa, w.pb);
-  for (int p = tid; p < m; p += NJ_THREADS) {
-    if (p == w.pa || p == w.pb) continue;
-    const int np = p - (p > w.pa) - (p > w.pb);
-    act2[np] = act[p];
-    ids2[np] = ids[p];
-  }
-  if (tid == 0) {
-    act2[m - 2] = ra;
-    ids2[m - 2] = S.next;
-    NjState o = S;
-    if (rebuild) o.R = S.next + 1;
-    o.m = m - 1;
-    o.next = S.next + 1;
-    o.flip = S.flip ^ 1;
-    *st = o;
-  }
+  const int n0 = a.n0;
+  const int* act = a.act + (flip ? n0 : 0);
+  const int* ids = a.ids + (flip ? n0 : 0);
+  const double* Tp = a.Tpos + (flip ? n0 : 0);
+  // the positions' values and the candidates load together
+  JoinBatch B0;
+  join_load(B0, tid, m, act, ids, Tp);
+  Best best = no_best();
+  for (int k = tid; k < nparts; k += NJ_THREADS) keep_better(best, part[k]);
+  const Best w = workgroup_best(L.b, best);
+  join_step(L, w, m, n0 + step, a.D, a.T, a.ld, act, ids, Tp, a.act + (flip ? 0 : n0),
+            a.ids + (flip ? 0 : n0), a.Tpos + (flip ? 0 : n0), &a.joins[step], B0);
 }
 
-// T of every active row after a rebuild step (the active list just compacted)
-__global__ void nj_rebuild_kernel(NjArgs a, const NjState* __restrict__ st) {
-  const NjState S = *st;
-  const int m1 = S.m;
-  const int* act = a.act + (S.flip ? a.n0 : 0);
+// T of every active row after a rebuild step (m1 active nodes in half flip)
+__global__ void nj_rebuild_kernel(NjArgs a, int m1, int flip) {
+  const int* act = a.act + (flip ? a.n0 : 0);
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= m1) return;
   const int r = act[p];
   double s = 0.0;
   for (int j = 0; j < m1; ++j) s += (double)a.D[(int64_t)act[j] * a.ld + r];
   a.T[r] = s;
+  a.Tpos[(flip ? a.n0 : 0) + p] = s;
 }
 
-hipError_t run_nj_steps(const NjArgs& a, void* state, void* parts, int nparts, hipStream_t st) {
+hipError_t run_nj_steps(const NjArgs& a, void* parts, int nparts, hipStream_t st) {
   const int n0 = a.n0;
-  NjState init{n0, n0, n0, 0};
-  hipError_t e = hipMemcpyAsync(state, &init, sizeof init, hipMemcpyHostToDevice, st);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(nj_rowsum_kernel, dim3((n0 + 255) / 256), dim3(256), 0, st, a.D, a.ld, n0, a.T);
+  hipLaunchKernelGGL(nj_rowsum_kernel, dim3((n0 + 255) / 256), dim3(256), 0, st, a.D, a.ld, n0, a.T,
+                     a.Tpos);
   // the rebuild schedule of nj_run (SpeciesBlock.cpp:864,1225-1247)
   int steps = (int)((float)n0 * a.ratio);
   steps = steps > 200 ? steps : 200;
+  int R = n0;
+  hipError_t e;
   for (int step = 0; step < n0 - 1; ++step) {
-    const int m = n0 - step;
-    const bool rebuild = steps == 0;
-    hipLaunchKernelGGL(nj_scan_kernel, dim3(nparts), dim3(NJ_SCAN_THREADS), 0, st, a, (const NjState*)state,
+    const int m = n0 - step, flip = step & 1;
+    hipLaunchKernelGGL(nj_scan_kernel, dim3(nparts), dim3(NJ_SCAN_THREADS), 0, st, a, m, R, flip,
                        (Best*)parts);
-    hipLaunchKernelGGL(nj_join_kernel, dim3(1), dim3(NJ_THREADS), 0, st, a, (NjState*)state,
-                       (const Best*)parts, nparts, step, rebuild ? 1 : 0);
-    if (rebuild) {
+    hipLaunchKernelGGL(nj_join_kernel, dim3(1), dim3(NJ_THREADS), 0, st, a, (const Best*)parts, nparts,
+                       step, m, flip);
+    if (steps == 0) {
       const int m1 = m - 1;
-      hipLaunchKernelGGL(nj_rebuild_kernel, dim3((m1 + 255) / 256), dim3(256), 0, st, a, (const NjState*)state);
+      hipLaunchKernelGGL(nj_rebuild_kernel, dim3((m1 + 255) / 256), dim3(256), 0, st, a, m1, flip ^ 1);
+      R = n0 + step + 1;
       steps = (m1 < 200) ? INT_MAX : (int)((float)m1 * a.ratio);
     } else {
       steps -= 1;
@@ -478,7 +586,6 @@ hipError_t run_nj_steps(const NjArgs& a, void* state, void* parts, int nparts, h
   return hipSuccess;
 }
 
-int64_t nj_state_bytes() { return sizeof(NjState); }
 int64_t nj_part_bytes(int nparts) { return (int64_t)sizeof(Best) * nparts; }
 
 __global__ void __launch_bounds__(NJ_THREADS) nj_kernel(NjArgs a) { nj_run(a); }
