@@ -74,7 +74,7 @@ static int run_nj(mst_ctx* ctx, int n, float ratio, mst_join* joins) {
   a.ratio = ratio;
   if (n >= kNjStepsMinN) {
     // large N: one launch per step, the scan over the whole device
-    const int nparts = ctx->num_cus * 2;
+    const int nparts = ctx->num_cus * 4;  // 16 waves per CU
     rc = ctx->ensure(&ctx->d_nj_parts, &ctx->nj_parts_cap, nj_part_bytes(nparts));
     if (rc) return rc;
     CK(run_nj_steps(a, ctx->d_nj_parts, nparts, ctx->stream));

@@ -93,66 +93,92 @@ __device__ __forceinline__ Best wave_best(Best best) {
   return best;
 }
 
-// q of the pairs (pa, pb), pb > pa, of row pa for the lane's columns
-// pb = pa + 1 + lane + 64k, folded into best. T is read by position (Tp, in
-// the active order), so only D goes through the storage row act[pb]. Four
-// columns per lane are loaded before any is used, so the dependent loads of a
-// column overlap those of the next three.
-__device__ __forceinline__ void scan_row(Best& best, int pa, int m, int R, double nm2, const int* act,
-                                         const int* ids, const float* D, const double* Tp, int ld,
-                                         int lane) {
-  const int ra = act[pa], ia = ids[pa];
-  const double Ta = Tp[pa];
-  const float* Drow = D + (int64_t)ra * ld;
-  // the row's best in scalars (a Best assigned per candidate stays in memory)
-  double bq = DBL_MAX, bTb = 0.0;
+// The scan's work in balanced units. The upper triangle's rows pa = 0 ..
+// m-2 (row pa: columns pa+1 .. m-1) are folded in pairs, row f with row
+// m-2-f, into ceil((m-1)/2) rows of m columns (the middle row of an odd count
+// alone), and each folded row is cut into segments of kScanSeg columns: unit
+// u is segment u % ceil(m / kScanSeg) of folded row u / ceil(m / kScanSeg).
+// Units are dealt round robin to the waves, so no wave sweeps much more than
+// another (a wave per whole row left the first rows' waves with three times
+// the average).
+constexpr int kScanSeg = 512;
+
+__device__ __forceinline__ int scan_units(int m) {
+  return ((m - 1 + 1) / 2) * ((m + kScanSeg - 1) / kScanSeg);
+}
+
+__device__ __forceinline__ void scan_unit(Best& best, int u, int m, int R, double nm2, const int* act,
+                                          const int* ids, const float* D, const double* Tp, int ld,
+                                          int lane) {
+  const int spr = (m + kScanSeg - 1) / kScanSeg;
+  const int f = u / spr, seg = u - f * spr;
+  const int rA = f, rB = m - 2 - f;
+  const int nA = m - 1 - f, n = rB > rA ? m : nA;  // columns of the folded row
+  const int c0 = seg * kScanSeg, c1 = c0 + kScanSeg < n ? c0 + kScanSeg : n;
+  if (c0 >= c1) return;
+  const int rBc = rB > rA ? rB : rA;
+  const int raA = act[rA], raB = act[rBc], iaA = ids[rA], iaB = ids[rBc];
+  const double TaA = Tp[rA], TaB = Tp[rBc];
+  const float* DA = D + (int64_t)raA * ld;
+  const float* DB = D + (int64_t)raB * ld;
+  // the unit's best in scalars (a Best assigned per candidate stays in memory)
+  double bq = DBL_MAX, bTa = 0.0, bTb = 0.0;
   float bd = 0.0f;
-  int bJ = -1, bpb = -1, brb = -1;
-  auto fold = [&](int pb, int rb, int ib, float dv, double Tb) {
+  int bJ = -1, bI = -1, bpa = -1, bpb = -1, bra = -1, brb = -1;
+  auto fold = [&](bool inA, int pb, int rb, int ib, float dv, double Tb) {
+    const int ra = inA ? raA : raB, ia = inA ? iaA : iaB;
+    const double Ta = inA ? TaA : TaB;
     const double dq = (double)dv * nm2;
     // pair created by an insertion after the last rebuild: storage order
     const bool swap = (ib >= R) && (rb < ra);
     const double q = swap ? (dq - Tb) - Ta : (dq - Ta) - Tb;
-    // within the row I = ia is fixed: the larger J wins a tie
-    const bool take = q < bq || (q == bq && ib > bJ);
+    const bool take = q < bq || (q == bq && (ib > bJ || (ib == bJ && ia > bI)));
     bq = take ? q : bq;
     bJ = take ? ib : bJ;
+    bI = take ? ia : bI;
+    bpa = take ? (inA ? rA : rB) : bpa;
     bpb = take ? pb : bpb;
+    bra = take ? ra : bra;
     brb = take ? rb : brb;
     bd = take ? dv : bd;
+    bTa = take ? Ta : bTa;
     bTb = take ? Tb : bTb;
   };
-  int pb = pa + 1 + lane;
-  for (; pb + 3 * 64 < m; pb += 4 * 64) {
-    int rb[4], ib[4];
+  // four columns per lane in flight: T, ids and storage rows by position,
+  // then D through the storage rows
+  for (int c = c0 + lane; c < c1; c += 4 * 64) {
+    int pb[4], rb[4], ib[4];
+    bool inA[4], ok[4];
     double tb[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      rb[k] = act[pb + 64 * k];
-      ib[k] = ids[pb + 64 * k];
-      tb[k] = Tp[pb + 64 * k];
+      const int cc = c + 64 * k;
+      ok[k] = cc < c1;
+      inA[k] = cc < nA;
+      pb[k] = inA[k] ? f + 1 + cc : rB + 1 + (cc - nA);
+      const int pc = ok[k] ? pb[k] : m - 1;
+      rb[k] = act[pc];
+      ib[k] = ids[pc];
+      tb[k] = Tp[pc];
     }
     float dv[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) dv[k] = Drow[rb[k]];
+    for (int k = 0; k < 4; ++k) dv[k] = (inA[k] ? DA : DB)[rb[k]];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) fold(pb + 64 * k, rb[k], ib[k], dv[k], tb[k]);
-  }
-  for (; pb < m; pb += 64) {
-    const int rb = act[pb];
-    fold(pb, rb, ids[pb], Drow[rb], Tp[pb]);
+    for (int k = 0; k < 4; ++k)
+      if (ok[k]) fold(inA[k], pb[k], rb[k], ib[k], dv[k], tb[k]);
   }
   if (bJ >= 0) {
     Best c;
     c.q = bq;
-    c.I = ia;
+    c.I = bI;
     c.J = bJ;
-    c.pa = pa;
+    c.pa = bpa;
     c.pb = bpb;
-    c.ra = ra;
+    c.ra = bra;
     c.rb = brb;
     c.d = bd;
-    c.Ta = Ta;
+    c.Ta = bTa;
     c.Tb = bTb;
     keep_better(best, c);
   }
@@ -437,8 +463,9 @@ __device__ __forceinline__ void nj_run(const NjArgs a) {
     Best best = no_best();
     // one wave per row pa (rows dealt round robin), lanes stride the columns
     const int wave = tid >> 6, lane = tid & 63;
-    for (int pa = wave; pa < m - 1; pa += NJ_THREADS / 64)
-      scan_row(best, pa, m, R, nm2, act, ids, D, Tp, ld, lane);
+    const int units = scan_units(m);
+    for (int u = wave; u < units; u += NJ_THREADS / 64)
+      scan_unit(best, u, m, R, nm2, act, ids, D, Tp, ld, lane);
     JoinBatch B0;
     join_load(B0, tid, m, act, ids, Tp);
     const Best w = workgroup_best(L.b, best);
@@ -522,8 +549,9 @@ __global__ void __launch_bounds__(NJ_SCAN_THREADS) nj_scan_kernel(NjArgs a, int 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nwaves = gridDim.x * (NJ_SCAN_THREADS / 64);
   Best best = no_best();
-  for (int pa = blockIdx.x * (NJ_SCAN_THREADS / 64) + wave; pa < m - 1; pa += nwaves)
-    scan_row(best, pa, m, R, nm2, act, ids, a.D, Tp, a.ld, lane);
+  const int units = scan_units(m);
+  for (int u = blockIdx.x * (NJ_SCAN_THREADS / 64) + wave; u < units; u += nwaves)
+    scan_unit(best, u, m, R, nm2, act, ids, a.D, Tp, a.ld, lane);
   best = workgroup_best(L, best);
   if (threadIdx.x == 0) part[blockIdx.x] = best;
 }
