@@ -17,6 +17,7 @@
 using namespace mst;
 
 int noalloc_init_device(int device, void** ptr);  // capi_noalloc.hip
+bool flag16_usable(const mst_ctx* ctx);            // capi_align.hip
 
 namespace {
 
@@ -116,6 +117,7 @@ extern "C" int mst_create(int gap_open, int gap_ext, const int8_t* subst, int de
   CK(hipMalloc(&ctx->d_S, 576));
   CK(hipMemcpy(ctx->d_S, ctx->S, 576, hipMemcpyHostToDevice));
   CK(init_encode_table());
+  ctx->flag16 = flag16_usable(ctx);
   {
     const int rc = noalloc_init_device(device, &ctx->d_scratch);
     if (rc) return ctx->fail(rc, "hipGetSymbolAddress", "no-allocation scratch array");
@@ -134,7 +136,7 @@ mst_ctx::~mst_ctx() {
   }
   void* ps[] = {d_S,    d_pairs,    d_bufs,  d_mat, d_pscore, d_nj,    d_nj_T,
                 d_nj_act, d_nj_joins, d_flags, d_ops, d_jobs,   d_jscore, d_acounter,
-                d_tiles,  d_trows,    d_tstate, d_nj_parts};
+                d_tiles,  d_trows,    d_tstate, d_nj_parts, d_f16, d_tab16};
   for (void* p : ps)
     if (p) hipFree(p);
   for (hipEvent_t e : launch_events) hipEventDestroy(e);
@@ -142,6 +144,7 @@ mst_ctx::~mst_ctx() {
   if (ev1) hipEventDestroy(ev1);
   for (hipStream_t st : bstream)
     if (st) hipStreamDestroy(st);
+  if (astream) hipStreamDestroy(astream);
   for (hipEvent_t e : bevent)
     if (e) hipEventDestroy(e);
   if (stream) hipStreamDestroy(stream);

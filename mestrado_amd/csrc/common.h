@@ -9,6 +9,7 @@
 
 // DPP controls of the GFX9 family (wave-wide shift by one lane).
 #define DPP_WAVE_SHR1 0x138
+#define DPP_WAVE_ROR1 0x13C
 
 namespace mst {
 

@@ -93,7 +93,7 @@ struct mst_ctx {
   size_t nj_parts_cap = 0;
 
   // traceback working set
-  int64_t flag_budget_bytes = (int64_t)4 << 30;
+  int64_t flag_budget_bytes = (int64_t)16 << 30;  // flags and score tables of one alignment batch
   void* d_flags = nullptr;
   size_t flags_cap = 0;
   void* d_ops = nullptr;
@@ -122,4 +122,11 @@ struct mst_ctx {
   void* d_tstate = nullptr;
   size_t tstate_cap = 0;
   int ensure(void** p, size_t* cap, size_t bytes);
+  // packed flag kernel: items and table strips, score tables of sequence jobs
+  bool flag16 = true;  // (set by mst_create when the substitution bytes allow it)
+  void* d_f16 = nullptr;
+  size_t f16_cap = 0;
+  void* d_tab16 = nullptr;
+  size_t tab16_cap = 0;
+  hipStream_t astream = nullptr;  // the int32 flag kernels beside the packed one (default priority)
 };

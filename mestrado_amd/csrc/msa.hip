@@ -411,6 +411,8 @@ hipError_t MergeEngine::enqueue(const MergeSpec* m, int count, const int8_t* d_S
     AlignJob& j = h_jobs_[q.job];
     j.n = q.nX;
     j.m = q.nY;
+    j.fmt = FMT_NIBBLE;
+    j.k = 8;
     j.x_off = j.y_off = 0;
     j.tab_off = q.tab;
     j.flag_off = fl / 4;

@@ -8,9 +8,15 @@ namespace mst {
 
 enum { SRC_SEQ = 0, SRC_TAB = 1 };
 enum { OP_DIAG = 0, OP_LEFT = 1, OP_UP = 2 };  // LEFT: X residue vs gap; UP: gap vs Y
+// Flag word of a lane and step (8 rows): FMT_NIBBLE, row i's nibble (LEFT, UP,
+// contUp, contLeft from bit 3 down) at bits 4i..4i+3; FMT_PLANES, bit planes
+// of the four sign flags of the packed kernel (nw_flag16.hip).
+enum { FMT_NIBBLE = 0, FMT_PLANES = 1 };
 
 struct AlignJob {
   int n, m;            // columns (X) and rows (Y)
+  int fmt;             // flag format: FMT_NIBBLE (int32 kernels) or FMT_PLANES (flag16_kernel)
+  int k;               // rows per lane of the flag layout (strips of 64 k rows; 8 for FMT_NIBBLE)
   int64_t x_off, y_off;  // residue offsets (SRC_SEQ)
   int64_t tab_off;     // byte offset of the score table (SRC_TAB), 8-byte aligned
   int64_t flag_off;    // word offset of the flag block
@@ -89,6 +95,44 @@ void launch_tile_diagonals(int src, TileArgs a, const TileJob* d_tiles, const st
 void launch_align_pipe(int src, int W, const AlignArgs& a, int grid, hipStream_t st);
 int64_t align_strips(int m);  // 512-row strips of an m-row job
 void launch_traceback(const TraceArgs& a, hipStream_t st);
-int64_t flag_words(int n, int m);  // flag block size of an n x m job
+
+// Packed 16-bit flag kernel (nw_flag16.hip): an item is two jobs of the same
+// rows per lane k (b < 0: job a alone) run in the two halves of one wave,
+// strips of 64 k rows in step, or one job with two consecutive strips in the
+// halves, the second 64 columns behind; scores from a byte table (AlignJob::tab_off,
+// 8 bytes per lane and step, the first k used) in the flag layout; flags in
+// FMT_PLANES. k: 1..8; items of every k in one launch.
+// Only for jobs whose DP fits 16 bits (pk16_bound) and whose table bytes
+// S + go + ge lie in [0, 255]. bufs: 2 * bufcap int2 per workgroup.
+struct F16Item {
+  int a, b;
+  int k;     // rows per lane of both jobs (1..8)
+  int mode;  // 0: jobs a and b strip by strip (b < 0: a alone); 1: job a two strips per pass
+};
+struct F16Args {
+  const AlignJob* jobs;
+  const F16Item* items;
+  int64_t nitems;
+  const uint8_t* tab;
+  uint32_t* flags;
+  int32_t* scores;
+  int2* bufs;
+  int64_t bufcap;
+  int go, ge;
+};
+void launch_flag16(const F16Args& a, int grid, hipStream_t st);
+// score tables of sequence jobs: one (job, strip) pair per entry of strips
+// (each job's own k)
+void launch_seq_table(const AlignJob* jobs, const int2* strips, int nstrips, const int8_t* seqs,
+                      const int8_t* subst, int bonus, uint8_t* tab, hipStream_t st);
+// rows per lane of a packed job of m rows: the fewest strips of at most 512
+// rows, then the fewest rows per lane that cover m with them
+inline int flag16_k(int m) {
+  const int s = (m + 511) / 512 > 0 ? (m + 511) / 512 : 1;
+  const int k = (m + 64 * s - 1) / (64 * s);
+  return k < 1 ? 1 : k;
+}
+// flag block size of an n x m job, strips of 64 k rows
+int64_t flag_words(int n, int m, int k = 8);
 
 }  // namespace mst

@@ -687,6 +687,7 @@ __global__ void __launch_bounds__(MST_WAVE) align_kernel(AlignArgs a) {
 
 constexpr int TB_GROUPS = 8;  // 8-row groups of a tile
 
+// (y: the row, or any value of its sign)
 __device__ __forceinline__ int boundary_nib(int x, int y) {
   if (x < 0 && y < 0) return 0;
   if (y < 0) return 9;  // LEFT + contLeft
@@ -700,16 +701,19 @@ __global__ void __launch_bounds__(MST_WAVE) traceback_kernel(TraceArgs a) {
     const AlignJob job = a.jobs[j];
     const int n = job.n, m = job.m;
     const int Tn = n + MST_WAVE - 1;
+    const int kr = job.k;  // rows per lane of the job's flag layout
     const uint32_t* fl = a.flags + job.flag_off;
     uint8_t* out = a.ops + job.ops_off;
-    // tile: columns [cx0, cx0 + 63], rows [cy0, cy1] (groups from cy0 / 8)
-    int cx0 = 1, cy0 = 1, cy1 = 0;
-    auto nib = [&](int x, int y) -> int {
-      if (x < 0 || y < 0) return boundary_nib(x, y);
-      if (x < cx0 || x > cx0 + MST_WAVE - 1 || y < cy0 || y > cy1) {
-        const int s = y / ASH;
-        const int gs = s * (ASH / AK);  // first group of the strip
-        const int g = y / AK;
+    // The walk's row y is kept as (g, r): group g = y / kr (a flag kernel
+    // lane's rows; 64 groups a strip) and row r = y % kr in it, stepped by
+    // hand so that no step divides. Tile: columns [cx0, cx0 + 63], groups
+    // [cg0, cg1].
+    int cx0 = 1, cg0 = 1, cg1 = 0;
+    auto nib = [&](int x, int g, int r) -> int {
+      if (x < 0 || g < 0) return boundary_nib(x, g);
+      if (x < cx0 || x > cx0 + MST_WAVE - 1 || g < cg0 || g > cg1) {
+        const int gs = g & ~(MST_WAVE - 1);  // first group of the strip
+        const int s = g / MST_WAVE;
         const int g0 = max(gs, g - (TB_GROUPS - 1));
         const int c = x - (MST_WAVE - 1) + lane;
         __syncthreads();  // the previous tile has been read
@@ -725,17 +729,23 @@ __global__ void __launch_bounds__(MST_WAVE) traceback_kernel(TraceArgs a) {
         }
         __syncthreads();
         cx0 = x - (MST_WAVE - 1);
-        cy0 = g0 * AK;
-        cy1 = g * AK + AK - 1;
+        cg0 = g0;
+        cg1 = g;
       }
-      const int gi = y / AK - cy0 / AK;
       // every lane reads the same word: keep the walk in scalar registers
-      const uint32_t w = __builtin_amdgcn_readfirstlane(tile[gi][x - cx0]);
-      return (int)((w >> (4 * (y % AK))) & 15u);
+      const uint32_t w = __builtin_amdgcn_readfirstlane(tile[g - cg0][x - cx0]);
+      if (job.fmt == FMT_NIBBLE) return (int)((w >> (4 * r)) & 15u);
+      // bit planes: s1 (diagonal not a maximum), s2 (E > F), s4 (F opened),
+      // s3 (E opened) of row r
+      const int s1 = (w >> r) & 1, s2 = (w >> (8 + r)) & 1;
+      const int s4 = (w >> (16 + r)) & 1, s3 = (w >> (24 + r)) & 1;
+      const int eqd = s1 ^ 1;
+      const int left = eqd | s2, up = eqd | (s2 ^ 1);
+      return (left << 3) | (up << 2) | ((s4 ^ 1) << 1) | (s3 ^ 1);
     };
-    int x = n - 1, y = m - 1;
+    int x = n - 1, g = (m - 1) / kr, r = (m - 1) % kr;
     int prev = 12;
-    int cur = nib(x, y);
+    int cur = nib(x, g, r);
     int k = 0;
     while (cur & 12) {
       // the state machine as selects (scalar, no branches): continue an
@@ -749,9 +759,13 @@ __global__ void __launch_bounds__(MST_WAVE) traceback_kernel(TraceArgs a) {
       if (lane == 0) out[k] = (uint8_t)(todo == 12 ? OP_DIAG : (todo == 8 ? OP_LEFT : OP_UP));
       ++k;
       x -= (todo >> 3) & 1;  // 8 and 12 move left
-      y -= (todo >> 2) & 1;  // 4 and 12 move up
+      if (todo & 4) {        // 4 and 12 move up
+        const bool wrap = r == 0;
+        r = wrap ? kr - 1 : r - 1;
+        g -= wrap ? 1 : 0;
+      }
       prev = todo | (cur & 3);
-      cur = nib(x, y);
+      cur = nib(x, g, r);
     }
     if (lane == 0) a.lens[j] = k;
     __syncthreads();  // the tile is rebuilt for the next job
@@ -821,8 +835,8 @@ void launch_traceback(const TraceArgs& a, hipStream_t st) {
   hipLaunchKernelGGL(traceback_kernel, dim3((unsigned)grid), dim3(MST_WAVE), 0, st, a);
 }
 
-int64_t flag_words(int n, int m) {
-  const int64_t nstrips = (m + ASH - 1) / ASH;
+int64_t flag_words(int n, int m, int k) {
+  const int64_t nstrips = (m + MST_WAVE * k - 1) / (MST_WAVE * k);
   return nstrips * (int64_t)(n + MST_WAVE - 1) * MST_WAVE;
 }
 
