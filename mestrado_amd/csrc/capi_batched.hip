@@ -611,20 +611,20 @@ extern "C" int mst_tune_set(mst_ctx* ctx, const char* key, int value) {
   if (!ctx || !key) return MST_E_ARG;
   int* slot = tune_slot(ctx, key);
   if (!slot) return ctx->fail(MST_E_ARG, "mst_tune_set", "unknown key");
-  if (!strcmp(key, "chunk") && (value < 1 || value > 4096))
-    return ctx->fail(MST_E_ARG, "mst_tune_set", "chunk out of range 1..4096");
+  if (!strcmp(key, "chunk") && (value < 1 || value > kStreamSeqs))
+    return ctx->fail(MST_E_ARG, "mst_tune_set", "chunk out of range 1..64");
   if (!strcmp(key, "k5min") && (value < 1 || value > 3))
     return ctx->fail(MST_E_ARG, "mst_tune_set", "k5min out of range 1..3");
   if (!strcmp(key, "stack") && value != 0 && value != 1)
     return ctx->fail(MST_E_ARG, "mst_tune_set", "stack is 0 or 1");
   if (!strcmp(key, "s6p2") && value != 0 && value != 1)
     return ctx->fail(MST_E_ARG, "mst_tune_set", "s6p2 is 0 or 1");
-  if (!strcmp(key, "chunk_k9") && (value < 0 || value > 4096))
-    return ctx->fail(MST_E_ARG, "mst_tune_set", "chunk_k9 out of range 0..4096");
-  if (!strcmp(key, "chunk6") && (value < 0 || value > 4096))
-    return ctx->fail(MST_E_ARG, "mst_tune_set", "chunk6 out of range 0..4096");
-  if (!strcmp(key, "chunk_w3") && (value < 0 || value > 4096))
-    return ctx->fail(MST_E_ARG, "mst_tune_set", "chunk_w3 out of range 0..4096");
+  if (!strcmp(key, "chunk_k9") && (value < 0 || value > kStreamSeqs))
+    return ctx->fail(MST_E_ARG, "mst_tune_set", "chunk_k9 out of range 0..64");
+  if (!strcmp(key, "chunk6") && (value < 0 || value > kStreamSeqs))
+    return ctx->fail(MST_E_ARG, "mst_tune_set", "chunk6 out of range 0..64");
+  if (!strcmp(key, "chunk_w3") && (value < 0 || value > kStreamSeqs))
+    return ctx->fail(MST_E_ARG, "mst_tune_set", "chunk_w3 out of range 0..64");
   if (!strcmp(key, "branches") && (value < 1 || value > 4))
     return ctx->fail(MST_E_ARG, "mst_tune_set", "branches out of range 1..4");
   if (!strcmp(key, "k6") && value != 0 && value != 3 && value != 4)

@@ -1406,11 +1406,11 @@ extern "C" int mst_server_set(mst_server* S, const char* key, double value) {
     S->s6p2 = (int)value;
     return MST_OK;
   }
-  if (!strcmp(key, "chunk6") && value >= 0 && value <= 4096) {
+  if (!strcmp(key, "chunk6") && value >= 0 && value <= kStreamSeqs) {
     S->chunk6 = (int)value;
     return MST_OK;
   }
-  if (!strcmp(key, "chunk") && value >= 1 && value <= 4096) {
+  if (!strcmp(key, "chunk") && value >= 1 && value <= kStreamSeqs) {
     S->chunk = (int)value;
     return MST_OK;
   }

@@ -4,6 +4,11 @@
 
 namespace mst {
 
+// Column sequences of one work item at most: a stream's column boundaries
+// live in one VGPR of the sweeping wave, a lane each (item-size tunables are
+// capped at this).
+constexpr int kStreamSeqs = 64;
+
 // Work item of the streamed kernels: a row pair (ya, yb) against the column
 // sequences at positions xbegin .. xbegin+xcount-1.
 struct Item3 {

@@ -77,26 +77,23 @@ __global__ void __launch_bounds__(MST_WAVE * W * Pipes<W>::P) score5_kernel(Scor
     if (it >= a.last) break;
     const Item3 item = a.items[it];
     Stream S = item_stream(item, it, a);
-    int rs = 0;  // first row of the lane's row pair in the strip
+    Stream S2 = S;  // stacked: the second row pair (uniform)
+    Stream Sp = S;  // the lane's row pair, for its profile rows
+    int rs = 0;     // first row of the lane's row pair in the strip
     if (ST) {
       Item3 i2 = item;
       const int2 q = a.pair2[it];
       i2.ya = q.x;
       i2.yb = q.y;
-      const Stream S2 = item_stream(i2, it, a);
+      S2 = item_stream(i2, it, a);
       if (lane >= MST_WAVE / 2) {
-        S.ya = S2.ya;
-        S.ma = S2.ma;
-        S.yb = S2.yb;
-        S.mb = S2.mb;
-        S.oya = S2.oya;
-        S.oyb = S2.oyb;
+        Sp = S2;
         rs = (MST_WAVE / 2) * K;
       }
     }
     // profile of every strip: wave wv fills residues wv, wv+NW, ...
     for (int js = 0; js < W; ++js)
-      build_strip_profile<K, WIDE>(prof + js * STRIP, S_lds, S, a, js * MST_WAVE * K - rs, wv, NW, bonus);
+      build_strip_profile<K, WIDE>(prof + js * STRIP, S_lds, Sp, a, js * MST_WAVE * K - rs, wv, NW, bonus);
     if (threadIdx.x < NW) {
       s_prog[threadIdx.x] = 0;
       s_cons[threadIdx.x] = 0;
@@ -119,22 +116,22 @@ __global__ void __launch_bounds__(MST_WAVE * W * Pipes<W>::P) score5_kernel(Scor
     if (kb < ke) {
       S.base = a.soff[kb];
       S.ncol = (int)(a.soff[ke] - S.base);
-      const int r0 = j * MST_WAVE * K - rs;
+      const Owners ow = make_owners<K>(S, j * MST_WAVE * K, ST ? &S2 : nullptr);
       const Link in{ring_in, 0, RING5 - 1}, out{ring_out, 0, RING5 - 1};
       int* prog = s_prog + p * W;
       int* cons = s_cons + p * W;
       if (W == 1) {
-        sweep5<K, true, false, WIDE, ST>(kb, r0, S, prof_lane, G2, in, 0, nullptr, nullptr, out, 0, nullptr,
-                               nullptr, 0, 1, a);
+        sweep5<K, true, false, WIDE, ST>(kb, ke, ow, S, prof_lane, G2, in, 0, nullptr, nullptr, out, 0, nullptr,
+                                         nullptr, 0, 1, a);
       } else if (j == 0) {
-        sweep5<K, true, true, WIDE>(kb, r0, S, prof_lane, G2, in, 0, nullptr, nullptr, out, 0, prog + j,
-                              cons + j + 1, RING5, 1, a);
+        sweep5<K, true, true, WIDE>(kb, ke, ow, S, prof_lane, G2, in, 0, nullptr, nullptr, out, 0, prog + j,
+                                    cons + j + 1, RING5, 1, a);
       } else if (j == W - 1) {
-        sweep5<K, false, false, WIDE>(kb, r0, S, prof_lane, G2, in, 0, prog + j - 1, cons + j, out, 0,
-                                nullptr, nullptr, 0, 1, a);
+        sweep5<K, false, false, WIDE>(kb, ke, ow, S, prof_lane, G2, in, 0, prog + j - 1, cons + j, out, 0,
+                                      nullptr, nullptr, 0, 1, a);
       } else {
-        sweep5<K, false, true, WIDE>(kb, r0, S, prof_lane, G2, in, 0, prog + j - 1, cons + j, out, 0,
-                               prog + j, cons + j + 1, RING5, 1, a);
+        sweep5<K, false, true, WIDE>(kb, ke, ow, S, prof_lane, G2, in, 0, prog + j - 1, cons + j, out, 0,
+                                     prog + j, cons + j + 1, RING5, 1, a);
       }
     }
     __syncthreads();  // profile, rings and counters are rebuilt for the next item

@@ -101,20 +101,21 @@ __global__ void __launch_bounds__(MST_WAVE * W6) score6_kernel(Score5Args a) {
         span = 0;
         pub = 16;
       }
-      const int r0 = s * ROWS;
+      const Owners ow = make_owners<K>(S, s * ROWS, nullptr);
+      const int kb = item.xbegin, ke = item.xbegin + item.xcount;
       if (first) {
         if (last)
-          sweep5<K, true, false>(item.xbegin, r0, S, prof_lane, G2, in, gin, prog_in, cons_out, out,
+          sweep5<K, true, false>(kb, ke, ow, S, prof_lane, G2, in, gin, prog_in, cons_out, out,
                                  g, prog_out, cons_in, span, pub, a);
         else
-          sweep5<K, true, true>(item.xbegin, r0, S, prof_lane, G2, in, gin, prog_in, cons_out, out,
+          sweep5<K, true, true>(kb, ke, ow, S, prof_lane, G2, in, gin, prog_in, cons_out, out,
                                 g, prog_out, cons_in, span, pub, a);
       } else {
         if (last)
-          sweep5<K, false, false>(item.xbegin, r0, S, prof_lane, G2, in, gin, prog_in, cons_out,
+          sweep5<K, false, false>(kb, ke, ow, S, prof_lane, G2, in, gin, prog_in, cons_out,
                                   out, g, prog_out, cons_in, span, pub, a);
         else
-          sweep5<K, false, true>(item.xbegin, r0, S, prof_lane, G2, in, gin, prog_in, cons_out,
+          sweep5<K, false, true>(kb, ke, ow, S, prof_lane, G2, in, gin, prog_in, cons_out,
                                  out, g, prog_out, cons_in, span, pub, a);
       }
     }
@@ -243,20 +244,20 @@ __global__ void __launch_bounds__(MST_WAVE * W6 * P7) score7_kernel(Score5Args a
           span = 0;
           pub = 16;
         }
-        const int r0 = s * ROWS;
+        const Owners ow = make_owners<K>(S, s * ROWS, nullptr);
         if (first) {
           if (last)
-            sweep5<K, true, false>(kb, r0, S, prof_lane, G2, in, 0, prog_in, cons_out, out, 0, prog_out,
+            sweep5<K, true, false>(kb, ke, ow, S, prof_lane, G2, in, 0, prog_in, cons_out, out, 0, prog_out,
                                    cons_in, span, pub, a);
           else
-            sweep5<K, true, true>(kb, r0, S, prof_lane, G2, in, 0, prog_in, cons_out, out, 0, prog_out,
+            sweep5<K, true, true>(kb, ke, ow, S, prof_lane, G2, in, 0, prog_in, cons_out, out, 0, prog_out,
                                   cons_in, span, pub, a);
         } else {
           if (last)
-            sweep5<K, false, false>(kb, r0, S, prof_lane, G2, in, 0, prog_in, cons_out, out, 0, prog_out,
+            sweep5<K, false, false>(kb, ke, ow, S, prof_lane, G2, in, 0, prog_in, cons_out, out, 0, prog_out,
                                     cons_in, span, pub, a);
           else
-            sweep5<K, false, true>(kb, r0, S, prof_lane, G2, in, 0, prog_in, cons_out, out, 0, prog_out,
+            sweep5<K, false, true>(kb, ke, ow, S, prof_lane, G2, in, 0, prog_in, cons_out, out, 0, prog_out,
                                    cons_in, span, pub, a);
         }
       }

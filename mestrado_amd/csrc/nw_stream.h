@@ -34,13 +34,21 @@ struct Link {
   int base, mask;
 };
 
-struct Cursor {
-  int kx;  // position of the column sequence the lane is in
-  int ce;  // its end column in the stream
+// A row sequence whose last row lies in this strip: the lane holding that row
+// reports the row's H at the end of every column sequence. All fields are
+// uniform across the wave.
+struct Owner {
+  int lane;  // lane of the last row (-1: the row sequence ends outside this strip)
+  int row;   // its row index within the lane
+  int hi;    // 0: pair a (low halves), 1: pair b (high halves)
+  int m;     // rows of the row sequence
+  int oy;    // its index in the output
+  int k;     // column sequence the lane is in (-1 before the stream's first column)
 };
-
-struct Own {
-  int ia, ib;  // row index within the lane of the last row of Ya / Yb, or -1
+constexpr int kMaxOwners = 4;
+struct Owners {
+  Owner o[kMaxOwners];
+  uint64_t lanes;  // bit l: lane l holds a last row
 };
 
 struct Stream {
@@ -52,93 +60,163 @@ struct Stream {
   int ld;
 };
 
-// Hm[idx] of the lane, as an or of masked rows: a chain of selects between
-// the rows would let the compiler turn the selection into a variable index and
-// move the lane state to scratch memory.
+// Owners of a strip whose first row is row r0 of the row pair of S (uniform),
+// and with STACK of the second row pair S2, whose rows start at lane 32.
 template <int K>
-__device__ __forceinline__ int pick(const Lane2<K>& L, int idx, bool hi) {
+__device__ __forceinline__ Owners make_owners(const Stream& S, int r0, const Stream* S2) {
+  Owners w;
+  auto set = [&](Owner& o, int m, int oy, int hi, int first_row, bool on) {
+    const int l = m - 1 - first_row;
+    const bool in = on && l >= 0 && l < MST_WAVE * K;
+    o.lane = in ? l / K : -1;
+    o.row = in ? l % K : 0;
+    o.hi = hi;
+    o.m = m;
+    o.oy = oy;
+    o.k = -1;
+  };
+  set(w.o[0], S.ma, S.oya, 0, r0, true);
+  set(w.o[1], S.mb, S.oyb, 1, r0, S.yb >= 0);
+  if (S2) {
+    set(w.o[2], S2->ma, S2->oya, 0, r0 - (MST_WAVE / 2) * K, true);
+    set(w.o[3], S2->mb, S2->oyb, 1, r0 - (MST_WAVE / 2) * K, S2->yb >= 0);
+  } else {
+    w.o[2].lane = w.o[3].lane = -1;
+    w.o[2].k = w.o[3].k = -1;
+  }
+  w.lanes = 0;
+#pragma unroll
+  for (int q = 0; q < kMaxOwners; ++q)
+    if (w.o[q].lane >= 0) w.lanes |= 1ull << w.o[q].lane;
+  return w;
+}
+
+// Hm[row] of the lane, half hi, as an or of masked rows: a chain of selects
+// would let the compiler turn the selection into a variable index and move
+// the lane state to scratch memory.
+template <int K>
+__device__ __forceinline__ int pick_row(const Lane2<K>& L, int row, int hi) {
   uint32_t v = 0;
 #pragma unroll
-  for (int i = 0; i < K; ++i) v |= as_u(L.Hm[i]) & (0u - (uint32_t)(idx == i));
+  for (int i = 0; i < K; ++i) v |= as_u(L.Hm[i]) & (0u - (uint32_t)(row == i));
   return hi ? (int)(short)(v >> 16) : (int)(short)(v & 0xFFFFu);
 }
 
-// scores of the column sequence at position kx, from the lanes that own the
-// last row of Ya or Yb
-template <int K>
-__device__ __forceinline__ void emit(const Lane2<K>& L, const Own& ow, int kx, const Stream& S,
-                                     const Score5Args& a) {
-  if (ow.ia < 0 && ow.ib < 0) return;
-  const int n = (int)(a.soff[kx + 1] - a.soff[kx]);
-  const int64_t ox = a.order[kx];
-  int32_t* out = a.out_mat + S.obase;
-  if (ow.ia >= 0) {
-    const int s = pick<K>(L, ow.ia, false) - a.ge * (S.ma + n);
-    out[ox * S.ld + S.oya] = s;
-    out[(int64_t)S.oya * S.ld + ox] = s;
-  }
-  if (ow.ib >= 0) {
-    const int s = pick<K>(L, ow.ib, true) - a.ge * (S.mb + n);
-    out[ox * S.ld + S.oyb] = s;
-    out[(int64_t)S.oyb * S.ld + ox] = s;
-  }
+// Column boundaries of the stream: B_k = first column of its k-th sequence
+// (B_0 = 0, lane k of `bnd`) for k < nseq, B_nseq = ncol, none after.
+__device__ __forceinline__ int boundary(int bnd, int k, int nseq, int ncol) {
+  return k < nseq ? __builtin_amdgcn_readlane(bnd, k) : (k == nseq ? ncol : 0x7FFFFFFF);
 }
 
-// One step t of the wave: lane l computes column c = t - l of its K rows.
-// w: the lane's profile dwords for the column's residue; bv: for lane 0, F and
-// H of the row above at column t (the other lanes take them from lane l-1).
-template <int K, bool PRED, bool CHECK, bool FIRSTW, bool STORE, bool WIDE, bool STACK>
-__device__ __forceinline__ void step5(Lane2<K>& L, Cursor& cu, const Own& ow, int t,
-                                      const uint32_t* w, int2 bv, const Stream& S, s2 G2,
-                                      const Link& out, const Score5Args& a) {
-  const int lane = lane_id();
-  uint32_t fin, hup;
-  if (FIRSTW) {
-    fin = shr1z(L.Fout);
-    hup = shr1z(L.Hout);
-  } else {
-    fin = shr1o((uint32_t)bv.x, L.Fout);
-    hup = shr1o((uint32_t)bv.y, L.Hout);
+// The owner's lane has reached a boundary: report the sequence it finished
+// (none at the stream's first column) and move on to the next.
+template <int K>
+__device__ __forceinline__ void emit_owner(const Lane2<K>& L, Owner& o, int bnd, int ord, int nseq,
+                                           const Stream& S, const Score5Args& a) {
+  if (o.k >= 0) {
+    const int n = boundary(bnd, o.k + 1, nseq, S.ncol) - boundary(bnd, o.k, nseq, S.ncol);
+    const int64_t ox = __builtin_amdgcn_readlane(ord, o.k);
+    const int v = pick_row<K>(L, o.row, o.hi);
+    if (lane_id() == o.lane) {
+      const int s = v - a.ge * (o.m + n);
+      int32_t* out = a.out_mat + S.obase;
+      out[ox * S.ld + o.oy] = s;
+      out[(int64_t)o.oy * S.ld + ox] = s;
+    }
   }
-  if (STACK && lane == MST_WAVE / 2) {
+  ++o.k;
+}
+
+// The row update of one step: lane l computes its K rows at column t - l.
+// fin, hup: F and H of the row above at this column; L.hdiag: H of the row
+// above at the previous column.
+template <int K, bool WIDE>
+__device__ __forceinline__ void rows5(Lane2<K>& L, uint32_t fin, uint32_t hup, s2 hd, const uint32_t* w,
+                                      s2 G2) {
+  s2 f = as_s2(fin), hu = as_s2(hup);
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    const s2 e = pmax(L.E[i], L.Hm[i]);
+    f = pmax(f, hu);
+    const s2 dg = hd + prow_w<WIDE>(w, i);
+    const s2 h = pmax(pmax(e, f), dg) - G2;
+    hd = L.Hm[i];
+    L.Hm[i] = h;
+    L.E[i] = e;
+    hu = h;
+  }
+  L.Fout = as_u(f);
+  L.Hout = as_u(hu);
+}
+
+// F and H of the row above a lane's first row at this step's column: lane
+// l-1's last row, lane 0's from bv (or the row -1 boundary in the first strip).
+template <bool FIRSTW, bool STACK>
+__device__ __forceinline__ void row_above(uint32_t Fout, uint32_t Hout, int2 bv, uint32_t& fin,
+                                          uint32_t& hup) {
+  if (FIRSTW) {
+    fin = shr1z(Fout);
+    hup = shr1z(Hout);
+  } else {
+    fin = shr1o((uint32_t)bv.x, Fout);
+    hup = shr1o((uint32_t)bv.y, Hout);
+  }
+  if (STACK && lane_id() == MST_WAVE / 2) {
     // first row of the second row pair: the row -1 boundary (0 when biased)
     fin = 0;
     hup = 0;
   }
-  const int c = t - lane;
-  const bool act = !PRED || (c >= 0 && c < S.ncol);
-  if (act) {
-    s2 hd = as_s2(L.hdiag);
-    if (CHECK && c == cu.ce) {
-      // first column of the next sequence: report the finished one, then
-      // restart from the column -1 boundary
-      emit<K>(L, ow, cu.kx, S, a);
+}
+
+// A step of a block in which no lane meets a column boundary.
+template <int K, bool FIRSTW, bool STORE, bool WIDE, bool STACK>
+__device__ __forceinline__ void step5(Lane2<K>& L, int t, const uint32_t* w, int2 bv, s2 G2,
+                                      const Link& out) {
+  uint32_t fin, hup;
+  row_above<FIRSTW, STACK>(L.Fout, L.Hout, bv, fin, hup);
+  rows5<K, WIDE>(L, fin, hup, as_s2(L.hdiag), w, G2);
+  if (STORE && lane_id() == MST_WAVE - 1)
+    out.p[(out.base + t - (MST_WAVE - 1)) & out.mask] = make_int2((int)L.Fout, (int)L.Hout);
+  L.hdiag = hup;
+}
+
+// A step of a block in which lanes may meet a column boundary. Bit l of the
+// uniform mask M is set when lane l's column is a boundary: the lane then
+// restarts from the column -1 boundary (all zero in the biased form) after
+// its owners have reported. Lanes outside the stream compute values nobody
+// reads; they restart at its first column, and the last boundary (ncol) ends
+// every lane's last sequence.
+template <int K, bool FIRSTW, bool STORE, bool WIDE, bool STACK>
+__device__ __forceinline__ void step5c(Lane2<K>& L, uint64_t& M, int& nb, int& bnext, Owners& ow, int bnd,
+                                       int ord, int nseq, int t, const uint32_t* w, int2 bv,
+                                       const Stream& S, s2 G2, const Link& out, const Score5Args& a) {
+  const bool enter = t == bnext;
+  M = (M << 1) | (enter ? 1ull : 0ull);
+  if (enter) {
+    ++nb;
+    bnext = boundary(bnd, nb, nseq, S.ncol);
+  }
+  if (M & ow.lanes) {
 #pragma unroll
-      for (int i = 0; i < K; ++i) {
-        L.Hm[i] = (s2){0, 0};
-        L.E[i] = (s2){0, 0};
-      }
-      hd = (s2){0, 0};
-      cu.kx += 1;
-      cu.ce = (int)(a.soff[cu.kx + 1] - S.base);
-    }
-    s2 f = as_s2(fin), hu = as_s2(hup);
+    for (int q = 0; q < kMaxOwners; ++q)
+      if (ow.o[q].lane >= 0 && ((M >> ow.o[q].lane) & 1ull)) emit_owner<K>(L, ow.o[q], bnd, ord, nseq, S, a);
+  }
+  uint32_t fin, hup;
+  row_above<FIRSTW, STACK>(L.Fout, L.Hout, bv, fin, hup);
+  const bool rst = (M >> lane_id()) & 1ull;
+  s2 hd = as_s2(L.hdiag);
+  if (rst) {
+    hd = (s2){0, 0};
 #pragma unroll
     for (int i = 0; i < K; ++i) {
-      const s2 e = pmax(L.E[i], L.Hm[i]);
-      f = pmax(f, hu);
-      const s2 dg = hd + prow_w<WIDE>(w, i);
-      const s2 h = pmax(pmax(e, f), dg) - G2;
-      hd = L.Hm[i];
-      L.Hm[i] = h;
-      L.E[i] = e;
-      hu = h;
+      L.Hm[i] = (s2){0, 0};
+      L.E[i] = (s2){0, 0};
     }
-    L.Fout = as_u(f);
-    L.Hout = as_u(hu);
-    if (STORE && lane == MST_WAVE - 1)
-      out.p[(out.base + c) & out.mask] = make_int2((int)L.Fout, (int)L.Hout);
   }
+  rows5<K, WIDE>(L, fin, hup, hd, w, G2);
+  const int c = t - (MST_WAVE - 1);
+  if (STORE && lane_id() == MST_WAVE - 1 && c >= 0 && c < S.ncol)
+    out.p[(out.base + c) & out.mask] = make_int2((int)L.Fout, (int)L.Hout);
   L.hdiag = hup;
 }
 
@@ -168,8 +246,9 @@ __device__ __forceinline__ int code_at(uint2 v, int j) {
   return (int)(((j < 4 ? v.x : v.y) >> (8 * (j & 3))) & 0xFFu);
 }
 
-// One wave's sweep of a stream for one strip (rows r0 .. r0+64K-1 of the row
-// pair). in/out: the row above / this strip's last row. Progress counters count
+// One wave's sweep of a stream (column sequences kb .. ke-1, at most
+// kStreamSeqs) for one strip of the row pair; ow: the strip's owners of last
+// rows. in/out: the row above / this strip's last row. Progress counters count
 // columns in global units: gin + local column for `in` (the producer's units),
 // gout + local column for `out`, so a wave that sweeps the stream again for a
 // later strip keeps them monotonic and an LDS ring continues across sweeps.
@@ -182,9 +261,13 @@ __device__ __forceinline__ int code_at(uint2 v, int j) {
 // Per block of 8 steps the wave loads the next block's residue codes (one
 // 8-byte load per lane), reads the 8 steps' profile dwords from LDS before the
 // first step, and reloads each step's `in` word for the next block right after
-// the step has used it, so no register moves between blocks.
+// the step has used it, so no register moves between blocks. The stream's
+// column boundaries (one per lane of a VGPR) drive a uniform 64-bit mask of the
+// lanes at a boundary, which shifts by one lane per step: a block where it is
+// empty and no boundary enters runs the plain steps, any other block the steps
+// with restarts and reports, without a branch per lane.
 template <int K, bool FIRSTW, bool STORE, bool WIDE = false, bool STACK = false>
-__device__ __forceinline__ void sweep5(int kb, int r0, const Stream& S, const uint8_t* prof_lane,
+__device__ __forceinline__ void sweep5(int kb, int ke, Owners ow, const Stream& S, const uint8_t* prof_lane,
                                        s2 G2, const Link& in, int gin, const int* prog_in,
                                        int* cons_out, const Link& out, int gout, int* prog_out,
                                        const int* cons_in, int out_span, int pub_every,
@@ -193,16 +276,12 @@ __device__ __forceinline__ void sweep5(int kb, int r0, const Stream& S, const ui
   const int lane = lane_id();
   const int ncol = S.ncol;
   const int T = ncol + MST_WAVE - 1;
+  const int nseq = ke - kb;  // <= kStreamSeqs
   const int8_t* X = a.scodes + S.base;
-  Own ow;
-  {
-    const int la = S.ma - 1 - r0, lb = S.mb - 1 - r0;
-    ow.ia = (la >= 0 && la < MST_WAVE * K && la / K == lane) ? la % K : -1;
-    ow.ib = (S.yb >= 0 && lb >= 0 && lb < MST_WAVE * K && lb / K == lane) ? lb % K : -1;
-  }
-  Cursor cu;
-  cu.kx = kb;
-  cu.ce = (int)(a.soff[kb + 1] - S.base);
+  const int bnd = lane < nseq ? (int)(a.soff[kb + lane] - S.base) : 0x7FFFFFFF;
+  const int ord = lane < nseq ? a.order[kb + lane] : 0;
+  uint64_t M = 0;
+  int nb = 0, bnext = 0;  // the next boundary to enter lane 0: B_0 = 0
   Lane2<K> L;
 #pragma unroll
   for (int i = 0; i < K; ++i) {
@@ -235,23 +314,21 @@ __device__ __forceinline__ void sweep5(int kb, int r0, const Stream& S, const ui
       const int need = gout + tb - (MST_WAVE - 1) + (U5 - 1) - out_span + 1;
       if (need > 0) wait_at_least(cons_in, min(need, gout + ncol));
     }
-    const bool full = (tb >= MST_WAVE - 1) && (tb + U5 <= ncol);
-    const bool cross =
-        __builtin_amdgcn_ballot_w64(cu.ce < ncol && cu.ce <= tb + (U5 - 1) - lane) != 0;
-#define MST_STEPS(PRED, CHECK)                                                            \
-  _Pragma("unroll") for (int j = 0; j < U5; ++j) {                                        \
-    if (!PRED || tb + j < T)                                                              \
-      step5<K, PRED, CHECK, FIRSTW, STORE, WIDE, STACK>(L, cu, ow, tb + j, w[j], bv[j], S, G2, out, a); \
-    if (!FIRSTW) bv[j] = in.p[(in.base + min(tb + U5 + j, ncol - 1)) & in.mask];           \
-  }
-    if (full && !cross) {
-      MST_STEPS(false, false)
-    } else if (!cross) {
-      MST_STEPS(true, false)
+    if (M == 0 && bnext >= tb + U5) {
+#pragma unroll
+      for (int j = 0; j < U5; ++j) {
+        step5<K, FIRSTW, STORE, WIDE, STACK>(L, tb + j, w[j], bv[j], G2, out);
+        if (!FIRSTW) bv[j] = in.p[(in.base + min(tb + U5 + j, ncol - 1)) & in.mask];
+      }
     } else {
-      MST_STEPS(true, true)
+#pragma unroll
+      for (int j = 0; j < U5; ++j) {
+        if (tb + j < T)
+          step5c<K, FIRSTW, STORE, WIDE, STACK>(L, M, nb, bnext, ow, bnd, ord, nseq, tb + j, w[j], bv[j], S,
+                                                G2, out, a);
+        if (!FIRSTW) bv[j] = in.p[(in.base + min(tb + U5 + j, ncol - 1)) & in.mask];
+      }
     }
-#undef MST_STEPS
     // the `in` words of columns < tb + 16 are read once the loads complete;
     // the release store orders them before the counter
     if (!FIRSTW && lane == 0) flag_store(cons_out, gin + min(tb + 2 * U5, ncol));
@@ -262,7 +339,11 @@ __device__ __forceinline__ void sweep5(int kb, int r0, const Stream& S, const ui
     }
   }
   if (STORE && lane == MST_WAVE - 1) flag_store(prog_out, gout + ncol);
-  emit<K>(L, ow, cu.kx, S, a);
+  // lane 63's last column is the stream's last: its owners report after the
+  // loop (every other lane has passed the final boundary)
+#pragma unroll
+  for (int q = 0; q < kMaxOwners; ++q)
+    if (ow.o[q].lane >= 0 && ow.o[q].k < nseq) emit_owner<K>(L, ow.o[q], bnd, ord, nseq, S, a);
 }
 
 // Profile bytes S[Y_r][x] + go + ge of rows r0 + lane*K + i for the residues
