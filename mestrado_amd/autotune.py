@@ -84,8 +84,8 @@ def apply_for(ctx, lengths):
 def candidates():
     # k5min = 3 lost by 5-8 ms on the 4,000-protein set in a wider search, k1max 14-16 and items of 72 or more sequences by
     # 3-70 ms; the search keeps the knobs that stayed close
-    for chunk, chunk6, k1max in itertools.product((32, 48, 64), (2, 4), (12, 13)):
-        yield {"chunk": chunk, "chunk6": chunk6, "wg5": 0, "k5min": 1, "k1max": k1max}
+    for chunk, chunk6, k1max, chunk_k9 in itertools.product((32, 48, 64), (2, 4), (12, 13), (0, 32)):
+        yield {"chunk": chunk, "chunk6": chunk6, "wg5": 0, "k5min": 1, "k1max": k1max, "chunk_k9": chunk_k9}
 
 
 def tune(ctx, seqs, reps=2, log=None):
