@@ -146,29 +146,7 @@ static int run_align_batches(mst_ctx* ctx, int src, const int8_t* d_seqs, const 
     // strips and columns) in pairs, then all items largest first
     std::vector<F16Item> items;
     std::vector<int2> tstrips;  // (job, strip) of the score tables to build
-    {
-      std::vector<int64_t> work;
-      for (int64_t k = 0; k < n0;) {
-        const int K = jobs[k0 + k].k;
-        const int64_t sh = 64 * K;
-        if (jobs[k0 + k].m > sh) {  // two strips: one item, a strip per half
-          items.push_back(F16Item{(int)k, -1, K, 1});
-          work.push_back((int64_t)(jobs[k0 + k].n + 127) * sh);
-          k += 1;
-          continue;
-        }
-        const bool pair = k + 1 < n0 && jobs[k0 + k + 1].k == K && jobs[k0 + k + 1].m <= sh;
-        items.push_back(F16Item{(int)k, pair ? (int)(k + 1) : -1, K, 0});
-        work.push_back((int64_t)(std::max(jobs[k0 + k].n, pair ? jobs[k0 + k + 1].n : 0) + 63) * sh);
-        k += pair ? 2 : 1;
-      }
-      std::vector<size_t> ord(items.size());
-      for (size_t q = 0; q < ord.size(); ++q) ord[q] = q;
-      std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) { return work[x] > work[y]; });
-      std::vector<F16Item> by_work(items.size());
-      for (size_t q = 0; q < ord.size(); ++q) by_work[q] = items[ord[q]];
-      items.swap(by_work);
-    }
+    plan_flag16_items(jobs.data() + k0, n0, items);
     if (src == SRC_SEQ)
       for (int64_t k = 0; k < n0; ++k) {
         const int sh = 64 * jobs[k0 + k].k;  // rows per strip

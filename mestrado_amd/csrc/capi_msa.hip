@@ -103,7 +103,8 @@ extern "C" int mst_msa(mst_ctx* ctx, int id, const mst_join* joins, int* msa_len
         batch.push_back(list[q]);
         ++q;
       }
-      CK(eng.enqueue(spec.data(), (int)spec.size(), ctx->d_S, ctx->go, ctx->ge, ctx->num_cus, ctx->stream));
+      CK(eng.enqueue(spec.data(), (int)spec.size(), ctx->d_S, ctx->go, ctx->ge, ctx->num_cus, ctx->stream,
+                   ctx->flag16 ? ctx->smax : -1));
       CK(hipStreamSynchronize(ctx->stream));
       for (size_t b = 0; b < batch.size(); ++b) {
         const mst_join& J = joins[batch[b]];

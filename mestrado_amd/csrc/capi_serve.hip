@@ -1015,7 +1015,8 @@ int launch_merges(mst_server* S) {
     }
     if (slot.jobs.empty()) return MST_OK;
     CK(slot.eng.leaves((const int8_t*)S->d_arena, leaves.data(), (int)leaves.size(), slot.st));
-    CK(slot.eng.enqueue(spec.data(), (int)spec.size(), ctx->d_S, ctx->go, ctx->ge, ctx->num_cus, slot.st));
+    CK(slot.eng.enqueue(spec.data(), (int)spec.size(), ctx->d_S, ctx->go, ctx->ge, ctx->num_cus, slot.st,
+                        ctx->flag16 ? ctx->smax : -1));
     CK(hipEventRecord(slot.ev, slot.st));
     slot.busy = true;
   }

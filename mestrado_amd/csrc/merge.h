@@ -55,8 +55,11 @@ class MergeEngine {
   ~MergeEngine();
   // Enqueues merges [0, count); the path length of merge i is in lens()[i]
   // once the stream has passed the enqueued work.
+  // f16_smax >= 0: merges whose DP fits 16 bits with that largest
+  // substitution score (and of at most two 512-row strips) take the packed
+  // flag kernel; < 0: the int32 kernels take every merge.
   hipError_t enqueue(const MergeSpec* m, int count, const int8_t* d_S, int go, int ge, int num_cus,
-                     hipStream_t st);
+                     hipStream_t st, int f16_smax = -1);
   // Enqueues the leaf rows of `count` sequences (codes: a reversed code block).
   hipError_t leaves(const int8_t* codes, const LeafDesc* l, int count, hipStream_t st);
   const int32_t* lens() const { return h_lens_; }
@@ -90,6 +93,9 @@ class MergeEngine {
   int64_t trows_cap_ = 0;
   int* d_tstate_ = nullptr;
   int64_t tstate_cap_ = 0;
+  F16Item* d_items_ = nullptr;  // packed flag kernel items
+  F16Item* h_items_ = nullptr;
+  int64_t items_cap_ = 0, hitems_cap_ = 0;
 };
 
 }  // namespace mst

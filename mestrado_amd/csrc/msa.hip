@@ -320,10 +320,10 @@ hipError_t MergeEngine::grow(void** p, int64_t* cap, int64_t bytes, bool pinned)
 
 MergeEngine::~MergeEngine() {
   void* dev[] = {d_scratch_, d_desc_, d_jobs_, d_res_,    d_lens_,  d_bufs_,
-                 d_counter_, d_leaf_, d_tiles_, d_trows_, d_tstate_};
+                 d_counter_, d_leaf_, d_tiles_, d_trows_, d_tstate_, d_items_};
   for (void* p : dev)
     if (p) hipFree(p);
-  void* host[] = {h_desc_, h_jobs_, h_lens_, h_leaf_, h_tiles_};
+  void* host[] = {h_desc_, h_jobs_, h_lens_, h_leaf_, h_tiles_, h_items_};
   for (void* p : host)
     if (p) hipHostFree(p);
 }
@@ -341,21 +341,39 @@ hipError_t MergeEngine::leaves(const int8_t* codes, const LeafDesc* l, int count
 }
 
 hipError_t MergeEngine::enqueue(const MergeSpec* m, int count, const int8_t* d_S, int go, int ge,
-                                int num_cus, hipStream_t st) {
+                                int num_cus, hipStream_t st, int f16_smax) {
   if (count <= 0) return hipSuccess;
   // flag jobs: one strip, 2..4 strips (pipeline of 4 waves), more (8 waves),
-  // long ones (tiles over anti-diagonal launches)
+  // long ones (tiles over anti-diagonal launches), and last those of the
+  // packed kernel (one or two strips whose 16-bit values cannot wrap)
   std::vector<int> cls(count, -1);
-  int64_t nb[4] = {0, 0, 0, 0};
+  int64_t nb[5] = {0, 0, 0, 0, 0};
   for (int i = 0; i < count; ++i) {
     if (m[i].nX > 0 && m[i].nY > 0) {
       const int s = strips_of(m[i].nY);
-      cls[i] = s == 1 ? 0 : (s <= 4 ? 1 : (tiled_job(m[i].nX, m[i].nY) ? 3 : 2));
+      const int64_t R = m[i].nY, n = m[i].nX;
+      const bool packed = f16_smax >= 0 && s <= 2 && !tiled_job(m[i].nX, m[i].nY) &&
+                          (int64_t)f16_smax * (std::min(R, n) + 1) + (int64_t)ge * (R + n + 2) + f16_smax +
+                                  go + ge <= 32767;
+      cls[i] = packed ? 4 : (s == 1 ? 0 : (s <= 4 ? 1 : (tiled_job(m[i].nX, m[i].nY) ? 3 : 2)));
       nb[cls[i]] += 1;
     }
   }
-  const int64_t njobs = nb[0] + nb[1] + nb[2] + nb[3];
-  int64_t next[4] = {0, nb[0], nb[0] + nb[1], nb[0] + nb[1] + nb[2]};
+  const int64_t njobs = nb[0] + nb[1] + nb[2] + nb[3] + nb[4];
+  int64_t next[5] = {0, nb[0], nb[0] + nb[1], nb[0] + nb[1] + nb[2], nb[0] + nb[1] + nb[2] + nb[3]};
+  const int64_t f0 = next[4];  // the packed jobs: [f0, njobs), by strips then columns, descending
+  std::vector<int> f16_job(count, -1);
+  {
+    std::vector<int> f;
+    for (int i = 0; i < count; ++i)
+      if (cls[i] == 4) f.push_back(i);
+    std::stable_sort(f.begin(), f.end(), [&](int x, int y) {
+      const int sx = strips_of(m[x].nY), sy = strips_of(m[y].nY);
+      if (sx != sy) return sx > sy;
+      return m[x].nX > m[y].nX;
+    });
+    for (size_t r = 0; r < f.size(); ++r) f16_job[f[r]] = (int)(f0 + (int64_t)r);
+  }
   // layout of the scratch block and the prefix sums
   const int64_t desc_bytes = (int64_t)count * (sizeof(MergeDesc) + sizeof(Prefix));
   hipError_t e;
@@ -407,11 +425,11 @@ hipError_t MergeEngine::enqueue(const MergeSpec* m, int count, const int8_t* d_S
     col0 += (int64_t)q.nX + q.nY;
     yrow0 += q.nY;
     item0 += (int64_t)strips_of(q.nY) * ((q.nX + MST_WAVE - 1 + q.tchunk - 1) / q.tchunk);
-    q.job = (int)next[cls[i]]++;
+    q.job = cls[i] == 4 ? f16_job[i] : (int)next[cls[i]]++;
     AlignJob& j = h_jobs_[q.job];
     j.n = q.nX;
     j.m = q.nY;
-    j.fmt = FMT_NIBBLE;
+    j.fmt = cls[i] == 4 ? FMT_PLANES : FMT_NIBBLE;
     j.k = 8;
     j.x_off = j.y_off = 0;
     j.tab_off = q.tab;
@@ -422,8 +440,12 @@ hipError_t MergeEngine::enqueue(const MergeSpec* m, int count, const int8_t* d_S
   const int64_t grid1 = std::max<int64_t>(1, std::min<int64_t>((int64_t)num_cus * 16, nb[0]));
   const int64_t grid2 = std::max<int64_t>(1, std::min<int64_t>((int64_t)num_cus * 2, nb[1]));
   const int64_t grid3 = std::max<int64_t>(1, std::min<int64_t>((int64_t)num_cus, nb[2]));
+  std::vector<F16Item> items;
+  plan_flag16_items(h_jobs_ + f0, nb[4], items);
+  const int64_t grid4 = std::max<int64_t>(1, std::min<int64_t>((int64_t)num_cus * 12, (int64_t)items.size()));
   const int64_t bufcap = maxn;
-  const int64_t bufs = bufcap * std::max(std::max(2 * grid1, grid2), grid3);
+  // the flag kernels run one after the other on the stream: one set of row buffers
+  const int64_t bufs = bufcap * std::max(std::max(std::max(2 * grid1, grid2), grid3), 2 * grid4);
   if ((e = grow((void**)&d_scratch_, &scratch_cap_, std::max<int64_t>(off, 256), false)) != hipSuccess)
     return e;
   if ((e = grow((void**)&d_desc_, &desc_cap_, desc_bytes, false)) != hipSuccess) return e;
@@ -484,11 +506,30 @@ hipError_t MergeEngine::enqueue(const MergeSpec* m, int count, const int8_t* d_S
       a.counter = d_counter_ + 1;
       launch_align_pipe(SRC_TAB, 8, a, (int)grid3, st);
     }
+    if (nb[4] > 0) {
+      const int64_t ib = sizeof(F16Item) * (int64_t)items.size();
+      if ((e = grow((void**)&d_items_, &items_cap_, ib, false)) != hipSuccess) return e;
+      if ((e = grow((void**)&h_items_, &hitems_cap_, ib, true)) != hipSuccess) return e;
+      std::copy(items.begin(), items.end(), h_items_);
+      if ((e = hipMemcpyAsync(d_items_, h_items_, ib, hipMemcpyHostToDevice, st)) != hipSuccess) return e;
+      F16Args f;
+      f.jobs = d_jobs_ + f0;
+      f.items = d_items_;
+      f.nitems = (int64_t)items.size();
+      f.tab = d_scratch_;
+      f.flags = (uint32_t*)d_scratch_;
+      f.scores = d_res_ + f0;
+      f.bufs = d_bufs_;
+      f.bufcap = bufcap;
+      f.go = go;
+      f.ge = ge;
+      launch_flag16(f, (int)grid4, st);
+    }
     if (nb[3] > 0) {
       std::vector<TileJob> tiles;
       std::vector<int64_t> diag_off;
       int64_t trows = 0, tstate = 0;
-      plan_tiles(h_jobs_, njobs - nb[3], njobs, kTileWidth, tiles, diag_off, &trows, &tstate);
+      plan_tiles(h_jobs_, f0 - nb[3], f0, kTileWidth, tiles, diag_off, &trows, &tstate);
       if ((e = grow((void**)&d_tiles_, &tiles_cap_, sizeof(TileJob) * tiles.size(), false)) != hipSuccess) return e;
       if ((e = grow((void**)&h_tiles_, &htiles_cap_, sizeof(TileJob) * tiles.size(), true)) != hipSuccess) return e;
       if ((e = grow((void**)&d_trows_, &trows_cap_, sizeof(int2) * trows, false)) != hipSuccess) return e;

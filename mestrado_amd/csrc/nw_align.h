@@ -121,6 +121,11 @@ struct F16Args {
   int go, ge;
 };
 void launch_flag16(const F16Args& a, int grid, hipStream_t st);
+// Items of the packed jobs jobs[0, n): each job's k set, jobs of one k
+// consecutive (sorted by strips, then columns, descending). Jobs of one strip
+// pair up (two jobs per item), a job of two strips is one item of mode 1;
+// items largest first.
+void plan_flag16_items(const AlignJob* jobs, int64_t n, std::vector<F16Item>& items);
 // score tables of sequence jobs: one (job, strip) pair per entry of strips
 // (each job's own k)
 void launch_seq_table(const AlignJob* jobs, const int2* strips, int nstrips, const int8_t* seqs,

@@ -28,6 +28,9 @@
 // biased form, as in the score kernels, which leaves every value unchanged;
 // the two flags they would otherwise set are forced to the reference's value
 // (contLeft = 0 in column 0, contUp = 0 in row 0, which see a -inf there).
+#include <algorithm>
+#include <vector>
+
 #include "common.h"
 #include "nw_align.h"
 #include "pk16.h"
@@ -302,6 +305,31 @@ __global__ void __launch_bounds__(MST_WAVE) flag16_kernel(F16Args a) {
       default: f16_item<8>(a, item, buf, G2); break;
     }
   }
+}
+
+void plan_flag16_items(const AlignJob* jobs, int64_t n, std::vector<F16Item>& items) {
+  items.clear();
+  std::vector<int64_t> work;
+  for (int64_t k = 0; k < n;) {
+    const int K = jobs[k].k;
+    const int64_t sh = 64 * K;
+    if (jobs[k].m > sh) {  // two strips: one item, a strip per half
+      items.push_back(F16Item{(int)k, -1, K, 1});
+      work.push_back((int64_t)(jobs[k].n + 127) * sh);
+      k += 1;
+      continue;
+    }
+    const bool pair = k + 1 < n && jobs[k + 1].k == K && jobs[k + 1].m <= sh;
+    items.push_back(F16Item{(int)k, pair ? (int)(k + 1) : -1, K, 0});
+    work.push_back((int64_t)(std::max(jobs[k].n, pair ? jobs[k + 1].n : 0) + 63) * sh);
+    k += pair ? 2 : 1;
+  }
+  std::vector<size_t> ord(items.size());
+  for (size_t q = 0; q < ord.size(); ++q) ord[q] = q;
+  std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) { return work[x] > work[y]; });
+  std::vector<F16Item> by_work(items.size());
+  for (size_t q = 0; q < ord.size(); ++q) by_work[q] = items[ord[q]];
+  items.swap(by_work);
 }
 
 void launch_flag16(const F16Args& a, int grid, hipStream_t st) {

@@ -101,18 +101,8 @@ __global__ void __launch_bounds__(MST_WAVE * W * Pipes<W>::P) score5_kernel(Scor
     __syncthreads();
     // pipeline p's part of the item: positions [kb, ke), balanced by columns
     const int xb0 = item.xbegin, xe0 = item.xbegin + item.xcount;
-    const int64_t c0 = a.soff[xb0], total = a.soff[xe0] - c0;
     int kb = xb0, ke = xe0;
-    if (P > 1) {
-      const int64_t lo = total * p / P, hi = total * (p + 1) / P;
-      kb = xe0;
-      ke = xe0;
-      for (int k = xb0; k < xe0; ++k) {
-        const int64_t s = a.soff[k] - c0;
-        if (kb == xe0 && s >= lo) kb = k;
-        if (ke == xe0 && s >= hi && p + 1 < P) ke = k;
-      }
-    }
+    if (P > 1) item_part(a, xb0, xe0, p, P, &kb, &ke);
     if (kb < ke) {
       S.base = a.soff[kb];
       S.ncol = (int)(a.soff[ke] - S.base);

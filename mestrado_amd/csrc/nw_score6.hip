@@ -193,16 +193,8 @@ __global__ void __launch_bounds__(MST_WAVE * W6 * P7) score7_kernel(Score5Args a
     const int nsweeps = (ns + W6 - 1) / W6;
     // pipeline p's part of the item: positions [kb, ke), balanced by columns
     const int xb0 = item.xbegin, xe0 = item.xbegin + item.xcount;
-    const int64_t c0 = a.soff[xb0], total = a.soff[xe0] - c0;
-    int kb = xe0, ke = xe0;
-    {
-      const int64_t lo = total * p / P7, hi = total * (p + 1) / P7;
-      for (int k = xb0; k < xe0; ++k) {
-        const int64_t sk = a.soff[k] - c0;
-        if (kb == xe0 && sk >= lo) kb = k;
-        if (ke == xe0 && sk >= hi && p + 1 < P7) ke = k;
-      }
-    }
+    int kb = xb0, ke = xe0;
+    item_part(a, xb0, xe0, p, P7, &kb, &ke);
     for (int q = 0; q < nsweeps; ++q) {
       const int s = j + q * W6;
       // slot j: strip s, its residues split between the two pipelines' wave j

@@ -384,6 +384,25 @@ __device__ __forceinline__ void build_strip_profile(uint8_t* prof, const int8_t*
   }
 }
 
+// Part p of P of an item's column sequences [xb0, xe0), balanced by columns:
+// part q ends before the first sequence whose middle column is at or past
+// q/P of the stream (an empty part when one sequence spans the cut). Every
+// part computes the same cuts, so neighbouring parts meet exactly.
+__device__ __forceinline__ void item_part(const Score5Args& a, int xb0, int xe0, int p, int P, int* kb,
+                                          int* ke) {
+  const int64_t c0 = a.soff[xb0], total = a.soff[xe0] - c0;
+  auto cut = [&](int q) {
+    if (q <= 0) return xb0;
+    if (q >= P) return xe0;
+    const int64_t target2 = 2 * (total * q / P);
+    for (int k = xb0; k < xe0; ++k)
+      if ((a.soff[k] - c0) + (a.soff[k + 1] - c0) >= target2) return k;
+    return xe0;
+  };
+  *kb = cut(p);
+  *ke = cut(p + 1);
+}
+
 // The row pair and output mapping of work item `it`.
 __device__ __forceinline__ Stream item_stream(const Item3& item, int64_t it, const Score5Args& a) {
   Stream S;
