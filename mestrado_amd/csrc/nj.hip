@@ -32,6 +32,8 @@
 namespace mst {
 
 constexpr int NJ_THREADS = 1024;
+// the per-step join kernel: fewer waves, shorter barriers (8 positions each)
+constexpr int NJ_JOIN_THREADS = 512;
 
 struct Best {
   double q;
@@ -75,22 +77,58 @@ __device__ __forceinline__ Best no_best() {
   return b;
 }
 
+// One step of the wave's reduction of candidates: each lane takes the better
+// of its own and the one DPP control CTRL brings (rows outside RM, and lanes
+// with no source, keep their own), tracking the lane the winner came from.
+template <int CTRL, int RM>
+__device__ __forceinline__ int dpp_keep(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, CTRL, RM, 0xF, false);
+}
+template <int CTRL, int RM>
+__device__ __forceinline__ void best_step(double& q, int& J, int& I, int& src) {
+  const long long qb = __double_as_longlong(q);
+  const int qlo = dpp_keep<CTRL, RM>((int)qb), qhi = dpp_keep<CTRL, RM>((int)(qb >> 32));
+  const int oJ = dpp_keep<CTRL, RM>(J), oI = dpp_keep<CTRL, RM>(I), os = dpp_keep<CTRL, RM>(src);
+  const double oq = __longlong_as_double(((long long)qhi << 32) | (long long)(unsigned int)qlo);
+  const bool t = oq < q || (oq == q && (oJ > J || (oJ == J && oI > I)));
+  q = t ? oq : q;
+  J = t ? oJ : J;
+  I = t ? oI : I;
+  src = t ? os : src;
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+
+// The wave's best candidate, the same in every lane: the keys (q, J, I)
+// reduced toward lane 63 by DPP (row shifts 1, 2, 4, 8, then the row
+// broadcasts of lanes 15 and 31), the winner's fields read from its lane. The
+// total order makes the winner the same whatever the order of the reduction.
 __device__ __forceinline__ Best wave_best(Best best) {
-  for (int off = 32; off > 0; off >>= 1) {
-    Best o;
-    o.q = __shfl_down(best.q, off);
-    o.Ta = __shfl_down(best.Ta, off);
-    o.Tb = __shfl_down(best.Tb, off);
-    o.d = __shfl_down(best.d, off);
-    o.J = __shfl_down(best.J, off);
-    o.I = __shfl_down(best.I, off);
-    o.pa = __shfl_down(best.pa, off);
-    o.pb = __shfl_down(best.pb, off);
-    o.ra = __shfl_down(best.ra, off);
-    o.rb = __shfl_down(best.rb, off);
-    keep_better(best, o);
-  }
-  return best;
+  double q = best.q;
+  int J = best.J, I = best.I, src = lane_id();
+  best_step<0x111, 0xF>(q, J, I, src);
+  best_step<0x112, 0xF>(q, J, I, src);
+  best_step<0x114, 0xF>(q, J, I, src);
+  best_step<0x118, 0xF>(q, J, I, src);
+  best_step<0x142, 0xA>(q, J, I, src);
+  best_step<0x143, 0xC>(q, J, I, src);
+  const int w = __builtin_amdgcn_readlane(src, 63);
+  Best b;
+  b.q = readlane_f64(best.q, w);
+  b.Ta = readlane_f64(best.Ta, w);
+  b.Tb = readlane_f64(best.Tb, w);
+  b.d = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(best.d), w));
+  b.J = __builtin_amdgcn_readlane(best.J, w);
+  b.I = __builtin_amdgcn_readlane(best.I, w);
+  b.pa = __builtin_amdgcn_readlane(best.pa, w);
+  b.pb = __builtin_amdgcn_readlane(best.pb, w);
+  b.ra = __builtin_amdgcn_readlane(best.ra, w);
+  b.rb = __builtin_amdgcn_readlane(best.rb, w);
+  return b;
 }
 
 // The scan's work in balanced units. The upper triangle's rows pa = 0 ..
@@ -341,17 +379,21 @@ __device__ __forceinline__ Best workgroup_best(BestLds& L, Best best) {
 
 // The per-position values of a join (storage row, node id, T by position) of
 // one batch of positions, loaded before the winner is known.
-constexpr int kJoinU = 4;
+// (U positions per thread and batch, TH threads: the whole-matrix loop's
+// workgroup takes 4 of NJ_THREADS, the per-step join kernel 8 of
+// NJ_JOIN_THREADS.)
+template <int U>
 struct JoinBatch {
-  int rk[kJoinU], ik[kJoinU];
-  double tk[kJoinU];
+  int rk[U], ik[U];
+  double tk[U];
 };
 
-__device__ __forceinline__ void join_load(JoinBatch& B, int p0, int m, const int* act, const int* ids,
+template <int TH, int U>
+__device__ __forceinline__ void join_load(JoinBatch<U>& B, int p0, int m, const int* act, const int* ids,
                                           const double* Tp) {
 #pragma unroll
-  for (int u = 0; u < kJoinU; ++u) {
-    const int p = p0 + u * NJ_THREADS;
+  for (int u = 0; u < U; ++u) {
+    const int p = p0 + u * TH;
     const int pc = p < m ? p : m - 1;
     B.rk[u] = act[pc];
     B.ik[u] = ids[pc];
@@ -365,10 +407,11 @@ __device__ __forceinline__ void join_load(JoinBatch& B, int p0, int m, const int
 // active arrays compacted into act2/ids2/Tp2 (pa and pb dropped, the new node
 // appended) in the same pass over the positions. B0: this thread's first
 // batch of positions (join_load at p0 = threadIdx.x), loaded by the caller.
+template <int TH, int U>
 __device__ __forceinline__ void join_step(JoinLds& L, const Best& w, int m, int next, float* D, double* T,
                                           int ld, const int* act, const int* ids, const double* Tp,
                                           int* act2, int* ids2, double* Tp2, NjJoin* rec,
-                                          const JoinBatch& B0) {
+                                          const JoinBatch<U>& B0) {
   const int tid = threadIdx.x;
   const int ra = w.ra, rb = w.rb;
   const float d = w.d;
@@ -384,19 +427,19 @@ __device__ __forceinline__ void join_step(JoinLds& L, const Best& w, int m, int 
   }
   const float* Da = D + (int64_t)ra * ld;
   const float* Db = D + (int64_t)rb * ld;
-  JoinBatch B = B0;
+  JoinBatch<U> B = B0;
   RowSum rs;
-  for (int p0 = tid; p0 < m; p0 += kJoinU * NJ_THREADS) {
-    if (p0 != tid) join_load(B, p0, m, act, ids, Tp);
-    float dik[kJoinU], djk[kJoinU];
+  for (int p0 = tid; p0 < m; p0 += U * TH) {
+    if (p0 != tid) join_load<TH, U>(B, p0, m, act, ids, Tp);
+    float dik[U], djk[U];
 #pragma unroll
-    for (int u = 0; u < kJoinU; ++u) {
+    for (int u = 0; u < U; ++u) {
       dik[u] = Da[B.rk[u]];
       djk[u] = Db[B.rk[u]];
     }
 #pragma unroll
-    for (int u = 0; u < kJoinU; ++u) {
-      const int p = p0 + u * NJ_THREADS;
+    for (int u = 0; u < U; ++u) {
+      const int p = p0 + u * TH;
       if (p >= m || p == w.pa || p == w.pb) continue;
       const int np = p - (p > w.pa) - (p > w.pb);
       const float tmp = ((dik[u] + djk[u]) - d) / 2;
@@ -466,10 +509,10 @@ __device__ __forceinline__ void nj_run(const NjArgs a) {
     const int units = scan_units(m);
     for (int u = wave; u < units; u += NJ_THREADS / 64)
       scan_unit(best, u, m, R, nm2, act, ids, D, Tp, ld, lane);
-    JoinBatch B0;
-    join_load(B0, tid, m, act, ids, Tp);
+    JoinBatch<4> B0;
+    join_load<NJ_THREADS, 4>(B0, tid, m, act, ids, Tp);
     const Best w = workgroup_best(L.b, best);
-    join_step(L, w, m, s_next, D, T, ld, act, ids, Tp, act2, ids2, Tp2, &a.joins[step], B0);
+    join_step<NJ_THREADS, 4>(L, w, m, s_next, D, T, ld, act, ids, Tp, act2, ids2, Tp2, &a.joins[step], B0);
     __syncthreads();
     {
       int* t = act;
@@ -556,8 +599,8 @@ __global__ void __launch_bounds__(NJ_SCAN_THREADS) nj_scan_kernel(NjArgs a, int 
   if (threadIdx.x == 0) part[blockIdx.x] = best;
 }
 
-__global__ void __launch_bounds__(NJ_THREADS) nj_join_kernel(NjArgs a, const Best* __restrict__ part,
-                                                             int nparts, int step, int m, int flip) {
+__global__ void __launch_bounds__(NJ_JOIN_THREADS) nj_join_kernel(NjArgs a, const Best* __restrict__ part,
+                                                                  int nparts, int step, int m, int flip) {
   __shared__ JoinLds L;
   const int tid = threadIdx.x;
   const int n0 = a.n0;
@@ -565,13 +608,13 @@ __global__ void __launch_bounds__(NJ_THREADS) nj_join_kernel(NjArgs a, const Bes
   const int* ids = a.ids + (flip ? n0 : 0);
   const double* Tp = a.Tpos + (flip ? n0 : 0);
   // the positions' values and the candidates load together
-  JoinBatch B0;
-  join_load(B0, tid, m, act, ids, Tp);
+  JoinBatch<8> B0;
+  join_load<NJ_JOIN_THREADS, 8>(B0, tid, m, act, ids, Tp);
   Best best = no_best();
-  for (int k = tid; k < nparts; k += NJ_THREADS) keep_better(best, part[k]);
+  for (int k = tid; k < nparts; k += NJ_JOIN_THREADS) keep_better(best, part[k]);
   const Best w = workgroup_best(L.b, best);
-  join_step(L, w, m, n0 + step, a.D, a.T, a.ld, act, ids, Tp, a.act + (flip ? 0 : n0),
-            a.ids + (flip ? 0 : n0), a.Tpos + (flip ? 0 : n0), &a.joins[step], B0);
+  join_step<NJ_JOIN_THREADS, 8>(L, w, m, n0 + step, a.D, a.T, a.ld, act, ids, Tp, a.act + (flip ? 0 : n0),
+                                a.ids + (flip ? 0 : n0), a.Tpos + (flip ? 0 : n0), &a.joins[step], B0);
 }
 
 // T of every active row after a rebuild step (m1 active nodes in half flip)
@@ -599,7 +642,7 @@ hipError_t run_nj_steps(const NjArgs& a, void* parts, int nparts, hipStream_t st
     const int m = n0 - step, flip = step & 1;
     hipLaunchKernelGGL(nj_scan_kernel, dim3(nparts), dim3(NJ_SCAN_THREADS), 0, st, a, m, R, flip,
                        (Best*)parts);
-    hipLaunchKernelGGL(nj_join_kernel, dim3(1), dim3(NJ_THREADS), 0, st, a, (const Best*)parts, nparts,
+    hipLaunchKernelGGL(nj_join_kernel, dim3(1), dim3(NJ_JOIN_THREADS), 0, st, a, (const Best*)parts, nparts,
                        step, m, flip);
     if (steps == 0) {
       const int m1 = m - 1;
