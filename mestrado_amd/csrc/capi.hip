@@ -102,6 +102,8 @@ extern "C" int mst_create(int gap_open, int gap_ext, const int8_t* subst, int de
   ctx->symmetric = true;
   ctx->smax = 0;
   for (int k = 0; k < 576; ++k) ctx->smax = std::max(ctx->smax, (int)ctx->S[k]);
+  ctx->smin = 127;
+  for (int k = 0; k < 576; ++k) ctx->smin = std::min(ctx->smin, (int)ctx->S[k]);
   for (int a = 0; a < 24; ++a)
     for (int b = 0; b < 24; ++b)
       if (ctx->S[a * 24 + b] != ctx->S[b * 24 + a]) ctx->symmetric = false;

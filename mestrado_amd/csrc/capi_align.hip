@@ -40,6 +40,14 @@ bool flag16_usable(const mst_ctx* ctx) {
   }
   return smin + ctx->go + ctx->ge >= 0 && smax + ctx->go + ctx->ge <= 255;
 }
+// Sequence jobs of the packed kernel take their scores from query profiles in
+// LDS, a nibble per row, when every entry of S is within 15 of the least;
+// otherwise from a byte table built first.
+static bool flag16_prof_usable(const mst_ctx* ctx) {
+  int hi = -128;
+  for (int k = 0; k < 576; ++k) hi = std::max(hi, (int)ctx->S[k]);
+  return hi - ctx->smin <= 15;
+}
 // Jobs of one or two 512-row strips: one pass of the packed kernel (two jobs
 // of a strip each, or one job's two strips); longer ones keep the pipelined
 // kernel, whose waves run their strips at once.
@@ -103,6 +111,7 @@ static int run_align_batches(mst_ctx* ctx, int src, const int8_t* d_seqs, const 
   paths.assign(nj, {});
   int64_t k0 = 0;
   const int64_t budget_words = ctx->flag_budget_bytes / 4;
+  const bool prof = src == SRC_SEQ && flag16_prof_usable(ctx);
   while (k0 < nj) {
     // batch [k0, k1); a packed job's score table (8 bytes per flag word, 2
     // words of the budget) is built for sequence jobs only
@@ -113,12 +122,12 @@ static int run_align_batches(mst_ctx* ctx, int src, const int8_t* d_seqs, const 
       const bool packed = align_class(ctx, jobs[k1]) == 0;
       jobs[k1].k = packed ? flag16_k(jobs[k1].m) : 8;
       const int64_t w = flag_words(jobs[k1].n, jobs[k1].m, jobs[k1].k);
-      const int64_t tw = (packed && src == SRC_SEQ) ? 2 * w : 0;
+      const int64_t tw = (packed && src == SRC_SEQ && !prof) ? 2 * w : 0;
       if (k1 > k0 && words + tabw + w + tw > budget_words) break;
       jobs[k1].flag_off = words;
       jobs[k1].ops_off = opsb;
       jobs[k1].fmt = packed ? FMT_PLANES : FMT_NIBBLE;
-      if (packed && src == SRC_SEQ) jobs[k1].tab_off = 8 * tabw / 2;
+      if (packed && src == SRC_SEQ && !prof) jobs[k1].tab_off = 8 * tabw / 2;
       words += w;
       tabw += tw;
       opsb += jobs[k1].n + jobs[k1].m;
@@ -147,7 +156,7 @@ static int run_align_batches(mst_ctx* ctx, int src, const int8_t* d_seqs, const 
     std::vector<F16Item> items;
     std::vector<int2> tstrips;  // (job, strip) of the score tables to build
     plan_flag16_items(jobs.data() + k0, n0, items);
-    if (src == SRC_SEQ)
+    if (src == SRC_SEQ && !prof)
       for (int64_t k = 0; k < n0; ++k) {
         const int sh = 64 * jobs[k0 + k].k;  // rows per strip
         for (int64_t q = 0; q < (jobs[k0 + k].m + sh - 1) / sh; ++q) tstrips.push_back(make_int2((int)k, (int)q));
@@ -164,7 +173,9 @@ static int run_align_batches(mst_ctx* ctx, int src, const int8_t* d_seqs, const 
       rc = ctx->ensure(&ctx->d_tstate, &ctx->tstate_cap, sizeof(int) * tstate);
       if (rc) return rc;
     }
-    const int grid0 = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ctx->num_cus * 12, (int64_t)items.size()));
+    // profile mode: twelve waves per compute unit (their LDS); else sixteen
+    const int grid0 = (int)std::max<int64_t>(
+        1, std::min<int64_t>((int64_t)ctx->num_cus * (prof ? 12 : 16), (int64_t)items.size()));
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ctx->num_cus * 16, n1));
     const int grid2 = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ctx->num_cus * 2, n2));
     const int64_t bufcap = maxn;
@@ -181,7 +192,7 @@ static int run_align_batches(mst_ctx* ctx, int src, const int8_t* d_seqs, const 
       rc = ctx->ensure(&ctx->d_f16, &ctx->f16_cap,
                        sizeof(F16Item) * items.size() + sizeof(int2) * std::max<size_t>(tstrips.size(), 1));
       if (rc) return rc;
-      if (src == SRC_SEQ) {
+      if (src == SRC_SEQ && !prof) {
         rc = ctx->ensure((void**)&ctx->d_tab16, &ctx->tab16_cap, std::max<int64_t>(4 * tabw, 256));
         if (rc) return rc;
       }
@@ -199,38 +210,6 @@ static int run_align_batches(mst_ctx* ctx, int src, const int8_t* d_seqs, const 
     hipEvent_t* ev = ctx->launch_events.data();
     CK(hipEventRecord(ev[5], ctx->stream));
     if (side) CK(hipStreamWaitEvent(st32, ev[5], 0));
-    if (n0 > 0) {
-      F16Item* d_items = (F16Item*)ctx->d_f16;
-      int2* d_tstrips = (int2*)(d_items + items.size());
-      CK(hipMemcpyAsync(d_items, items.data(), sizeof(F16Item) * items.size(), hipMemcpyHostToDevice,
-                        ctx->stream));
-      const uint8_t* tab = d_tab;
-      if (src == SRC_SEQ) {
-        CK(hipMemcpyAsync(d_tstrips, tstrips.data(), sizeof(int2) * tstrips.size(), hipMemcpyHostToDevice,
-                          ctx->stream));
-        launch_seq_table((const AlignJob*)ctx->d_jobs, d_tstrips, (int)tstrips.size(), d_seqs, ctx->d_S,
-                         ctx->go + ctx->ge, (uint8_t*)ctx->d_tab16, ctx->stream);
-        CK(hipGetLastError());
-        tab = (const uint8_t*)ctx->d_tab16;
-      }
-      CK(hipEventRecord(ev[6], ctx->stream));
-      F16Args f;
-      f.jobs = (const AlignJob*)ctx->d_jobs;
-      f.items = d_items;
-      f.nitems = (int64_t)items.size();
-      f.tab = tab;
-      f.flags = (uint32_t*)ctx->d_flags;
-      f.scores = (int32_t*)ctx->d_jscore;
-      f.bufs = ctx->d_bufs;
-      f.bufcap = bufcap;
-      f.go = ctx->go;
-      f.ge = ctx->ge;
-      launch_flag16(f, grid0, ctx->stream);
-      CK(hipGetLastError());
-    } else {
-      CK(hipEventRecord(ev[6], ctx->stream));
-    }
-    CK(hipEventRecord(ev[7], ctx->stream));  // (the packed kernel's end)
     CK(hipEventRecord(ev[0], st32));
     AlignArgs a;
     a.jobs = (const AlignJob*)ctx->d_jobs;
@@ -280,6 +259,44 @@ static int run_align_batches(mst_ctx* ctx, int src, const int8_t* d_seqs, const 
       launch_tile_diagonals(src, ta, (const TileJob*)ctx->d_tiles, diag_off, st32);
       CK(hipGetLastError());
     }
+    // the packed kernel after the int32 ones (another stream): their fewer,
+    // longer workgroups are placed first, the packed items fill the rest
+    if (n0 > 0) {
+      F16Item* d_items = (F16Item*)ctx->d_f16;
+      int2* d_tstrips = (int2*)(d_items + items.size());
+      CK(hipMemcpyAsync(d_items, items.data(), sizeof(F16Item) * items.size(), hipMemcpyHostToDevice,
+                        ctx->stream));
+      const uint8_t* tab = d_tab;
+      if (src == SRC_SEQ && !prof) {
+        CK(hipMemcpyAsync(d_tstrips, tstrips.data(), sizeof(int2) * tstrips.size(), hipMemcpyHostToDevice,
+                          ctx->stream));
+        launch_seq_table((const AlignJob*)ctx->d_jobs, d_tstrips, (int)tstrips.size(), d_seqs, ctx->d_S,
+                         ctx->go + ctx->ge, (uint8_t*)ctx->d_tab16, ctx->stream);
+        CK(hipGetLastError());
+        tab = (const uint8_t*)ctx->d_tab16;
+      }
+      CK(hipEventRecord(ev[6], ctx->stream));
+      F16Args f;
+      f.jobs = (const AlignJob*)ctx->d_jobs;
+      f.items = d_items;
+      f.nitems = (int64_t)items.size();
+      f.tab = tab;
+      f.flags = (uint32_t*)ctx->d_flags;
+      f.scores = (int32_t*)ctx->d_jscore;
+      f.bufs = ctx->d_bufs;
+      f.bufcap = bufcap;
+      f.go = ctx->go;
+      f.ge = ctx->ge;
+      f.prof = prof ? 1 : 0;
+      f.smin = ctx->smin;
+      f.seqs = d_seqs;
+      f.subst = ctx->d_S;
+      launch_flag16(f, grid0, ctx->stream);
+      CK(hipGetLastError());
+    } else {
+      CK(hipEventRecord(ev[6], ctx->stream));
+    }
+    CK(hipEventRecord(ev[7], ctx->stream));  // (the packed kernel's end)
     CK(hipEventRecord(ev[4], st32));
     if (side) CK(hipStreamWaitEvent(ctx->stream, ev[4], 0));
     CK(hipEventRecord(ev[8], ctx->stream));  // every flag kernel is done

@@ -58,7 +58,8 @@ struct mst_ctx {
   int go = 10, ge = 2;
   int8_t S[576];
   bool symmetric = true;
-  int smax = 0;  // largest entry of S
+  int smax = 0;  // largest entry of S (at least 0)
+  int smin = 0;  // least entry of S
   int8_t* d_S = nullptr;
   void* d_scratch = nullptr;  // this device's address of the no-allocation scratch array
   std::string err;

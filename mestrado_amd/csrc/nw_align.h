@@ -119,6 +119,13 @@ struct F16Args {
   int2* bufs;
   int64_t bufcap;
   int go, ge;
+  // prof = 1 (sequence jobs, every S - smin in 0..15): no table; each pass
+  // builds its strips' query profiles in LDS, a nibble per row (S - smin),
+  // from seqs and subst, and a step reads the word of its column's residue
+  int prof = 0;
+  int smin = 0;
+  const int8_t* seqs = nullptr;
+  const int8_t* subst = nullptr;
 };
 void launch_flag16(const F16Args& a, int grid, hipStream_t st);
 // Items of the packed jobs jobs[0, n): each job's k set, jobs of one k

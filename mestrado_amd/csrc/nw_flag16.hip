@@ -60,6 +60,8 @@ struct F16Strip {
   uint32_t* fb;
   uint32_t sel;          // lane 0's row above: bytes of (rotated word, row buffer word)
   uint32_t cua, cub;     // 0x10000 where the half is a job's first strip (contUp = 0 in row 0)
+  const int8_t* xa;      // profile mode: column residues of each half's job
+  const int8_t* xb;
 };
 
 __device__ __forceinline__ uint32_t bfi_u(uint32_t mask, uint32_t a, uint32_t b) {
@@ -81,7 +83,7 @@ constexpr uint32_t kSelPair = 0x05040302u;     // a: the buffer's half b (previo
 // some lane may be outside a half's columns; the inactive half keeps its
 // state, so each half's lanes end holding column n-1 of their rows. bf, bh:
 // F and H of the row buffer at column t for lane 0 (wave-uniform).
-template <int K, bool PRED, bool STORE>
+template <int K, bool PRED, bool STORE, bool NIB = false>
 __device__ __forceinline__ void f16_step(F16Lane<K>& L, int t, uint2 wa, uint2 wb, uint32_t bf, uint32_t bh,
                                          const F16Strip& S, s2 G2, int2* bout) {
   const int lane = lane_id();
@@ -101,9 +103,13 @@ __device__ __forceinline__ void f16_step(F16Lane<K>& L, int t, uint2 wa, uint2 w
     const uint32_t keep = (aa ? 0xFFFFu : 0u) | (ab ? 0xFFFF0000u : 0u);
 #pragma unroll
     for (int i = 0; i < K; ++i) {
-      // row i's table bytes of the two halves, zero-extended into them
-      const uint32_t sel = 0x0C000C00u | ((uint32_t)(4 + (i & 3)) << 16) | (uint32_t)(i & 3);
-      const s2 sb = as_s2(__builtin_amdgcn_perm(i < 4 ? wb.x : wb.y, i < 4 ? wa.x : wa.y, sel));
+      // row i's score bytes of the two halves, zero-extended into them: byte
+      // i&3 of word i/4 (table), or byte i/2 of word i&1 (profile words split
+      // into even and odd rows)
+      const int bi = NIB ? (i >> 1) : (i & 3);
+      const bool hiw = NIB ? (i & 1) != 0 : i >= 4;
+      const uint32_t sel = 0x0C000C00u | ((uint32_t)(4 + bi) << 16) | (uint32_t)bi;
+      const s2 sb = as_s2(__builtin_amdgcn_perm(hiw ? wb.y : wb.x, hiw ? wa.y : wa.x, sel));
       const s2 eo = L.E[i], ho = L.Hm[i];
       const s2 e = pmax(eo, ho);
       const s2 s3 = eo - ho;
@@ -208,6 +214,111 @@ __device__ void f16_pass(F16Lane<K>& L, const F16Strip& S, const int2* bin, int2
   }
 }
 
+// Profile mode. The residues lane l needs in the block of steps t0 .. t0+7
+// of a half (columns t0-l .. t0+7-l of its job, clamped into 0 .. n-1; none
+// when the half is idle), one per byte.
+__device__ __forceinline__ uint2 f16_codes(const int8_t* __restrict__ X, int t0, int n) {
+  if (n <= 0) return make_uint2(0u, 0u);
+  const int c0 = t0 - lane_id();
+  uint2 v;
+  if (c0 >= 0 && c0 + 8 <= n) {
+    __builtin_memcpy(&v, X + c0, sizeof v);
+  } else {
+    uint32_t q[2] = {0u, 0u};
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      int c = c0 + j;
+      c = c < 0 ? 0 : (c >= n ? n - 1 : c);
+      q[j >> 2] |= (uint32_t)(uint8_t)X[c] << (8 * (j & 3));
+    }
+    v = make_uint2(q[0], q[1]);
+  }
+  return v;
+}
+
+__device__ __forceinline__ int f16_code_at(uint2 v, int j) {
+  return (int)(((j < 4 ? v.x : v.y) >> (8 * (j & 3))) & 0xFFu);
+}
+
+// A profile word as two words of bytes S + go + ge: rows 0, 2, 4, 6 and rows
+// 1, 3, 5, 7 (nibble plus the offset smin + go + ge, at most 255).
+__device__ __forceinline__ uint2 f16_unpack(uint32_t w, uint32_t off4) {
+  return make_uint2((w & 0x0F0F0F0Fu) + off4, ((w >> 4) & 0x0F0F0F0Fu) + off4);
+}
+
+// The strip's query profile: word [x][lane] holds, at bits 4i..4i+3, S[Y[r]][x]
+// - smin of the lane's row r = r0 + K lane + i (0 past the job's m rows, a row
+// below every real one). Each lane writes and later reads only its own words.
+template <int K>
+__device__ __forceinline__ void f16_build_prof(uint32_t* prof, const int8_t* Sn, const int8_t* Y, int m, int r0) {
+  const int lane = lane_id();
+  int yr[K];
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    const int r = r0 + lane * K + i;
+    yr[i] = r < m ? (int)Y[r] * MST_LETTERS : -1;
+  }
+  for (int x = 0; x < MST_LETTERS; ++x) {
+    uint32_t w = 0;
+#pragma unroll
+    for (int i = 0; i < K; ++i)
+      if (yr[i] >= 0) w |= (uint32_t)(uint8_t)Sn[yr[i] + x] << (4 * i);
+    prof[x * MST_WAVE + lane] = w;
+  }
+}
+
+// One pass in profile mode: blocks of 8 steps; each step's profile words are
+// read from LDS at the block's start, the next block's residues loaded
+// meanwhile.
+template <int K, bool READ, bool STORE>
+__device__ __forceinline__ void f16_pass_prof(F16Lane<K>& L, const F16Strip& S, const int2* bin, int2* bout, s2 G2,
+                              const uint32_t* profA, const uint32_t* profB, uint32_t off4) {
+  constexpr int U = 2;
+  const int lane = lane_id();
+  const int T = S.n + MST_WAVE - 1;
+  const int bn = S.na > 0 ? S.na : 1;  // columns of the row buffer (half a's)
+  int2 r0 = make_int2(0, 0), r1 = make_int2(0, 0);
+  if (READ) {
+    r0 = bin[lane < bn ? lane : bn - 1];
+    r1 = bin[MST_WAVE + lane < bn ? MST_WAVE + lane : bn - 1];
+  }
+  uint2 ca = f16_codes(S.xa, 0, S.na), cb = f16_codes(S.xb, -S.db, S.nb);
+  for (int t0 = 0; t0 < T; t0 += U) {
+    uint32_t wa[U], wb[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      wa[j] = profA[f16_code_at(ca, j) * MST_WAVE + lane];
+      wb[j] = profB[f16_code_at(cb, j) * MST_WAVE + lane];
+    }
+    ca = f16_codes(S.xa, t0 + U, S.na);
+    cb = f16_codes(S.xb, t0 + U - S.db, S.nb);
+    const bool full =
+        (t0 >= MST_WAVE - 1 + S.db) && (t0 + U <= S.na) && (t0 + U <= S.nb + S.db);
+    const int q = t0 & (MST_WAVE - 1);  // lane of the row-buffer word of step t0
+    if (full) {
+#pragma unroll
+      for (int j = 0; j < U; ++j)
+        f16_step<K, false, STORE, true>(L, t0 + j, f16_unpack(wa[j], off4), f16_unpack(wb[j], off4),
+                                        READ ? (uint32_t)__builtin_amdgcn_readlane(r0.x, q + j) : 0u,
+                                        READ ? (uint32_t)__builtin_amdgcn_readlane(r0.y, q + j) : 0u, S, G2,
+                                        bout);
+    } else {
+#pragma unroll
+      for (int j = 0; j < U; ++j)
+        if (t0 + j < T)
+          f16_step<K, true, STORE, true>(L, t0 + j, f16_unpack(wa[j], off4), f16_unpack(wb[j], off4),
+                                         READ ? (uint32_t)__builtin_amdgcn_readlane(r0.x, q + j) : 0u,
+                                         READ ? (uint32_t)__builtin_amdgcn_readlane(r0.y, q + j) : 0u, S, G2,
+                                         bout);
+    }
+    if (READ && q + U == MST_WAVE) {  // next 64 columns of the row buffer
+      r0 = r1;
+      const int c = t0 + U + MST_WAVE + lane;
+      r1 = bin[c < bn ? c : bn - 1];
+    }
+  }
+}
+
 // H of row idx of the lane's K (low half: half a, high half: half b), as an
 // or of masked rows (a select chain would let the compiler index the state).
 template <int K>
@@ -220,8 +331,9 @@ __device__ __forceinline__ int f16_pick(const F16Lane<K>& L, int idx, bool hi) {
 
 // One item on the calling wave: two jobs of K rows per lane, strips in step
 // (mode 0; b < 0: job a alone), or one job two strips per pass (mode 1).
-template <int K>
-__device__ void f16_item(const F16Args& a, const F16Item item, int2* buf, s2 G2) {
+template <int K, bool PROF>
+__device__ __forceinline__ void f16_item(const F16Args& a, const F16Item item, int2* buf, s2 G2, uint32_t* profA,
+                         uint32_t* profB, const int8_t* Sn) {
   constexpr int FSH = MST_WAVE * K;
   const int lane = lane_id();
   const AlignJob ja = a.jobs[item.a];
@@ -252,6 +364,12 @@ __device__ void f16_item(const F16Args& a, const F16Item item, int2* buf, s2 G2)
     S.sel = pair ? (p == 0 ? kSelFirstPair : kSelPair) : (p == 0 ? kSelZero : kSelBuf);
     S.cua = xa == 0 ? 0x00010000u : 0u;
     S.cub = xb == 0 ? 0x00010000u : 0u;
+    S.xa = PROF ? a.seqs + ja.x_off : nullptr;
+    S.xb = PROF ? a.seqs + jb.x_off : nullptr;
+    if constexpr (PROF) {
+      if (xa >= 0) f16_build_prof<K>(profA, Sn, a.seqs + ja.y_off, ja.m, xa * FSH);
+      if (xb >= 0) f16_build_prof<K>(profB, Sn, a.seqs + jb.y_off, jb.m, xb * FSH);
+    }
 #pragma unroll
     for (int i = 0; i < K; ++i) {
       L.Hm[i] = (s2){0, 0};
@@ -261,16 +379,31 @@ __device__ void f16_item(const F16Args& a, const F16Item item, int2* buf, s2 G2)
     const int2* bin = buf + (int64_t)(p & 1) * a.bufcap;
     int2* bout = buf + (int64_t)((p + 1) & 1) * a.bufcap;
     const bool more = p + 1 < np;
-    if (p == 0) {
-      if (more)
-        f16_pass<K, false, true>(L, S, bin, bout, G2);
-      else
-        f16_pass<K, false, false>(L, S, bin, bout, G2);
+    if constexpr (PROF) {
+      const uint32_t off4 = 0x01010101u * (uint32_t)(a.smin + a.go + a.ge);
+      if (p == 0) {
+        if (more)
+          f16_pass_prof<K, false, true>(L, S, bin, bout, G2, profA, profB, off4);
+        else
+          f16_pass_prof<K, false, false>(L, S, bin, bout, G2, profA, profB, off4);
+      } else {
+        if (more)
+          f16_pass_prof<K, true, true>(L, S, bin, bout, G2, profA, profB, off4);
+        else
+          f16_pass_prof<K, true, false>(L, S, bin, bout, G2, profA, profB, off4);
+      }
     } else {
-      if (more)
-        f16_pass<K, true, true>(L, S, bin, bout, G2);
-      else
-        f16_pass<K, true, false>(L, S, bin, bout, G2);
+      if (p == 0) {
+        if (more)
+          f16_pass<K, false, true>(L, S, bin, bout, G2);
+        else
+          f16_pass<K, false, false>(L, S, bin, bout, G2);
+      } else {
+        if (more)
+          f16_pass<K, true, true>(L, S, bin, bout, G2);
+        else
+          f16_pass<K, true, false>(L, S, bin, bout, G2);
+      }
     }
     if (more) agent_fence();
     // scores from the lanes holding the last rows
@@ -288,21 +421,31 @@ __device__ void f16_item(const F16Args& a, const F16Item item, int2* buf, s2 G2)
 }
 
 // Items of every rows-per-lane class in one launch (their order is the
-// host's, largest first), each wave taking items round robin.
+// host's, largest first), each wave taking items round robin. PROF: the
+// strips' profiles in LDS (12 KB: twelve waves per compute unit).
+template <bool PROF>
 __global__ void __launch_bounds__(MST_WAVE) flag16_kernel(F16Args a) {
+  __shared__ uint32_t prof[PROF ? 2 * MST_LETTERS * MST_WAVE : 1];
+  __shared__ int8_t Sn[PROF ? MST_LETTERS * MST_LETTERS : 1];
+  if (PROF)
+    for (int i = threadIdx.x; i < MST_LETTERS * MST_LETTERS; i += MST_WAVE)
+      Sn[i] = (int8_t)(a.subst[i] - a.smin);
+  __syncthreads();
+  uint32_t* profA = prof;
+  uint32_t* profB = prof + (PROF ? MST_LETTERS * MST_WAVE : 0);
   const s2 G2 = (s2){(short)(a.go - a.ge), (short)(a.go - a.ge)};
   int2* buf = a.bufs + (int64_t)blockIdx.x * 2 * a.bufcap;
   for (int64_t it = blockIdx.x; it < a.nitems; it += gridDim.x) {
     const F16Item item = a.items[it];
     switch (item.k) {
-      case 1: f16_item<1>(a, item, buf, G2); break;
-      case 2: f16_item<2>(a, item, buf, G2); break;
-      case 3: f16_item<3>(a, item, buf, G2); break;
-      case 4: f16_item<4>(a, item, buf, G2); break;
-      case 5: f16_item<5>(a, item, buf, G2); break;
-      case 6: f16_item<6>(a, item, buf, G2); break;
-      case 7: f16_item<7>(a, item, buf, G2); break;
-      default: f16_item<8>(a, item, buf, G2); break;
+      case 1: f16_item<1, PROF>(a, item, buf, G2, profA, profB, Sn); break;
+      case 2: f16_item<2, PROF>(a, item, buf, G2, profA, profB, Sn); break;
+      case 3: f16_item<3, PROF>(a, item, buf, G2, profA, profB, Sn); break;
+      case 4: f16_item<4, PROF>(a, item, buf, G2, profA, profB, Sn); break;
+      case 5: f16_item<5, PROF>(a, item, buf, G2, profA, profB, Sn); break;
+      case 6: f16_item<6, PROF>(a, item, buf, G2, profA, profB, Sn); break;
+      case 7: f16_item<7, PROF>(a, item, buf, G2, profA, profB, Sn); break;
+      default: f16_item<8, PROF>(a, item, buf, G2, profA, profB, Sn); break;
     }
   }
 }
@@ -334,7 +477,10 @@ void plan_flag16_items(const AlignJob* jobs, int64_t n, std::vector<F16Item>& it
 
 void launch_flag16(const F16Args& a, int grid, hipStream_t st) {
   if (a.nitems <= 0) return;
-  hipLaunchKernelGGL(flag16_kernel, dim3(grid), dim3(MST_WAVE), 0, st, a);
+  if (a.prof)
+    hipLaunchKernelGGL(flag16_kernel<true>, dim3(grid), dim3(MST_WAVE), 0, st, a);
+  else
+    hipLaunchKernelGGL(flag16_kernel<false>, dim3(grid), dim3(MST_WAVE), 0, st, a);
 }
 
 // The score table of sequence jobs in the flag kernels' layout: byte i of
