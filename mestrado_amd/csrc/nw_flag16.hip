@@ -273,7 +273,7 @@ __device__ __forceinline__ void f16_build_prof(uint32_t* prof, const int8_t* Sn,
 template <int K, bool READ, bool STORE>
 __device__ __forceinline__ void f16_pass_prof(F16Lane<K>& L, const F16Strip& S, const int2* bin, int2* bout, s2 G2,
                               const uint32_t* profA, const uint32_t* profB, uint32_t off4) {
-  constexpr int U = 2;
+  constexpr int U = 4;
   const int lane = lane_id();
   const int T = S.n + MST_WAVE - 1;
   const int bn = S.na > 0 ? S.na : 1;  // columns of the row buffer (half a's)
