@@ -67,8 +67,21 @@ def profiled_traffic(kernel):
     return None
 
 
-def cpu_baseline(seqs, target_seconds=12.0):
-    """Oracle recurrence on the host cores over a random sample of pairs."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(seqs, target_seconds=4.0, runs=3):
+    """Oracle recurrence on the host cores over random samples of pairs: the
+    median of `runs` samples of about `target_seconds` each. Threads: the CPU
+    share the environment grants this process (OMP_NUM_THREADS; a GPU box
+    gives each GPU's job its own share of the host), else every core."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_bind as O
 
@@ -76,31 +89,40 @@ def cpu_baseline(seqs, target_seconds=12.0):
     codes, offs = O.pack_seqs(seqs)
     n = len(seqs)
     L = np.array([len(s) for s in seqs], dtype=np.int64)
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    nproc = os.cpu_count() or 1
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or nproc
     rng = np.random.default_rng(123)
-    # calibrate on a small sample, then size the measured sample
+    # calibrate on a small sample, then size the measured samples
     pi = rng.integers(0, n, 256)
     pj = rng.integers(0, n, 256)
     keep = pi != pj
     pi, pj = pi[keep], pj[keep]
     _, secs = O.pair_list(codes, offs, pi, pj, S, threads=threads)
     rate = (L[pi] * L[pj]).sum() / max(secs, 1e-6)
-    want_cells = rate * target_seconds
     mean_cells = float((L.mean()) ** 2)
-    npairs = int(max(512, min(2_000_000, want_cells / mean_cells)))
-    pi = rng.integers(0, n, npairs)
-    pj = rng.integers(0, n, npairs)
-    keep = pi != pj
-    pi, pj = pi[keep], pj[keep]
-    _, secs = O.pair_list(codes, offs, pi, pj, S, threads=threads)
-    cells = int((L[pi] * L[pj]).sum())
+    npairs = int(max(512, min(2_000_000, rate * target_seconds / mean_cells)))
+    vals, tot_cells, tot_secs, tot_pairs = [], 0, 0.0, 0
+    for _ in range(runs):
+        pi = rng.integers(0, n, npairs)
+        pj = rng.integers(0, n, npairs)
+        keep = pi != pj
+        pi, pj = pi[keep], pj[keep]
+        _, secs = O.pair_list(codes, offs, pi, pj, S, threads=threads)
+        cells = int((L[pi] * L[pj]).sum())
+        vals.append(cells / secs / 1e9)
+        tot_cells += cells
+        tot_secs += secs
+        tot_pairs += len(pi)
     return {
-        "value": round(cells / secs / 1e9, 3),
+        "value": round(float(np.median(vals)), 3),
         "unit": "GCUPS",
         "cores": threads,
+        "nproc": nproc,
+        "cpu_model": cpu_model(),
+        "runs": [round(v, 3) for v in vals],
         "kind": "port",
-        "sample": "%d random pairs (i != j) of the same 4,000-sequence set, %.3g cells, %.1f s wall"
-                  % (len(pi), cells, secs),
+        "sample": "median of %d samples of %d random pairs (i != j) of the same 4,000-sequence set, "
+                  "%.3g cells and %.1f s wall in all" % (runs, len(pi), tot_cells, tot_secs),
     }
 
 
