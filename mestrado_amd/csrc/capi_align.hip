@@ -78,7 +78,8 @@ int run_align_jobs(mst_ctx* ctx, int src, const int8_t* d_seqs, const uint8_t* d
   for (size_t k = 0; k < nj; ++k) cls[k] = align_class(ctx, jobs[k]);
   std::stable_sort(perm.begin(), perm.end(), [&](size_t x, size_t y) {
     if (cls[x] != cls[y]) return cls[x] < cls[y];
-    if (cls[x] != 0) return false;
+    if (cls[x] != 0)  // the int32 kernels take jobs in order: largest first
+      return (int64_t)jobs[x].n * jobs[x].m > (int64_t)jobs[y].n * jobs[y].m;
     // the packed kernel's pairs: by rows per lane, strips, then columns
     const int kx = flag16_k(jobs[x].m), ky = flag16_k(jobs[y].m);
     if (kx != ky) return kx > ky;
@@ -197,7 +198,7 @@ static int run_align_batches(mst_ctx* ctx, int src, const int8_t* d_seqs, const 
         if (rc) return rc;
       }
     }
-    rc = ctx->ensure((void**)&ctx->d_acounter, &ctx->acounter_cap, sizeof(unsigned int));
+    rc = ctx->ensure((void**)&ctx->d_acounter, &ctx->acounter_cap, 2 * sizeof(unsigned int));
     if (rc) return rc;
     CK(hipMemcpyAsync(ctx->d_jobs, jobs.data() + k0, sizeof(AlignJob) * nb, hipMemcpyHostToDevice,
                       ctx->stream));
@@ -291,6 +292,8 @@ static int run_align_batches(mst_ctx* ctx, int src, const int8_t* d_seqs, const 
       f.smin = ctx->smin;
       f.seqs = d_seqs;
       f.subst = ctx->d_S;
+      f.counter = ctx->d_acounter + 1;
+      CK(hipMemsetAsync(f.counter, 0, sizeof(unsigned int), ctx->stream));
       launch_flag16(f, grid0, ctx->stream);
       CK(hipGetLastError());
     } else {

@@ -114,7 +114,7 @@ struct mst_ctx {
 
   ~mst_ctx();
   int fail(int code, const char* what, const char* detail = nullptr);
-  unsigned int* d_acounter = nullptr;  // work queue head of the pipelined flag kernel
+  unsigned int* d_acounter = nullptr;  // work queue heads: the pipelined flag kernel, the packed one
   size_t acounter_cap = 0;
   void* d_tiles = nullptr;  // tiled flag kernel: tile lists, row buffers, lane states
   size_t tiles_cap = 0;

@@ -126,6 +126,9 @@ struct F16Args {
   int smin = 0;
   const int8_t* seqs = nullptr;
   const int8_t* subst = nullptr;
+  // work queue head (zero before the launch): waves take items in order as
+  // they get a slot; null: wave w takes items w, w + grid, ...
+  unsigned int* counter = nullptr;
 };
 void launch_flag16(const F16Args& a, int grid, hipStream_t st);
 // Items of the packed jobs jobs[0, n): each job's k set, jobs of one k

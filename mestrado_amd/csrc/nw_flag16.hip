@@ -421,7 +421,8 @@ __device__ __forceinline__ void f16_item(const F16Args& a, const F16Item item, i
 }
 
 // Items of every rows-per-lane class in one launch (their order is the
-// host's, largest first), each wave taking items round robin. PROF: the
+// host's, largest first), each wave taking the next item from the queue as it
+// finishes one (or round robin without a queue). PROF: the
 // strips' profiles in LDS (12 KB: twelve waves per compute unit).
 template <bool PROF>
 __global__ void __launch_bounds__(MST_WAVE) flag16_kernel(F16Args a) {
@@ -435,7 +436,10 @@ __global__ void __launch_bounds__(MST_WAVE) flag16_kernel(F16Args a) {
   uint32_t* profB = prof + (PROF ? MST_LETTERS * MST_WAVE : 0);
   const s2 G2 = (s2){(short)(a.go - a.ge), (short)(a.go - a.ge)};
   int2* buf = a.bufs + (int64_t)blockIdx.x * 2 * a.bufcap;
-  for (int64_t it = blockIdx.x; it < a.nitems; it += gridDim.x) {
+  for (int64_t it = a.counter ? (int64_t)__builtin_amdgcn_readfirstlane(
+                                    threadIdx.x == 0 ? (int)atomicAdd(a.counter, 1u) : 0)
+                              : (int64_t)blockIdx.x;
+       it < a.nitems;) {
     const F16Item item = a.items[it];
     switch (item.k) {
       case 1: f16_item<1, PROF>(a, item, buf, G2, profA, profB, Sn); break;
@@ -447,6 +451,8 @@ __global__ void __launch_bounds__(MST_WAVE) flag16_kernel(F16Args a) {
       case 7: f16_item<7, PROF>(a, item, buf, G2, profA, profB, Sn); break;
       default: f16_item<8, PROF>(a, item, buf, G2, profA, profB, Sn); break;
     }
+    it = a.counter ? (int64_t)__builtin_amdgcn_readfirstlane(threadIdx.x == 0 ? (int)atomicAdd(a.counter, 1u) : 0)
+                   : it + gridDim.x;
   }
 }
 
