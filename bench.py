@@ -59,7 +59,7 @@ def profiled_traffic(kernel):
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic "):
+        if line.startswith("traffic ") and " fetch_KiB=" in line and " write_KiB=" in line:
             name, rest = line[len("traffic "):].split(" fetch_KiB=")
             if name == kernel:
                 fetch, write = rest.split(" write_KiB=")
