@@ -1,9 +1,9 @@
 """Autotuner of the batched all-pairs form.
 
 Searches the tunables of include/mestrado.h (mst_tune_set: "chunk" column
-sequences per work item, "wg5" workgroups per CU of the streamed kernel, "k5min" least rows per lane of its
-one-wave classes, "k1max" most rows per lane of a one-wave pipeline,
-"chunk_w3" item size of its 3- and 4-wave pipelines) on a workload, timing each
+sequences per work item, "chunk_k9" the item size of the classes of 9 or more
+rows per lane, "k8" the rows that take the strips-in-turn kernel and "chunk8"
+its item size; KEYS lists every tunable the table can hold) on a workload, timing each
 candidate with the resident all-pairs pass (device time, plan and graph built
 outside the timed region), and persists the best per device architecture and
 length bucket in mestrado_amd/tuning/<arch>.json:
@@ -27,7 +27,7 @@ import numpy as np
 
 TUNING_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning")
 KEYS = ("chunk", "wg5", "k5min", "chunk_w3", "k1max", "k2max", "branches", "wmax", "k6", "chunk6",
-        "chunk_k9", "s6p2", "stack")
+        "chunk_k9", "s6p2", "stack", "rebase", "k8", "chunk8")
 BUCKETS = ((256, "<=256"), (1024, "<=1024"), (None, ">1024"))
 
 
@@ -82,10 +82,15 @@ def apply_for(ctx, lengths):
 
 
 def candidates():
-    # k5min = 3 lost by 5-8 ms on the 4,000-protein set in a wider search, k1max 14-16 and items of 72 or more sequences by
-    # 3-70 ms; the search keeps the knobs that stayed close
-    for chunk, chunk6, k1max, chunk_k9 in itertools.product((32, 48, 64), (2, 4), (12, 13), (0, 32)):
-        yield {"chunk": chunk, "chunk6": chunk6, "wg5": 0, "k5min": 1, "k1max": k1max, "chunk_k9": chunk_k9}
+    # k5min = 3 lost by 5-8 ms on the 4,000-protein set in a wider search, k1max 14-16 and items of 72 or
+    # more sequences by 3-70 ms, k1max 12 and chunk6 2 by 1-3 ms; the search keeps the knobs that stayed
+    # close: the item sizes and where the strips-in-turn kernel (score8) takes over (k8 0: rows above
+    # 1,024 on the multi-sweep kernel, 2: rows above 832 on score8)
+    for chunk, chunk_k9, k8, chunk8 in itertools.product((48, 64), (0, 32), (0, 2), (4, 8)):
+        if k8 == 0 and chunk8 != 4:
+            continue  # chunk8 only sizes score8's items
+        yield {"chunk": chunk, "chunk6": 4, "wg5": 0, "k5min": 1, "k1max": 13, "chunk_k9": chunk_k9,
+               "k8": k8, "chunk8": chunk8}
 
 
 def tune(ctx, seqs, reps=2, log=None):

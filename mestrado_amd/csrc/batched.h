@@ -15,6 +15,8 @@ struct BClass {
   int64_t pairs = 0;
   int W = 1;          // waves per pipeline (pipelined packed kernel)
   bool stacked = false;  // two row pairs per strip (one-wave kernel)
+  bool rebase = false;   // score8: re-based items (one column sequence per wave)
+  int64_t cap = 0;       // score8: most columns of one wave's part
 };
 // Tunables of the batched form (set through mst_tune_set; the autotuner in
 // mestrado_amd/autotune.py searches them and persists the best per device).
@@ -34,6 +36,13 @@ struct Tune {
                              // per lane in all (0: chunk)
   int s6p2 = 0;              // 1: the multi-sweep classes as two pipelines per workgroup (score7)
   int stack = 0;             // 1: two row pairs of K = 2, 3 rows per lane share a strip of 2K
+  int rebase = 1;            // pairs past the 16-bit bound: the re-based packed kernel (score8)
+                             // instead of the int32 kernel when 2, or when 1 and there are
+                             // four items per compute unit; 0: the int32 kernel
+  int k8 = 2;                // rows that take score8 (strips in turn on one wave): 0 none
+                             // besides the re-based pairs, 1 rows above 1024 (instead of the
+                             // multi-sweep kernel), 2 also the two-wave pipelines' rows
+  int chunk8 = 4;            // column sequences per plain score8 item (four waves' parts)
 };
 
 struct BatchedPlan {
@@ -46,6 +55,8 @@ struct BatchedPlan {
   std::vector<mst::Item3> items6;  // multi-sweep packed kernel (score6)
   std::vector<BClass> c6;
   int64_t carry_cols = 0;          // longest score6 column stream
+  std::vector<mst::Item3> items8;  // strips-in-turn packed kernel (score8)
+  std::vector<BClass> c8;
   std::vector<int32_t> fb_i, fb_j;  // int32 fallback pairs (column, row)
   // device copies
   bool uploaded = false;
@@ -54,6 +65,11 @@ struct BatchedPlan {
   mst::Item3* d_items6 = nullptr;
   int2* d_carry = nullptr;      // score6 row buffers, one per workgroup
   int grid6 = 0;
+  mst::Item3* d_items8 = nullptr;
+  int2* d_carry8 = nullptr;     // score8 row buffers (per class, workgroup and wave)
+  int2* d_frames8 = nullptr;    // and their frames (re-based classes)
+  int grid8 = 0;
+  std::vector<int64_t> carry8_off, frames8_off;  // per score8 class, in int2 elements
   int8_t* d_scodes = nullptr;   // codes in length order
   int64_t* d_soff = nullptr;    // their offsets by sorted position
   unsigned int* d_counters = nullptr;

@@ -81,6 +81,50 @@ bool pk16_usable(const mst_ctx* ctx) {
 constexpr int kOneWaveShortK = 5;
 constexpr int64_t kScore6MaxCols = 16384;
 
+// score8's strips for m rows: ns strips of at most 512 rows, K rows per lane
+static int score8_k(int m, int* ns) {
+  int s = (m + 511) / 512;
+  if (s < 1) s = 1;
+  *ns = s;
+  const int K = (m + 64 * s - 1) / (64 * s);
+  return std::max(kScore8MinK, std::min(kScore8MaxK, K));
+}
+
+// A re-based wave's values stay within 16 bits whatever the lengths: in the
+// biased form a value changes by -G .. G + smax + 2 ge from one row or column
+// to the next (G = go - ge), so the front of a strip of 64K rows spans at most
+// up*(64K-1) + dn*63 above and dn*(64K-1) + up*63 below lane 0's first row,
+// which the frame keeps within kRebaseAt (checked every 8 steps, and lane 0
+// drifting for the 63 steps after its last column); E, F and the diagonal
+// term add at most up + G + 255.
+static bool rebase_fits(const mst_ctx* ctx, int K) {
+  const int64_t G = ctx->go - ctx->ge;
+  if (G < 0) return false;
+  const int64_t up = G + ctx->smax + 2 * ctx->ge, dn = G;
+  const int64_t rows = 64 * K - 1;
+  const int64_t drift = (8 + 63) * up;
+  const int64_t hi = kRebaseAt + drift + up * rows + dn * 63 + G + 255;
+  const int64_t lo = kRebaseAt + drift + dn * rows + up * 63 + up + G;
+  return hi <= 32767 && lo <= 32768;
+}
+
+// The parts item_part (nw_stream.h) cuts an item's columns into, on the host:
+// the most columns one part holds.
+static int64_t max_part_cols(const std::vector<int64_t>& soff, int xb0, int xe0, int P) {
+  const int64_t c0 = soff[xb0], total = soff[xe0] - c0;
+  auto cut = [&](int q) {
+    if (q <= 0) return xb0;
+    if (q >= P) return xe0;
+    const int64_t target2 = 2 * (total * q / P);
+    for (int k = xb0; k < xe0; ++k)
+      if ((soff[k] - c0) + (soff[k + 1] - c0) >= target2) return k;
+    return xe0;
+  };
+  int64_t best = 0;
+  for (int p = 0; p < P; ++p) best = std::max(best, soff[cut(p + 1)] - soff[cut(p)]);
+  return best;
+}
+
 static int rows_per_lane(int m, int* nstrips) {
   int s = (m + 1023) / 1024;
   if (s < 1) s = 1;
@@ -105,7 +149,10 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
     int64_t cost, cells, pairs;
     int2 p2 = make_int2(-1, -1);  // stacked items: the second row pair
   };
-  std::vector<It3> v5, v6;
+  std::vector<It3> v5, v6, v8, v8r;  // v8r: re-based score8 items
+  std::vector<int64_t> soff(N + 1, 0);  // code offsets by sorted position
+  for (int k = 0; k < N; ++k) soff[k + 1] = soff[k] + len(order[k]);
+  int64_t cap8 = 0, cap8r = 0;
   for (int k = 0; 2 * k < N; ++k) {
     const int ya = order[2 * k];
     const bool single = (2 * k + 1 >= N);
@@ -153,17 +200,31 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
       }
       kind = 6;
     }
+    // score8 (strips in turn on one wave) for the rows the k8 setting names
+    if ((kind == 6 && ctx->tune.k8 >= 1) || (kind == 5 && W >= 2 && ctx->tune.k8 >= 2)) kind = 8;
+    int ns8;
+    const int K8 = score8_k(m, &ns8);
+    if (kind == 8) {
+      K = K8;
+      W = 1;
+    }
+    const bool rebase = ctx->tune.rebase && rebase_fits(ctx, K8);
     // padded rows (bound of the 16-bit values)
-    const int64_t R = kind == 5 ? (int64_t)W * 64 * K : (int64_t)(m + 64 * K - 1) / (64 * K) * 64 * K;
+    const int64_t R = kind == 5 ? (int64_t)W * 64 * K
+                    : kind == 8 ? (int64_t)ns8 * 64 * K8
+                                : (int64_t)(m + 64 * K - 1) / (64 * K) * 64 * K;
     // rows (ya, yb) against columns s[first..last]; yb < 0: ya alone. The
     // streamed kernels name rows by position (pa, pb), the others by id.
     auto add_cols = [&](int first, int last, int pa, int pb) {
       const int ya_ = order[pa], yb_ = pb >= 0 ? order[pb] : -1;
-      std::vector<int> elig;
+      const int64_t rows = len(ya_) + (yb_ >= 0 ? len(yb_) : 0);
+      std::vector<int> elig, past;  // past: beyond the 16-bit bound, re-based
       for (int t = first; t <= last; ++t) {
         const int x = order[t];
         const int n = len(x);
-        if (n == 0 || pk16_bound(ctx, R, n) > 32767) {
+        if (n > 0 && pk16_bound(ctx, R, n) > 32767 && rebase) {
+          past.push_back(t);
+        } else if (n == 0 || pk16_bound(ctx, R, n) > 32767) {
           fb_i.push_back(x);
           fb_j.push_back(ya_);
           if (yb_ >= 0) {
@@ -174,10 +235,30 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
           elig.push_back(t);
         }
       }
+      // re-based items: up to four consecutive positions, one per wave
+      for (size_t c0 = 0; c0 < past.size();) {
+        size_t c1 = c0 + 1;
+        while (c1 < past.size() && c1 - c0 < (size_t)kScore8Waves && past[c1] == past[c1 - 1] + 1) ++c1;
+        Item3 it;
+        it.ya = pa;
+        it.yb = pb;
+        it.xbegin = past[c0];
+        it.xcount = (int)(c1 - c0);
+        int64_t cost = 0, cells = 0;
+        for (size_t q = c0; q < c1; ++q) {
+          const int64_t n = len(order[past[q]]);
+          cost = std::max(cost, (n + 63) * R);
+          cells += n * rows;
+          cap8r = std::max(cap8r, n);
+        }
+        v8r.push_back(It3{it, K8, 1, cost, cells, (int64_t)(c1 - c0) * (yb_ >= 0 ? 2 : 1)});
+        c0 = c1;
+      }
       {
         // pipelines of three or four waves take their own item size
         int Cw = (kind == 5 && W >= 3 && ctx->tune.chunk_w3 > 0) ? ctx->tune.chunk_w3 : C;
         if (kind == 6 && ctx->tune.chunk6 > 0) Cw = ctx->tune.chunk6;
+        if (kind == 8) Cw = ctx->tune.chunk8;
         // the classes of many rows per lane have few row pairs: smaller items
         if (kind == 5 && W <= 2 && W * K >= 9 && ctx->tune.chunk_k9 > 0) Cw = ctx->tune.chunk_k9;
         size_t c0 = 0;
@@ -197,16 +278,20 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
           it.xbegin = elig[c0];
           it.xcount = (int)(c1 - c0);
           int64_t cost = 0, cells = 0;
-          const int64_t rows = len(ya_) + (yb_ >= 0 ? len(yb_) : 0);
           for (size_t q = c0; q < c1; ++q) {
             cost += (int64_t)(len(order[elig[q]]) + 63) * 64 * K;
             cells += (int64_t)len(order[elig[q]]) * rows;
           }
           const int64_t npairs = (int64_t)(c1 - c0) * (yb_ >= 0 ? 2 : 1);
-          if (kind == 5)
+          if (kind == 5) {
             v5.push_back(It3{it, K, W, cost * W, cells, npairs});
-          else
+          } else if (kind == 8) {
+            const int64_t part = max_part_cols(soff, elig[c0], elig[c1 - 1] + 1, kScore8Waves);
+            cap8 = std::max(cap8, part);
+            v8.push_back(It3{it, K, 1, (part + 63) * R, cells, npairs});
+          } else {
             v6.push_back(It3{it, K, (int)(R / 256), cost * (R / (64 * K)), cells, npairs});
+          }
           c0 = c1;
         }
       }
@@ -313,6 +398,50 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
     for (int t = v6[q].it.xbegin; t < v6[q].it.xbegin + v6[q].it.xcount; ++t) cols += len(order[t]);
     P.carry_cols = std::max(P.carry_cols, cols);
   }
+  // score8: plain classes by K, then the re-based ones, each longest first
+  auto by_cost = [](const It3& a, const It3& b) {
+    if (a.K != b.K) return a.K > b.K;
+    return a.cost > b.cost;
+  };
+  std::stable_sort(v8.begin(), v8.end(), by_cost);
+  // Re-based items are long (every strip of a row pair on one wave): too few
+  // of them leave most of the device idle for their whole run, where the
+  // int32 kernel (a wave per pair) finishes sooner. rebase 1: only when there
+  // are four items per compute unit (the 4,000-protein set's ~400 long-pair
+  // items ran 17 ms re-based against 6 ms on the int32 kernel); 2: always.
+  if (ctx->tune.rebase == 1 && (int64_t)v8r.size() < 4 * (int64_t)ctx->num_cus) {
+    for (const It3& e : v8r) {
+      const int ya_ = order[e.it.ya], yb_ = e.it.yb >= 0 ? order[e.it.yb] : -1;
+      for (int t = e.it.xbegin; t < e.it.xbegin + e.it.xcount; ++t) {
+        fb_i.push_back(order[t]);
+        fb_j.push_back(ya_);
+        if (yb_ >= 0) {
+          fb_i.push_back(order[t]);
+          fb_j.push_back(yb_);
+        }
+      }
+    }
+    v8r.clear();
+    cap8r = 0;
+  }
+  std::stable_sort(v8r.begin(), v8r.end(), by_cost);
+  for (int r = 0; r < 2; ++r) {
+    const std::vector<It3>& v = r ? v8r : v8;
+    for (size_t q = 0; q < v.size(); ++q) {
+      const int64_t at = (int64_t)P.items8.size();
+      P.items8.push_back(v[q].it);
+      if (q == 0 || P.c8.back().K != v[q].K) {
+        BClass c{v[q].K, at, at};
+        c.rebase = r == 1;
+        c.cap = r ? cap8r : cap8;
+        P.c8.push_back(c);
+      }
+      P.c8.back().end = at + 1;
+      P.c8.back().work += v[q].cost;
+      P.c8.back().cells += v[q].cells;
+      P.c8.back().pairs += v[q].pairs;
+    }
+  }
   for (size_t q = 0; q < v5.size(); ++q) {
     P.items5.push_back(v5[q].it);
     P.pair2_5.push_back(v5[q].p2);
@@ -345,8 +474,28 @@ int run_plan_on(mst_ctx* ctx, const SeqSet& s, const PairPlan& plan, const int2*
 int upload_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P) {
   if (P.uploaded) return MST_OK;
   CK(hipMalloc(&P.d_xlist, sizeof(int32_t) * std::max<size_t>(P.xlist.size(), 1)));
-  P.ncounters = (int)(P.c5.size() + P.c6.size()) + 1;
-  const bool streamed = !P.c5.empty() || !P.c6.empty();
+  P.ncounters = (int)(P.c5.size() + P.c6.size() + P.c8.size()) + 1;
+  const bool streamed = !P.c5.empty() || !P.c6.empty() || !P.c8.empty();
+  if (!P.c8.empty()) {
+    CK(hipMalloc(&P.d_items8, sizeof(Item3) * P.items8.size()));
+    CK(hipMemcpy(P.d_items8, P.items8.data(), sizeof(Item3) * P.items8.size(), hipMemcpyHostToDevice));
+    // a row buffer per wave of every workgroup of every class (the classes may
+    // run at the same time on parallel branches of the graph), and the frames
+    // of the re-based classes' row buffers
+    P.grid8 = ctx->num_cus * score8_groups_per_cu();
+    int64_t nc = 0, nf = 0;
+    P.carry8_off.clear();
+    P.frames8_off.clear();
+    for (const BClass& c : P.c8) {
+      const int64_t waves = (int64_t)std::min<int64_t>(P.grid8, c.end - c.begin) * kScore8Waves;
+      P.carry8_off.push_back(nc);
+      P.frames8_off.push_back(nf);
+      nc += waves * std::max<int64_t>(c.cap, 1);
+      if (c.rebase) nf += waves * ((c.cap + 63) / 8 + 2);
+    }
+    CK(hipMalloc(&P.d_carry8, sizeof(int2) * std::max<int64_t>(nc, 1)));
+    CK(hipMalloc(&P.d_frames8, sizeof(int2) * std::max<int64_t>(nf, 1)));
+  }
   if (!P.c6.empty()) {
     CK(hipMalloc(&P.d_items6, sizeof(Item3) * P.items6.size()));
     CK(hipMemcpy(P.d_items6, P.items6.data(), sizeof(Item3) * P.items6.size(), hipMemcpyHostToDevice));
@@ -454,13 +603,14 @@ int enqueue_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P, int32
     }
   }
   struct Launch {
-    bool six;
+    int kind;  // 5, 6 or 8
     const BClass* c;
     int ctr;
   };
   std::vector<Launch> ls;
-  for (const BClass& c : P.c6) ls.push_back(Launch{true, &c, ctr++});
-  for (const BClass& c : P.c5) ls.push_back(Launch{false, &c, ctr++});
+  for (const BClass& c : P.c6) ls.push_back(Launch{6, &c, ctr++});
+  for (const BClass& c : P.c5) ls.push_back(Launch{5, &c, ctr++});
+  for (const BClass& c : P.c8) ls.push_back(Launch{8, &c, ctr++});
   std::vector<int> where(ls.size(), 0);
   if (nb > 1) {
     std::vector<size_t> ord(ls.size());
@@ -476,12 +626,23 @@ int enqueue_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P, int32
   for (size_t k = 0; k < ls.size(); ++k) {
     const BClass& c = *ls[k].c;
     hipStream_t st = br[where[k]];
-    Score5Args a = args5(c, ls[k].six ? P.d_items6 : P.d_items5);
+    const int kind = ls[k].kind;
+    Score5Args a = args5(c, kind == 6 ? P.d_items6 : (kind == 8 ? P.d_items8 : P.d_items5));
     a.counter = P.d_counters + ls[k].ctr;
-    if (ls[k].six)
+    if (kind == 6)
       a.carry += (int64_t)(&c - P.c6.data()) * P.grid6 * (ctx->tune.s6p2 ? 2 : 1) * a.carry_cap;
+    if (kind == 8) {
+      const size_t ci = &c - P.c8.data();
+      a.carry = P.d_carry8 + P.carry8_off[ci];
+      a.carry_cap = std::max<int64_t>(c.cap, 1);
+      a.frames = P.d_frames8 + P.frames8_off[ci];
+      a.frame_cap = (c.cap + 63) / 8 + 2;
+    }
     if (timed) CK(hipEventRecord(ctx->launch_events[ev++], st));
-    if (ls[k].six) {
+    if (kind == 8) {
+      const int g = (int)std::max<int64_t>(1, std::min<int64_t>(P.grid8, c.end - c.begin));
+      launch_score8(c.K, c.rebase, a, g, st);
+    } else if (kind == 6) {
       if (ctx->tune.s6p2)
         launch_score7(c.K, a, P.grid6, st);
       else
@@ -507,7 +668,7 @@ int enqueue_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P, int32
 
 int batched_launch_count(mst_ctx* ctx, int id) {
   const BatchedPlan& P = ctx->bplans[id];
-  return (int)(P.c6.size() + P.c5.size() +
+  return (int)(P.c6.size() + P.c5.size() + P.c8.size() +
                ctx->fbplans[id].buckets.size());
 }
 
@@ -547,6 +708,8 @@ int run_batched(mst_ctx* ctx, int id, int32_t* d_out, bool timed) {
       if ((rc = rec(-2000 - c.K, c.pairs, c.cells))) return rc;
     for (const BClass& c : P.c5)
       if ((rc = rec(c.stacked ? -(3000 + c.K) : -(1000 + 100 * c.W + c.K), c.pairs, c.cells))) return rc;
+    for (const BClass& c : P.c8)
+      if ((rc = rec(c.rebase ? -(5000 + c.K) : -(4000 + c.K), c.pairs, c.cells))) return rc;
     for (const Bucket& b : ctx->fbplans[id].buckets)
       if ((rc = rec(b.K, b.end - b.begin, b.cells))) return rc;
     return MST_OK;
@@ -581,8 +744,8 @@ int run_batched(mst_ctx* ctx, int id, int32_t* d_out, bool timed) {
 
 void free_batched(BatchedPlan& P) {
   if (P.graph) hipGraphExecDestroy(P.graph);
-  void* ps[] = {P.d_xlist, P.d_counters, P.d_items5,
-                P.d_scodes, P.d_soff, P.d_items6, P.d_carry, P.d_pair2};
+  void* ps[] = {P.d_xlist, P.d_counters, P.d_items5, P.d_scodes, P.d_soff, P.d_items6,
+                P.d_carry, P.d_pair2, P.d_items8, P.d_carry8, P.d_frames8};
   for (void* p : ps)
     if (p) hipFree(p);
   P = BatchedPlan();
@@ -604,6 +767,9 @@ static int* tune_slot(mst_ctx* ctx, const char* key) {
   if (!strcmp(key, "stack")) return &ctx->tune.stack;
   if (!strcmp(key, "chunk_k9")) return &ctx->tune.chunk_k9;
   if (!strcmp(key, "s6p2")) return &ctx->tune.s6p2;
+  if (!strcmp(key, "rebase")) return &ctx->tune.rebase;
+  if (!strcmp(key, "k8")) return &ctx->tune.k8;
+  if (!strcmp(key, "chunk8")) return &ctx->tune.chunk8;
   return nullptr;
 }
 
@@ -619,6 +785,12 @@ extern "C" int mst_tune_set(mst_ctx* ctx, const char* key, int value) {
     return ctx->fail(MST_E_ARG, "mst_tune_set", "stack is 0 or 1");
   if (!strcmp(key, "s6p2") && value != 0 && value != 1)
     return ctx->fail(MST_E_ARG, "mst_tune_set", "s6p2 is 0 or 1");
+  if (!strcmp(key, "rebase") && (value < 0 || value > 2))
+    return ctx->fail(MST_E_ARG, "mst_tune_set", "rebase is 0, 1 or 2");
+  if (!strcmp(key, "k8") && (value < 0 || value > 2))
+    return ctx->fail(MST_E_ARG, "mst_tune_set", "k8 is 0, 1 or 2");
+  if (!strcmp(key, "chunk8") && (value < 1 || value > kStreamSeqs))
+    return ctx->fail(MST_E_ARG, "mst_tune_set", "chunk8 out of range 1..64");
   if (!strcmp(key, "chunk_k9") && (value < 0 || value > kStreamSeqs))
     return ctx->fail(MST_E_ARG, "mst_tune_set", "chunk_k9 out of range 0..64");
   if (!strcmp(key, "chunk6") && (value < 0 || value > kStreamSeqs))
@@ -636,7 +808,7 @@ extern "C" int mst_tune_set(mst_ctx* ctx, const char* key, int value) {
   if (!strcmp(key, "k1max") && (value < 5 || value > 16))
     return ctx->fail(MST_E_ARG, "mst_tune_set", "k1max out of range 5..16");
   if (strcmp(key, "chunk") && strcmp(key, "chunk_w3") && strcmp(key, "chunk6") &&
-      strcmp(key, "chunk_k9") && (value < 0 || value > 64))
+      strcmp(key, "chunk_k9") && strcmp(key, "chunk8") && (value < 0 || value > 64))
     return ctx->fail(MST_E_ARG, "mst_tune_set", "workgroups per CU out of range 0..64");
   if (*slot == value) return MST_OK;
   hipSetDevice(ctx->device);

@@ -36,7 +36,9 @@ struct Score5Args {
   const int8_t* subst;
   int go, ge;
   int2* carry;             // score6: per-workgroup row buffer between sweeps
-  int64_t carry_cap;       // its columns per workgroup
+  int64_t carry_cap;       // its columns per workgroup (score8: per wave)
+  int2* frames = nullptr;  // score8, re-based items: the row buffer's frames per wave
+  int64_t frame_cap = 0;   // blocks of 8 columns per wave
   // stacked launches: the second row pair (positions) of each item, on lanes
   // 32..63 of the strip (the first on lanes 0..31); null otherwise
   const int2* pair2 = nullptr;
@@ -63,5 +65,21 @@ void launch_score6(int K, const Score5Args& a, int grid, hipStream_t st);
 int score7_block_threads();
 int score7_groups_per_cu();
 void launch_score7(int K, const Score5Args& a, int grid, hipStream_t st);
+
+// score8: a workgroup of four waves on one row pair, wave p sweeping the p-th
+// part of the item's columns through every strip of 64*K rows in turn (the
+// strips' profile shared, built once per strip), its strip's last row kept
+// in its own row buffer (`carry`, carry_cap columns per wave). Re-based items
+// (REBASE) hold one column sequence per wave and reach any length: the
+// 16-bit values are kept in a moving frame (nw_stream.h, Rebase), with the
+// row buffer's frames in `frames`. K = 4..8.
+constexpr int kScore8MinK = 4, kScore8MaxK = 8;
+// a re-based wave moves its frame when lane 0's first row leaves
+// [-kRebaseAt, kRebaseAt]
+constexpr int kRebaseAt = 16384;
+constexpr int kScore8Waves = 4;
+int score8_block_threads();
+int score8_groups_per_cu();
+void launch_score8(int K, bool rebase, const Score5Args& a, int grid, hipStream_t st);
 
 }  // namespace mst

@@ -34,6 +34,23 @@ struct Link {
   int base, mask;
 };
 
+// Re-based sweeps (REBASE): the lanes' 16-bit values stand for v + o, where
+// o is a wave-uniform offset per half (o_a, o_b) that the wave raises by lane
+// 0's first-row value whenever that value leaves [-kRebaseAt, kRebaseAt]
+// (checked once per block of 8 steps). In the biased form a value changes by
+// -8..+23 from one row or column to the next (BLOSUM62, go = 10, ge = 2; the
+// planner checks the general bound), so with 512 rows per strip every value
+// of the wave's front lies within -5,600..+12,300 of lane 0's first row, and
+// the values never wrap however long the sequences are. The row handed to the
+// next strip is written in the writer's frame; the frame of every block of 8
+// written columns goes to `fr_out` (index: the writer's step / 8), and the
+// reader converts each column by the difference of the two frames (modulo
+// 2^16, exact because the converted value is in range).
+struct Rebase {
+  int2* fr_out;       // frames of the blocks this sweep writes (null: none)
+  const int2* fr_in;  // frames of the blocks the previous strip wrote
+};
+
 // A row sequence whose last row lies in this strip: the lane holding that row
 // reports the row's H at the end of every column sequence. All fields are
 // uniform across the wave.
@@ -110,13 +127,14 @@ __device__ __forceinline__ int boundary(int bnd, int k, int nseq, int ncol) {
 
 // The owner's lane has reached a boundary: report the sequence it finished
 // (none at the stream's first column) and move on to the next.
+// oa, ob: the frame of a re-based sweep (0 otherwise).
 template <int K>
 __device__ __forceinline__ void emit_owner(const Lane2<K>& L, Owner& o, int bnd, int ord, int nseq,
-                                           const Stream& S, const Score5Args& a) {
+                                           const Stream& S, const Score5Args& a, int oa = 0, int ob = 0) {
   if (o.k >= 0) {
     const int n = boundary(bnd, o.k + 1, nseq, S.ncol) - boundary(bnd, o.k, nseq, S.ncol);
     const int64_t ox = __builtin_amdgcn_readlane(ord, o.k);
-    const int v = pick_row<K>(L, o.row, o.hi);
+    const int v = pick_row<K>(L, o.row, o.hi) + (o.hi ? ob : oa);
     if (lane_id() == o.lane) {
       const int s = v - a.ge * (o.m + n);
       int32_t* out = a.out_mat + S.obase;
@@ -186,10 +204,12 @@ __device__ __forceinline__ void step5(Lane2<K>& L, int t, const uint32_t* w, int
 // its owners have reported. Lanes outside the stream compute values nobody
 // reads; they restart at its first column, and the last boundary (ncol) ends
 // every lane's last sequence.
+// z: the boundary value in the sweep's frame (0 unless re-based).
 template <int K, bool FIRSTW, bool STORE, bool WIDE, bool STACK>
 __device__ __forceinline__ void step5c(Lane2<K>& L, uint64_t& M, int& nb, int& bnext, Owners& ow, int bnd,
                                        int ord, int nseq, int t, const uint32_t* w, int2 bv,
-                                       const Stream& S, s2 G2, const Link& out, const Score5Args& a) {
+                                       const Stream& S, s2 G2, const Link& out, const Score5Args& a,
+                                       s2 z = (s2){0, 0}, int oa = 0, int ob = 0) {
   const bool enter = t == bnext;
   M = (M << 1) | (enter ? 1ull : 0ull);
   if (enter) {
@@ -199,18 +219,20 @@ __device__ __forceinline__ void step5c(Lane2<K>& L, uint64_t& M, int& nb, int& b
   if (M & ow.lanes) {
 #pragma unroll
     for (int q = 0; q < kMaxOwners; ++q)
-      if (ow.o[q].lane >= 0 && ((M >> ow.o[q].lane) & 1ull)) emit_owner<K>(L, ow.o[q], bnd, ord, nseq, S, a);
+      if (ow.o[q].lane >= 0 && ((M >> ow.o[q].lane) & 1ull))
+        emit_owner<K>(L, ow.o[q], bnd, ord, nseq, S, a, oa, ob);
   }
   uint32_t fin, hup;
   row_above<FIRSTW, STACK>(L.Fout, L.Hout, bv, fin, hup);
-  const bool rst = (M >> lane_id()) & 1ull;
+  // bit l of the uniform M is lane l's condition: used as the lane mask as is
+  const bool rst = __builtin_amdgcn_inverse_ballot_w64(M);
   s2 hd = as_s2(L.hdiag);
   if (rst) {
-    hd = (s2){0, 0};
+    hd = z;
 #pragma unroll
     for (int i = 0; i < K; ++i) {
-      L.Hm[i] = (s2){0, 0};
-      L.E[i] = (s2){0, 0};
+      L.Hm[i] = z;
+      L.E[i] = z;
     }
   }
   rows5<K, WIDE>(L, fin, hup, hd, w, G2);
@@ -266,13 +288,21 @@ __device__ __forceinline__ int code_at(uint2 v, int j) {
 // lanes at a boundary, which shifts by one lane per step: a block where it is
 // empty and no boundary enters runs the plain steps, any other block the steps
 // with restarts and reports, without a branch per lane.
-template <int K, bool FIRSTW, bool STORE, bool WIDE = false, bool STACK = false>
+//
+// REBASE (a stream of one column sequence; see Rebase): the frame is checked
+// at the start of every block, the row -1 boundary and the restart values are
+// taken in the current frame, `in` words are converted from the writer's
+// frame as they are used, and the frame of each block goes to rb.fr_out.
+template <int K, bool FIRSTW, bool STORE, bool WIDE = false, bool STACK = false, bool REBASE = false>
 __device__ __forceinline__ void sweep5(int kb, int ke, Owners ow, const Stream& S, const uint8_t* prof_lane,
                                        s2 G2, const Link& in, int gin, const int* prog_in,
                                        int* cons_out, const Link& out, int gout, int* prog_out,
                                        const int* cons_in, int out_span, int pub_every,
-                                       const Score5Args& a) {
+                                       const Score5Args& a, Rebase rb = Rebase{nullptr, nullptr}) {
   constexpr int D = KDW<K, WIDE>::v;
+  // the step's row -1 input comes from bv (the converted `in` word, or the
+  // boundary in the current frame) unless the sweep is a plain first strip
+  constexpr bool ZFILL = FIRSTW && !REBASE;
   const int lane = lane_id();
   const int ncol = S.ncol;
   const int T = ncol + MST_WAVE - 1;
@@ -282,6 +312,7 @@ __device__ __forceinline__ void sweep5(int kb, int ke, Owners ow, const Stream& 
   const int ord = lane < nseq ? a.order[kb + lane] : 0;
   uint64_t M = 0;
   int nb = 0, bnext = 0;  // the next boundary to enter lane 0: B_0 = 0
+  int oa = 0, ob = 0;     // frame (REBASE)
   Lane2<K> L;
 #pragma unroll
   for (int i = 0; i < K; ++i) {
@@ -301,6 +332,44 @@ __device__ __forceinline__ void sweep5(int kb, int ke, Owners ow, const Stream& 
   uint2 codes = block_codes(X, 0, ncol);
   int blk = 0;
   for (int tb = 0; tb < T; tb += U5, ++blk) {
+    s2 z = (s2){0, 0};      // the boundary value in the current frame
+    s2 d0 = z, d1 = z;      // frame conversion of the `in` words of columns tb, tb+1..
+    if (REBASE) {
+      const uint32_t rep = __builtin_amdgcn_readfirstlane(as_u(L.Hm[0]));
+      const int ra = (int)(short)(rep & 0xFFFFu), rbh = (int)(short)(rep >> 16);
+      const int da = (ra > kRebaseAt || ra < -kRebaseAt) ? ra : 0;
+      const int db = (rbh > kRebaseAt || rbh < -kRebaseAt) ? rbh : 0;
+      if (da | db) {
+        const s2 dl = as_s2(((uint32_t)da & 0xFFFFu) | ((uint32_t)db << 16));
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+          L.Hm[i] = L.Hm[i] - dl;
+          L.E[i] = L.E[i] - dl;
+        }
+        L.Fout = as_u(as_s2(L.Fout) - dl);
+        L.Hout = as_u(as_s2(L.Hout) - dl);
+        L.hdiag = as_u(as_s2(L.hdiag) - dl);
+        oa += da;
+        ob += db;
+      }
+      z = as_s2(((uint32_t)(-oa) & 0xFFFFu) | ((uint32_t)(-ob) << 16));
+      if (STORE && lane == MST_WAVE - 1) rb.fr_out[tb >> 3] = make_int2(oa, ob);
+      if (!FIRSTW) {
+        // the writer wrote column c in its block (c + 63) / 8 (c clamped into
+        // the stream): column tb in one block, tb+1 .. tb+7 in the next
+        const int2 f0 = rb.fr_in[(min(tb, ncol - 1) + MST_WAVE - 1) >> 3];
+        const int2 f1 = rb.fr_in[(min(tb + 1, ncol - 1) + MST_WAVE - 1) >> 3];
+        d0 = as_s2(((uint32_t)(f0.x - oa) & 0xFFFFu) | ((uint32_t)(f0.y - ob) << 16));
+        d1 = as_s2(((uint32_t)(f1.x - oa) & 0xFFFFu) | ((uint32_t)(f1.y - ob) << 16));
+      }
+    }
+    // the row -1 input of step j: the `in` word, converted, or the boundary
+    auto above = [&](int j) -> int2 {
+      if (!REBASE) return bv[j];
+      if (FIRSTW) return make_int2((int)as_u(z), (int)as_u(z));
+      const s2 d = j == 0 ? d0 : d1;
+      return make_int2((int)as_u(as_s2((uint32_t)bv[j].x) + d), (int)as_u(as_s2((uint32_t)bv[j].y) + d));
+    };
     // profile dwords of this block's 8 steps
     uint32_t w[U5][D];
 #pragma unroll
@@ -317,15 +386,15 @@ __device__ __forceinline__ void sweep5(int kb, int ke, Owners ow, const Stream& 
     if (M == 0 && bnext >= tb + U5) {
 #pragma unroll
       for (int j = 0; j < U5; ++j) {
-        step5<K, FIRSTW, STORE, WIDE, STACK>(L, tb + j, w[j], bv[j], G2, out);
+        step5<K, ZFILL, STORE, WIDE, STACK>(L, tb + j, w[j], above(j), G2, out);
         if (!FIRSTW) bv[j] = in.p[(in.base + min(tb + U5 + j, ncol - 1)) & in.mask];
       }
     } else {
 #pragma unroll
       for (int j = 0; j < U5; ++j) {
         if (tb + j < T)
-          step5c<K, FIRSTW, STORE, WIDE, STACK>(L, M, nb, bnext, ow, bnd, ord, nseq, tb + j, w[j], bv[j], S,
-                                                G2, out, a);
+          step5c<K, ZFILL, STORE, WIDE, STACK>(L, M, nb, bnext, ow, bnd, ord, nseq, tb + j, w[j], above(j), S,
+                                               G2, out, a, z, oa, ob);
         if (!FIRSTW) bv[j] = in.p[(in.base + min(tb + U5 + j, ncol - 1)) & in.mask];
       }
     }
@@ -343,7 +412,7 @@ __device__ __forceinline__ void sweep5(int kb, int ke, Owners ow, const Stream& 
   // loop (every other lane has passed the final boundary)
 #pragma unroll
   for (int q = 0; q < kMaxOwners; ++q)
-    if (ow.o[q].lane >= 0 && ow.o[q].k < nseq) emit_owner<K>(L, ow.o[q], bnd, ord, nseq, S, a);
+    if (ow.o[q].lane >= 0 && ow.o[q].k < nseq) emit_owner<K>(L, ow.o[q], bnd, ord, nseq, S, a, oa, ob);
 }
 
 // Profile bytes S[Y_r][x] + go + ge of rows r0 + lane*K + i for the residues

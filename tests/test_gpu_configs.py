@@ -29,8 +29,9 @@ def long_set():
 
 
 def test_long_pairs_sampled(ctx, long_set):
-    # every pair here is past the 16-bit bound: the int32 kernel, through the
-    # batched all-pairs matrix and through explicit pair lists of both forms
+    # every pair here is past the 16-bit bound: the re-based packed kernel in
+    # the batched all-pairs matrix, the int32 kernel through explicit pair
+    # lists of both forms
     seqs = long_set
     lens = np.array([len(s) for s in seqs])
     longest = list(np.argsort(lens)[-4:])

@@ -189,15 +189,17 @@ def test_batched_two_wave_large_k(ctx):
     seqs = [_rand(rng, int(L)) for L in lens]
     o, _ = O.all_pairs(seqs, S)
     st = ctx.upload(seqs)
-    saved = ctx.tune_get("k2max")
+    saved = ctx.tune_get("k2max"), ctx.tune_get("k8")
     try:
         ctx.tune_set("k2max", 8)
+        ctx.tune_set("k8", 0)  # these rows on the two-wave pipelines, not on score8
         g = st.all_pairs(M.FORM_BATCHED)
         assert (g == o).all()
         classes = {k for k, _, _, _ in st.launch_stats(M.FORM_BATCHED)}
         assert {-1207, -1208} <= classes
     finally:
-        ctx.tune_set("k2max", saved)
+        ctx.tune_set("k2max", saved[0])
+        ctx.tune_set("k8", saved[1])
         st.free()
 
 
@@ -215,8 +217,9 @@ def test_batched_multi_sweep_rows(ctx, seed, n, lo, hi, k6, chunk6, s6p2):
     lens = list(rng.integers(lo, hi + 1, n)) + [1025, 1280, 1281, 2048]
     seqs = [_rand(rng, int(L)) for L in lens]
     st = ctx.upload(seqs)
-    saved = ctx.tune_get("k6"), ctx.tune_get("chunk6"), ctx.tune_get("s6p2")
+    saved = ctx.tune_get("k6"), ctx.tune_get("chunk6"), ctx.tune_get("s6p2"), ctx.tune_get("k8")
     try:
+        ctx.tune_set("k8", 0)  # rows above 1024 on the multi-sweep kernel, not on score8
         ctx.tune_set("k6", k6)
         ctx.tune_set("chunk6", chunk6)
         ctx.tune_set("s6p2", s6p2)
@@ -235,6 +238,7 @@ def test_batched_multi_sweep_rows(ctx, seed, n, lo, hi, k6, chunk6, s6p2):
         ctx.tune_set("k6", saved[0])
         ctx.tune_set("chunk6", saved[1])
         ctx.tune_set("s6p2", saved[2])
+        ctx.tune_set("k8", saved[3])
         st.free()
 
 
@@ -275,9 +279,10 @@ def test_batched_many_wave_rows(ctx, wmax):
     seqs = [_rand(rng, int(L)) for L in lens]
     o, _ = O.all_pairs(seqs, S)
     st = ctx.upload(seqs)
-    saved = ctx.tune_get("wmax")
+    saved = ctx.tune_get("wmax"), ctx.tune_get("k8")
     try:
         ctx.tune_set("wmax", wmax)
+        ctx.tune_set("k8", 0)  # the many-wave pipelines, not score8
         g = st.all_pairs(M.FORM_BATCHED)
         bad = np.argwhere(g != o)
         assert len(bad) == 0, [(lens[i], lens[j], g[i, j], o[i, j]) for i, j in bad[:10]]
@@ -286,7 +291,8 @@ def test_batched_many_wave_rows(ctx, wmax):
         classes = {k for k, _, _, _ in st.launch_stats(M.FORM_BATCHED)}
         assert {-1504, -1000 - 100 * wmax - 4} <= classes
     finally:
-        ctx.tune_set("wmax", saved)
+        ctx.tune_set("wmax", saved[0])
+        ctx.tune_set("k8", saved[1])
         st.free()
 
 
@@ -381,8 +387,9 @@ def test_config2_subset_all_pairs_batched(ctx):
 
 
 def test_config2_full_matrix_forms_agree(ctx):
-    # the whole 4,000-sequence matrix (1.8e12 cells, every packed class, the
-    # multi-sweep kernel at both K and the int32 pairs): the batched form equals
+    # the whole 4,000-sequence matrix (1.8e12 cells, every packed class of the
+    # default routes: one-wave classes up to 832 rows, score8 above, and the
+    # int32 pairs): the batched form equals
     # the plain int32 form entry for entry, both are symmetric with the self
     # scores on the diagonal, and a random sample of entries equals the oracle
     seqs = M.synth(2, 0)
@@ -393,7 +400,9 @@ def test_config2_full_matrix_forms_agree(ctx):
     assert len(bad) == 0, [(len(seqs[i]), len(seqs[j]), gb[i, j], gp[i, j]) for i, j in bad[:8]]
     assert (gb == gb.T).all()
     kinds = {r[0] for r in st.launch_stats(M.FORM_BATCHED)}
-    assert {-2003, -2004, -1105, -1208} <= kinds and any(k > 0 for k in kinds)
+    # rows above 832 on score8 at K = 6..8, and the few pairs past the 16-bit
+    # bound on the int32 kernel (too few re-based items to fill the device)
+    assert {-4006, -4007, -4008, -1105, -1113} <= kinds and any(k > 0 for k in kinds)
     rng = np.random.default_rng(33)
     pi = rng.integers(0, len(seqs), 1500)
     pj = rng.integers(0, len(seqs), 1500)
@@ -401,6 +410,90 @@ def test_config2_full_matrix_forms_agree(ctx):
     o, _ = O.pair_list(codes, offs, pi, pj, S)
     assert (gb[pi, pj] == o).all()
     st.free()
+
+
+@pytest.mark.parametrize("seed,n,lo,hi,k8,chunk8",
+                         [(41, 9, 800, 2100, 1, 16), (42, 10, 833, 1600, 2, 4),
+                          (43, 8, 1025, 2300, 1, 1), (44, 12, 600, 1300, 2, 64)])
+def test_batched_strips_in_turn(ctx, seed, n, lo, hi, k8, chunk8):
+    # rows above 1024 (k8 1), and the two-wave pipelines' rows too (k8 2), on
+    # score8: four waves share each strip's profile and sweep their own parts
+    # of the columns through every strip in turn, the row between strips in
+    # each wave's own HBM buffer; strips of 64*K rows, K = 6..8 per row pair
+    rng = np.random.default_rng(seed)
+    lens = list(rng.integers(lo, hi + 1, n)) + [833, 1024, 1025, 1536, 1537, 2048]
+    seqs = [_rand(rng, int(L)) for L in lens]
+    st = ctx.upload(seqs)
+    saved = ctx.tune_get("k8"), ctx.tune_get("chunk8")
+    try:
+        ctx.tune_set("k8", k8)
+        ctx.tune_set("chunk8", chunk8)
+        g = st.all_pairs(M.FORM_BATCHED)
+        o, _ = O.all_pairs(seqs, S)
+        bad = np.argwhere(g != o)
+        assert len(bad) == 0, [(lens[i], lens[j], g[i, j], o[i, j]) for i, j in bad[:10]]
+        for _ in range(2):  # graph replays
+            assert (st.all_pairs(M.FORM_BATCHED) == o).all()
+        kinds = {r[0] for r in st.launch_stats(M.FORM_BATCHED)}
+        assert any(-4008 <= k <= -4004 for k in kinds)
+        assert not any(-2004 <= k <= -2003 for k in kinds)  # no multi-sweep launches
+    finally:
+        ctx.tune_set("k8", saved[0])
+        ctx.tune_set("chunk8", saved[1])
+        st.free()
+
+
+def _evolve(rng, root, subst, indel):
+    out = []
+    for ch in root:
+        r = rng.random()
+        if r < indel / 2:
+            continue
+        if r < indel:
+            out.append(_rand(rng, int(rng.integers(1, 4))))
+        out.append(_rand(rng, 1) if rng.random() < subst else ch)
+    return "".join(out)
+
+
+@pytest.mark.parametrize("kind", ["related", "tryptophan", "mixed"])
+def test_batched_rebased_past_16bit_bound(ctx, kind):
+    # pairs whose biased values outgrow 16 bits (both sequences above ~2,000
+    # residues) keep the packed kernel: the wave's values move in a frame that
+    # follows lane 0's first row. "tryptophan": runs of W, the largest
+    # substitution score, so the values climb as fast as BLOSUM62 allows
+    # (15 per diagonal step in the biased form); "related": close relatives,
+    # whose scores are far past 32,767; "mixed": the frame also falls behind
+    # rows of unrelated residues below W runs
+    rng = np.random.default_rng({"related": 51, "tryptophan": 52, "mixed": 53}[kind])
+    if kind == "related":
+        root = _rand(rng, 4200)
+        seqs = [_evolve(rng, root, 0.05, 0.01) for _ in range(7)] + [_rand(rng, 2600)]
+    elif kind == "tryptophan":
+        seqs = ["W" * int(L) for L in (2300, 2500, 3100, 3333)] + ["W" * 1500 + _rand(rng, 1200)]
+    else:
+        seqs = ["W" * 1800 + _rand(rng, 900), _rand(rng, 700) + "W" * 2000, "W" * 2900,
+                _rand(rng, 2800), "W" * 600 + "A" * 600 + "W" * 1400]
+    lens = [len(x) for x in seqs]
+    st = ctx.upload(seqs)
+    o, _ = O.all_pairs(seqs, S)
+    # the last cell's value in the biased form (score + ge*(m+n) + 2 ge) is
+    # past the 16-bit range for some pair
+    L = np.array(lens, dtype=np.int64)
+    assert (o + 2 * (L[:, None] + L[None, :]) + 4).max() > 32767
+    saved = ctx.tune_get("rebase")
+    try:
+        # 2: re-based whatever their number; 1 (default): these few items go
+        # to the int32 kernel; 0: the int32 kernel
+        for rb in (2, 1, 0):
+            ctx.tune_set("rebase", rb)
+            g = st.all_pairs(M.FORM_BATCHED)
+            bad = np.argwhere(g != o)
+            assert len(bad) == 0, [(rb, lens[i], lens[j], g[i, j], o[i, j]) for i, j in bad[:10]]
+            kinds = {r[0] for r in st.launch_stats(M.FORM_BATCHED)}
+            assert any(-5008 <= k <= -5004 for k in kinds) == (rb == 2)
+    finally:
+        ctx.tune_set("rebase", saved)
+        st.free()
 
 
 def test_batched_launch_stats_cover_all_cells(ctx):
