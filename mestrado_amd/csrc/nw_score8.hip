@@ -23,8 +23,14 @@ namespace mst {
 
 constexpr int P8 = kScore8Waves;
 
+// Three waves per SIMD (the LDS allows three workgroups per CU): the hint
+// holds the registers to 168 (K = 8 spills a few, outside the cell updates),
+// which measured faster than two waves at up to 193 registers (the
+// 4,000-protein classes 12.5 / 13.7 ms against 14.0 / 15.8 ms at K = 8 / 7;
+// 256 x 8k-20k 840 against 991 ms).
 template <int K, bool REBASE>
-__global__ void __launch_bounds__(MST_WAVE * P8) score8_kernel(Score5Args a) {
+__global__ void __launch_bounds__(MST_WAVE * P8) __attribute__((amdgpu_waves_per_eu(3)))
+score8_kernel(Score5Args a) {
   constexpr bool WIDE = true;
   constexpr int D = KDW<K, WIDE>::v;
   constexpr int STRIP = MST_LETTERS * MST_WAVE * D * 4;
