@@ -35,7 +35,9 @@ struct Tune {
   int chunk_k9 = 0;          // column sequences per item of one- and two-wave classes of 9+ rows
                              // per lane in all (0: chunk)
   int s6p2 = 0;              // 1: the multi-sweep classes as two pipelines per workgroup (score7)
-  int stack = 0;             // 1: two row pairs of K = 2, 3 rows per lane share a strip of 2K
+  int stack = 4;             // two row pairs of K = 1..stack rows per lane share a strip of 2K
+                             // (0: none; at most 4: stacking 5 and 6 measured slower,
+                             // 216.1 and 219.7 against 213.1 ms on the 4,000-protein set)
   int rebase = 1;            // pairs past the 16-bit bound: the re-based packed kernel (score8)
                              // instead of the int32 kernel when 2, or when 1 and there are
                              // four items per compute unit; 0: the int32 kernel

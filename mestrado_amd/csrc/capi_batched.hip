@@ -297,12 +297,12 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
       }
     };
     // Stacking: this row pair and the next, both of the one-wave classes of
-    // K = 2 or 3 rows per lane and of the same K, share a strip of 2K rows
+    // K = 1..stack rows per lane and of the same K, share a strip of 2K rows
     // per lane. With a symmetric matrix the stacked items take every column
     // from this pair's ya on: the few pairs both row pairs then meet (each
     // other's rows) are scored twice, to the same value; otherwise over their
     // common columns only (from the next pair's yb on), the rest as before.
-    const bool stack = ctx->tune.stack && kind == 5 && W == 1 && K >= 2 && K <= 3 &&
+    const bool stack = ctx->tune.stack && kind == 5 && W == 1 && K >= 1 && K <= ctx->tune.stack &&
                        !single &&
                        len(ya) > 0 && 2 * k + 3 < N &&
                        std::max(ctx->tune.k5min, (len(order[2 * k + 3]) + 63) / 64) == K &&
@@ -781,8 +781,8 @@ extern "C" int mst_tune_set(mst_ctx* ctx, const char* key, int value) {
     return ctx->fail(MST_E_ARG, "mst_tune_set", "chunk out of range 1..64");
   if (!strcmp(key, "k5min") && (value < 1 || value > 3))
     return ctx->fail(MST_E_ARG, "mst_tune_set", "k5min out of range 1..3");
-  if (!strcmp(key, "stack") && value != 0 && value != 1)
-    return ctx->fail(MST_E_ARG, "mst_tune_set", "stack is 0 or 1");
+  if (!strcmp(key, "stack") && (value < 0 || value > 4))
+    return ctx->fail(MST_E_ARG, "mst_tune_set", "stack out of range 0..4");
   if (!strcmp(key, "s6p2") && value != 0 && value != 1)
     return ctx->fail(MST_E_ARG, "mst_tune_set", "s6p2 is 0 or 1");
   if (!strcmp(key, "rebase") && (value < 0 || value > 2))

@@ -154,7 +154,7 @@ int score5_groups_per_cu(int K, int W) {
   return g < 1 ? 1 : g;
 }
 
-bool score5_stacked_supported(int K) { return K == 2 || K == 4 || K == 6; }
+bool score5_stacked_supported(int K) { return K == 2 || K == 4 || K == 6 || K == 8; }
 
 void launch_score5(int K, int W, const Score5Args& a, int grid, hipStream_t st, bool stacked) {
   if (stacked) {
@@ -165,6 +165,8 @@ void launch_score5(int K, int W, const Score5Args& a, int grid, hipStream_t st, 
       hipLaunchKernelGGL((score5_kernel<4, 1, true>), dim3(grid), dim3(score5_block_threads(1)), 0, st, a);
     else if (K == 6)
       hipLaunchKernelGGL((score5_kernel<6, 1, true>), dim3(grid), dim3(score5_block_threads(1)), 0, st, a);
+    else if (K == 8)
+      hipLaunchKernelGGL((score5_kernel<8, 1, true>), dim3(grid), dim3(score5_block_threads(1)), 0, st, a);
     return;
   }
 #define MST_CASE(k, w)                                                                          \

@@ -390,11 +390,15 @@ __device__ __forceinline__ void sweep5(int kb, int ke, Owners ow, const Stream& 
         if (!FIRSTW) bv[j] = in.p[(in.base + min(tb + U5 + j, ncol - 1)) & in.mask];
       }
     } else {
+      // Steps past T in the last block are harmless, so they are not tested
+      // for: the final boundary reaches lane 63 at step T, whose owners
+      // report then (in the loop) or after it; later steps only shift that
+      // bit out of M, read clamped codes and `in` words, and store nothing
+      // (the stores check the column); no lane's values are read afterwards.
 #pragma unroll
       for (int j = 0; j < U5; ++j) {
-        if (tb + j < T)
-          step5c<K, ZFILL, STORE, WIDE, STACK>(L, M, nb, bnext, ow, bnd, ord, nseq, tb + j, w[j], above(j), S,
-                                               G2, out, a, z, oa, ob);
+        step5c<K, ZFILL, STORE, WIDE, STACK>(L, M, nb, bnext, ow, bnd, ord, nseq, tb + j, w[j], above(j), S,
+                                             G2, out, a, z, oa, ob);
         if (!FIRSTW) bv[j] = in.p[(in.base + min(tb + U5 + j, ncol - 1)) & in.mask];
       }
     }

@@ -145,12 +145,13 @@ def test_batched_one_wave_small_k(ctx, k5min):
     saved = ctx.tune_get("k5min"), ctx.tune_get("stack")
     try:
         ctx.tune_set("k5min", k5min)
-        for stack in (0, 1):
+        for stack in (0, 2):
             ctx.tune_set("stack", stack)
             g = st.all_pairs(M.FORM_BATCHED)
             assert (g == o).all()
             classes = {k for k, _, _, _ in st.launch_stats(M.FORM_BATCHED)}
-            assert (-3004 if stack and k5min == 2 else -(1000 + 100 + k5min)) in classes
+            # stacked: two row pairs of k5min rows per lane on 2 * k5min
+            assert (-(3000 + 2 * k5min) if stack else -(1000 + 100 + k5min)) in classes
     finally:
         ctx.tune_set("k5min", saved[0])
         ctx.tune_set("stack", saved[1])
@@ -242,14 +243,15 @@ def test_batched_multi_sweep_rows(ctx, seed, n, lo, hi, k6, chunk6, s6p2):
         st.free()
 
 
-@pytest.mark.parametrize("stack", [0, 1])
+@pytest.mark.parametrize("stack", [0, 3, 4])
 def test_batched_stacked_row_pairs(ctx, stack):
-    # row pairs of 65..192 rows: consecutive pairs of the same class share a
+    # row pairs of 1..256 rows: consecutive pairs of the same class share a
     # strip (lanes 0..31 and 32..63), with a symmetric matrix over every
     # column from the first pair's ya on (the pairs the two row pairs meet
     # twice get the same score twice); the same matrix with stacking off
     rng = np.random.default_rng(70)
-    lens = list(rng.integers(1, 200, 90)) + [1, 2, 31, 32, 33, 63, 64, 65, 96, 97, 128, 129, 191, 192, 193, 400]
+    lens = list(rng.integers(1, 260, 120)) + [1, 2, 31, 32, 33, 63, 64, 65, 96, 97, 128, 129, 191, 192, 193,
+                                               255, 256, 257, 400]
     seqs = [_rand(rng, int(L)) for L in lens]
     o, _ = O.all_pairs(seqs, S)
     st = ctx.upload(seqs)
@@ -261,8 +263,12 @@ def test_batched_stacked_row_pairs(ctx, stack):
         assert len(bad) == 0, [(lens[i], lens[j], g[i, j], o[i, j]) for i, j in bad[:10]]
         assert (st.all_pairs(M.FORM_BATCHED) == o).all()  # graph replay
         kinds = {r[0] for r in st.launch_stats(M.FORM_BATCHED)}
-        stacked = {k for k in kinds if k <= -3000}
-        assert stacked == ({-3004, -3006} if stack else set())
+        stacked = {k for k in kinds if -3100 < k <= -3000}
+        if stack:
+            assert {-3002, -3004, -3006} <= stacked <= {-3002, -3004, -3006, -3008}
+            assert (-3008 in stacked) == (stack == 4)
+        else:
+            assert not stacked
         L = np.array(lens, dtype=np.int64)
         assert sum(r[2] for r in st.launch_stats(M.FORM_BATCHED)) == (L.sum() ** 2 + (L * L).sum()) // 2
     finally:
