@@ -297,16 +297,18 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
       }
     };
     // Stacking: this row pair and the next, both of the one-wave classes of
-    // K = 1..stack rows per lane and of the same K, share a strip of 2K rows
-    // per lane. With a symmetric matrix the stacked items take every column
+    // K = 1..stack rows per lane, share a strip: 32 lanes each, of Ks rows per
+    // lane, Ks = ceil(longest of the four rows / 32) (at least k5min), so the
+    // rows are padded to a multiple of 32 instead of 64. With a symmetric matrix the stacked items take every column
     // from this pair's ya on: the few pairs both row pairs then meet (each
     // other's rows) are scored twice, to the same value; otherwise over their
     // common columns only (from the next pair's yb on), the rest as before.
+    const int Ks = 2 * k + 3 < N ? std::max(ctx->tune.k5min, (len(order[2 * k + 3]) + 31) / 32) : 0;
     const bool stack = ctx->tune.stack && kind == 5 && W == 1 && K >= 1 && K <= ctx->tune.stack &&
                        !single &&
                        len(ya) > 0 && 2 * k + 3 < N &&
-                       std::max(ctx->tune.k5min, (len(order[2 * k + 3]) + 63) / 64) == K &&
-                       score5_stacked_supported(2 * K);
+                       (len(order[2 * k + 3]) + 63) / 64 <= ctx->tune.stack &&
+                       score5_stacked_supported(Ks);
     if (stack) {
       int from = 2 * k;
       if (!ctx->symmetric) {
@@ -350,11 +352,11 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
         for (size_t q = c0; q < c1; ++q) {
           int64_t nr;
           const int64_t rl = rows_at(elig[q], &nr);
-          cost += (int64_t)(len(order[elig[q]]) + 63) * 64 * 2 * K;
+          cost += (int64_t)(len(order[elig[q]]) + 63) * 64 * Ks;
           cells += (int64_t)len(order[elig[q]]) * rl;
           npairs += nr;
         }
-        It3 e{it, 2 * K, 1, cost, cells, npairs};
+        It3 e{it, Ks, 1, cost, cells, npairs};
         e.p2 = make_int2(2 * k + 2, 2 * k + 3);
         v5.push_back(e);
         c0 = c1;

@@ -50,7 +50,7 @@ bool score5_supported(int K, int W);
 int score5_block_threads(int W);
 int score5_groups_per_cu(int K, int W);
 void launch_score5(int K, int W, const Score5Args& a, int grid, hipStream_t st, bool stacked = false);
-// stacked one-wave launches exist for K = 2, 4, 6, 8 (two row pairs of up to 32*K rows)
+// stacked one-wave launches exist for K = 1..8 (two row pairs of up to 32*K rows)
 bool score5_stacked_supported(int K);
 
 // score6: any number of strips of 64*K rows (K = 3 or 4), four waves sweeping

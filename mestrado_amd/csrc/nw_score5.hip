@@ -154,19 +154,25 @@ int score5_groups_per_cu(int K, int W) {
   return g < 1 ? 1 : g;
 }
 
-bool score5_stacked_supported(int K) { return K == 2 || K == 4 || K == 6 || K == 8; }
+bool score5_stacked_supported(int K) { return K >= 1 && K <= 8; }
 
 void launch_score5(int K, int W, const Score5Args& a, int grid, hipStream_t st, bool stacked) {
   if (stacked) {
     if (W != 1) return;
-    if (K == 2)
-      hipLaunchKernelGGL((score5_kernel<2, 1, true>), dim3(grid), dim3(score5_block_threads(1)), 0, st, a);
-    else if (K == 4)
-      hipLaunchKernelGGL((score5_kernel<4, 1, true>), dim3(grid), dim3(score5_block_threads(1)), 0, st, a);
-    else if (K == 6)
-      hipLaunchKernelGGL((score5_kernel<6, 1, true>), dim3(grid), dim3(score5_block_threads(1)), 0, st, a);
-    else if (K == 8)
-      hipLaunchKernelGGL((score5_kernel<8, 1, true>), dim3(grid), dim3(score5_block_threads(1)), 0, st, a);
+#define MST_CASE_ST(k)                                                                                  \
+  if (K == k) {                                                                                       \
+    hipLaunchKernelGGL((score5_kernel<k, 1, true>), dim3(grid), dim3(score5_block_threads(1)), 0, st, a); \
+    return;                                                                                           \
+  }
+    MST_CASE_ST(1)
+    MST_CASE_ST(2)
+    MST_CASE_ST(3)
+    MST_CASE_ST(4)
+    MST_CASE_ST(5)
+    MST_CASE_ST(6)
+    MST_CASE_ST(7)
+    MST_CASE_ST(8)
+#undef MST_CASE_ST
     return;
   }
 #define MST_CASE(k, w)                                                                          \
