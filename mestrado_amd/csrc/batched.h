@@ -39,8 +39,8 @@ struct Tune {
                              // (0: none; at most 4: stacking 5 and 6 measured slower,
                              // 216.1 and 219.7 against 213.1 ms on the 4,000-protein set)
   int rebase = 1;            // pairs past the 16-bit bound: the re-based packed kernel (score8)
-                             // instead of the int32 kernel when 2, or when 1 and there are
-                             // four items per compute unit; 0: the int32 kernel
+                             // instead of the int32 kernel when 2, or when 1 and its items are
+                             // many or end well within the rest of the pass; 0: the int32 kernel
   int k8 = 2;                // rows that take score8 (strips in turn on one wave): 0 none
                              // besides the re-based pairs, 1 rows above 1024 (instead of the
                              // multi-sweep kernel), 2 also the two-wave pipelines' rows

@@ -406,12 +406,20 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
     return a.cost > b.cost;
   };
   std::stable_sort(v8.begin(), v8.end(), by_cost);
-  // Re-based items are long (every strip of a row pair on one wave): too few
-  // of them leave most of the device idle for their whole run, where the
-  // int32 kernel (a wave per pair) finishes sooner. rebase 1: only when there
-  // are four items per compute unit (the 4,000-protein set's ~400 long-pair
-  // items ran 17 ms re-based against 6 ms on the int32 kernel); 2: always.
-  if (ctx->tune.rebase == 1 && (int64_t)v8r.size() < 4 * (int64_t)ctx->num_cus) {
+  // Re-based items are long (every strip of a row pair on one wave) and run
+  // on a graph branch of their own beside the other classes. rebase 1: when
+  // there are four items per compute unit, or when the longest item (at
+  // ~220 ns per step, as measured with the device shared) ends well within
+  // the rest of the pass (total cells at ~8.5 T cells/s): the 4,000-protein
+  // set's long pairs ran 206.0 ms re-based against 209.4 ms on the int32
+  // kernel, which follows the graph; otherwise the int32 kernel. 2: always.
+  int64_t steps_max = 0, cells_all = 0;
+  for (const It3& e : v8r) steps_max = std::max(steps_max, e.cost / (64 * (int64_t)e.K));
+  for (const auto* v : {&v5, &v6, &v8, &v8r})
+    for (const It3& e : *v) cells_all += e.cells;
+  const bool rebase_pays = (int64_t)v8r.size() >= 4 * (int64_t)ctx->num_cus ||
+                           (double)steps_max * 220e-9 < 0.6 * (double)cells_all / 8.5e12;
+  if (ctx->tune.rebase == 1 && !rebase_pays) {
     for (const It3& e : v8r) {
       const int ya_ = order[e.it.ya], yb_ = e.it.yb >= 0 ? order[e.it.yb] : -1;
       for (int t = e.it.xbegin; t < e.it.xbegin + e.it.xcount; ++t) {
@@ -590,16 +598,23 @@ int enqueue_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P, int32
   // its workgroups are left, overlaps the next class's start. Classes go to
   // the branch with the fewest cells so far, largest first.
   const int nb = timed ? 1 : std::max(1, std::min(4, ctx->tune.branches));
+  // Re-based classes are few long items on part of the device: they take a
+  // branch of their own, started with the others, so the other classes fill
+  // the rest of the device meanwhile (when the branches allow one more).
+  bool rb_branch = false;
+  for (const BClass& c : P.c8) rb_branch |= c.rebase;
+  rb_branch = rb_branch && !timed && nb < 4;
+  const int nbt = nb + (rb_branch ? 1 : 0);  // branch streams in all
   std::vector<hipStream_t> br(1, ctx->stream);
-  if (nb > 1) {
-    for (int b = 0; b < nb; ++b) {
+  if (nbt > 1) {
+    for (int b = 0; b < nbt; ++b) {
       if (!ctx->bstream[b]) CK(hipStreamCreateWithFlags(&ctx->bstream[b], hipStreamNonBlocking));
       if (!ctx->bevent[b]) CK(hipEventCreateWithFlags(&ctx->bevent[b], hipEventDisableTiming));
     }
     if (!ctx->bevent[4]) CK(hipEventCreateWithFlags(&ctx->bevent[4], hipEventDisableTiming));
     CK(hipEventRecord(ctx->bevent[4], ctx->stream));
     br.clear();
-    for (int b = 0; b < nb; ++b) {
+    for (int b = 0; b < nbt; ++b) {
       CK(hipStreamWaitEvent(ctx->bstream[b], ctx->bevent[4], 0));
       br.push_back(ctx->bstream[b]);
     }
@@ -614,12 +629,16 @@ int enqueue_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P, int32
   for (const BClass& c : P.c5) ls.push_back(Launch{5, &c, ctr++});
   for (const BClass& c : P.c8) ls.push_back(Launch{8, &c, ctr++});
   std::vector<int> where(ls.size(), 0);
-  if (nb > 1) {
+  if (nbt > 1) {
     std::vector<size_t> ord(ls.size());
     for (size_t k = 0; k < ls.size(); ++k) ord[k] = k;
     std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) { return ls[x].c->cells > ls[y].c->cells; });
     std::vector<int64_t> load(nb, 0);
     for (size_t k : ord) {
+      if (rb_branch && ls[k].kind == 8 && ls[k].c->rebase) {
+        where[k] = nb;
+        continue;
+      }
       const int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
       where[k] = b;
       load[b] += ls[k].c->cells;
@@ -658,8 +677,8 @@ int enqueue_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P, int32
     CK(hipGetLastError());
     if (timed) CK(hipEventRecord(ctx->launch_events[ev++], st));
   }
-  if (nb > 1) {
-    for (int b = 0; b < nb; ++b) {
+  if (nbt > 1) {
+    for (int b = 0; b < nbt; ++b) {
       CK(hipEventRecord(ctx->bevent[b], ctx->bstream[b]));
       CK(hipStreamWaitEvent(ctx->stream, ctx->bevent[b], 0));
     }

@@ -407,9 +407,10 @@ def test_config2_full_matrix_forms_agree(ctx):
     assert len(bad) == 0, [(len(seqs[i]), len(seqs[j]), gb[i, j], gp[i, j]) for i, j in bad[:8]]
     assert (gb == gb.T).all()
     kinds = {r[0] for r in st.launch_stats(M.FORM_BATCHED)}
-    # rows above 832 on score8 at K = 6..8, and the few pairs past the 16-bit
-    # bound on the int32 kernel (too few re-based items to fill the device)
-    assert {-4006, -4007, -4008, -1105, -1113} <= kinds and any(k > 0 for k in kinds)
+    # rows above 832 on score8 at K = 6..8, and the pairs past the 16-bit
+    # bound re-based (their longest item ends well within the pass)
+    assert {-4006, -4007, -4008, -1105, -1113} <= kinds and any(-5008 <= k <= -5004 for k in kinds)
+    assert not any(k > 0 for k in kinds)
     rng = np.random.default_rng(33)
     pi = rng.integers(0, len(seqs), 1500)
     pj = rng.integers(0, len(seqs), 1500)
