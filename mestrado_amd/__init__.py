@@ -151,6 +151,7 @@ class Context:
             raise MestradoError("mst_create failed (%d)" % rc)
         self.h = h
         self.tuning = {}
+        self.tuned = tuned
         if tuned:
             from . import autotune
             self.tuning = autotune.apply(self)
@@ -234,6 +235,13 @@ class Context:
         return list(joins)[: n - 1]
 
     def upload(self, seqs):
+        """Uploads a sequence set. A tuned context first applies the tuning
+        table's bucket of the set (by median length), when the table has one."""
+        if self.tuned:
+            from . import autotune
+            applied = autotune.apply_for(self, [len(s) for s in seqs])
+            if applied:
+                self.tuning = applied
         buf, off = pack(seqs)
         sid = C.c_int(0)
         self._ck(self._L.mst_seqset_upload(self.h, _p(buf), _p(off), len(seqs), C.byref(sid)),
@@ -321,9 +329,11 @@ class SeqSet:
         return ms.value
 
     def launch_stats(self, form=FORM_BATCHED):
-        """One pass with per-launch events -> [(kernel_class, pairs, cells, ms)]
-        (kernel_class: K rows per lane of the int32 kernel, -K of the shared-
-        profile 16-bit kernel, -100-K of the multi-strip 16-bit kernel)."""
+        """One pass with per-launch events -> [(kernel_class, pairs, cells, ms)].
+        kernel_class: K > 0, the int32 kernel at K rows per lane; the packed
+        kernels at K rows per lane: -(1000 + 100 W + K) the streamed kernel of
+        W waves, -(2000 + K) the multi-sweep kernel, -(3000 + K) stacked row
+        pairs, -(4000 + K) score8, -(5000 + K) score8 re-based."""
         L = self.ctx._L
         self.ctx._ck(L.mst_all_pairs_launches(self.ctx.h, self.id, form), "mst_all_pairs_launches")
         cnt = C.c_int(0)
