@@ -469,6 +469,19 @@ def main():
     # the set's plans and graph (whose parallel branches hold a stream of
     # their own) are dropped before the other workloads
     st.free()
+    tunables = dict(ctx.tuning, **tunes)
+    long_pairs = None if (args.no_long or args.batched_only) else long_pairs_bench(M, ctx)
+    served = None
+    if not (args.no_serve or args.batched_only):
+        # the served loop on a context of its own: the streams the earlier
+        # workloads left (the alignment form's second stream) otherwise share
+        # the hardware queues with its 16 (offline ~300 against ~355 jobs/s)
+        ctx.close()
+        ctx = M.Context(device=0)
+        served = served_bench(
+            M, ctx, args.serve_jobs, load=args.serve_load,
+            sets=[(kv.split("=", 1)[0], float(kv.split("=", 1)[1])) for kv in args.serve_set],
+            what={"msa": M.SERVE_MSA, "tree": M.SERVE_TREE, "scores": M.SERVE_SCORES}[args.serve_what])
 
     line = {
         "metric": "all-pairs NW GCUPS (bit-exact), batched form",
@@ -492,14 +505,11 @@ def main():
         "device_ms_plain": round(float(np.mean(devs_p)), 3) if devs_p else None,
         "roofline": roof,
         "batched_launches": launches,
-        "tunables": dict(ctx.tuning, **tunes),
+        "tunables": tunables,
         "traceback": tb,
         "nj_ms": nj_ms,
-        "long_pairs": None if (args.no_long or args.batched_only) else long_pairs_bench(M, ctx),
-        "served": None if (args.no_serve or args.batched_only) else served_bench(
-            M, ctx, args.serve_jobs, load=args.serve_load,
-            sets=[(kv.split("=", 1)[0], float(kv.split("=", 1)[1])) for kv in args.serve_set],
-            what={"msa": M.SERVE_MSA, "tree": M.SERVE_TREE, "scores": M.SERVE_SCORES}[args.serve_what]),
+        "long_pairs": long_pairs,
+        "served": served,
         "cpu_baseline": None if args.no_cpu else cpu_baseline(seqs),
     }
     print(json.dumps(line), flush=True)
