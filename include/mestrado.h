@@ -97,30 +97,36 @@ int mst_all_pairs_launches(mst_ctx* ctx, int set_id, int form);
 int mst_last_launch_stats(mst_ctx* ctx, int* rows_per_lane, int64_t* pairs, int64_t* cells,
                           float* ms, int cap, int* count);
 /* Tunables of the batched form (no counterpart in the reference; the
- * reference's window size wSize becomes such a hint): "chunk" (column
- * sequences per work item, 1..4096; default 32), "k5min" (least rows per lane of the one-wave streamed kernel, 1..3: short
- * row pairs of m rows take ceil(m/64) rows per lane when that is larger;
- * default 1), "chunk_w3" (column sequences per work item of the pipelines
- * of three and four waves, 0 = "chunk"; default 0), "k1max" (most rows per
- * lane of the one-wave streamed kernel, 5..16: row pairs of up to 64 * k1max
- * rows take one wave; default 13), "k2max" (most rows per lane of the
- * two-wave pipelines, 4..8: longer row pairs of up to 128 * k2max rows take
- * two waves; default 8), "branches" (parallel branches of the captured
- * graph's class launches, 1..4, so one class's tail overlaps the next
- * class's start; default 2), "wmax" (most waves, 4..8, of a one-sweep
- * pipeline of 4 rows per lane for rows 1025..256 * wmax; 4: every row pair
- * above 1024 rows takes the multi-sweep kernel, the default), "k6" (rows per
- * lane of the multi-sweep kernel: 3, 4, or 0 = per row pair, the one whose
- * four-strip sweeps leave fewer idle rows; default 0), "chunk6" (column
- * sequences per work item of the multi-sweep kernel, 0 = "chunk"; default
- * 0), "chunk_k9" (column sequences per work item of the one- and two-wave
- * classes of 9 or more rows per lane in all, 0 = "chunk"; default 0),
- * "s6p2" (1: the multi-sweep classes as two pipelines per workgroup sharing
- * the strip profiles, in step per sweep; default 0), "stack" (1: consecutive row pairs of the one-wave classes of 2 and 3
- * rows per lane share a strip of 4 or 6 rows per lane, one pair on each half
- * of the wave; default 0), "wg5" (workgroups per CU of the streamed
- * kernel, 0 = default). Setting a value drops the cached
- * batched plans. Unknown keys or values out of range return MST_E_ARG. */
+ * reference's window size wSize becomes such a hint). Compiled defaults in
+ * brackets; a tuned Python context applies the persisted table instead.
+ *  "chunk"    column sequences per work item, 1..64 [32]
+ *  "k5min"    least rows per lane of the one-wave kernel, 1..3 [1]
+ *  "k1max"    most rows per lane of the one-wave kernel, 5..16: rows up to
+ *             64 * k1max take one wave [13]
+ *  "stack"    consecutive row pairs of the one-wave classes of 1..stack rows
+ *             per lane share a wave, 32 lanes each, at ceil(longest row /
+ *             32) rows per lane, 0..4 [4]
+ *  "k8"       rows that take the strips-in-turn kernel (every strip of a row
+ *             pair on one wave): 0 none, 1 rows above 1024, 2 also rows
+ *             above 64 * k1max [2]
+ *  "chunk8"   column sequences per work item of that kernel, 1..64 [4]
+ *  "rebase"   pairs past the 16-bit bound: 0 the int32 kernel, 1 the
+ *             re-based packed kernel when its items are many or end well
+ *             within the pass, 2 always the re-based kernel [1]
+ *  "k2max"    most rows per lane of the two-wave pipelines (k8 0), 4..8 [8]
+ *  "wmax"     most waves, 4..8, of a one-sweep pipeline of 4 rows per lane
+ *             for rows 1025..256 * wmax (k8 0) [4]
+ *  "k6"       rows per lane of the multi-sweep kernel (k8 0): 3, 4, or 0 =
+ *             per row pair [0]
+ *  "chunk6"   column sequences per item of the multi-sweep kernel, 0 = chunk [0]
+ *  "s6p2"     1: the multi-sweep kernel as two pipelines per workgroup [0]
+ *  "chunk_w3" column sequences per item of 3- and 4-wave pipelines, 0 = chunk [0]
+ *  "chunk_k9" column sequences per item of the classes of 9 or more rows per
+ *             lane, 0 = chunk [0]
+ *  "branches" parallel branches of the captured graph's class launches, 1..4 [2]
+ *  "wg5"      workgroups per CU of the streamed kernel, 0 = as the LDS allows [0]
+ * Setting a value drops the cached batched plans. Unknown keys or values out
+ * of range return MST_E_ARG. */
 int mst_tune_set(mst_ctx* ctx, const char* key, int value);
 int mst_tune_get(mst_ctx* ctx, const char* key, int* value);
 /* One join of the guide tree (TreeNode::join, TreeNode.cpp:30-58). */
