@@ -190,7 +190,13 @@ class Context:
         return cus.value, hbm.value
 
     def tune_set(self, key, value):
-        """Set a tunable of the batched form (drops cached batched plans)."""
+        """Set a tunable of the batched form (drops cached batched plans). An
+        explicit setting ends the per-set application of the tuning table's
+        buckets at upload, so it is not overridden."""
+        self.tuned = False
+        self._tune_set(key, value)
+
+    def _tune_set(self, key, value):
         self._ck(self._L.mst_tune_set(self.h, key.encode(), int(value)), "mst_tune_set")
 
     def tune_get(self, key):
@@ -235,8 +241,9 @@ class Context:
         return list(joins)[: n - 1]
 
     def upload(self, seqs):
-        """Uploads a sequence set. A tuned context first applies the tuning
-        table's bucket of the set (by median length), when the table has one."""
+        """Uploads a sequence set. A tuned context (until tune_set is called)
+        first applies the tuning table's bucket of the set (by median
+        length), when the table has one."""
         if self.tuned:
             from . import autotune
             applied = autotune.apply_for(self, [len(s) for s in seqs])

@@ -59,7 +59,7 @@ def load(arch):
 def _set(ctx, params):
     for k in KEYS:
         if k in params:
-            ctx.tune_set(k, int(params[k]))
+            ctx._tune_set(k, int(params[k]))
 
 
 def apply(ctx, bucket=None):

@@ -203,19 +203,16 @@ __device__ __forceinline__ void step5(Lane2<K>& L, int t, const uint32_t* w, int
 // restarts from the column -1 boundary (all zero in the biased form) after
 // its owners have reported. Lanes outside the stream compute values nobody
 // reads; they restart at its first column, and the last boundary (ncol) ends
-// every lane's last sequence.
+// every lane's last sequence. enter: a boundary reaches lane 0 at this step
+// (the block's entries are found before its first step, so the steps carry
+// no boundary search of their own).
 // z: the boundary value in the sweep's frame (0 unless re-based).
 template <int K, bool FIRSTW, bool STORE, bool WIDE, bool STACK>
-__device__ __forceinline__ void step5c(Lane2<K>& L, uint64_t& M, int& nb, int& bnext, Owners& ow, int bnd,
+__device__ __forceinline__ void step5c(Lane2<K>& L, uint64_t& M, uint32_t enter, Owners& ow, int bnd,
                                        int ord, int nseq, int t, const uint32_t* w, int2 bv,
                                        const Stream& S, s2 G2, const Link& out, const Score5Args& a,
                                        s2 z = (s2){0, 0}, int oa = 0, int ob = 0) {
-  const bool enter = t == bnext;
-  M = (M << 1) | (enter ? 1ull : 0ull);
-  if (enter) {
-    ++nb;
-    bnext = boundary(bnd, nb, nseq, S.ncol);
-  }
+  M = (M << 1) | (uint64_t)enter;
   if (M & ow.lanes) {
 #pragma unroll
     for (int q = 0; q < kMaxOwners; ++q)
@@ -390,6 +387,14 @@ __device__ __forceinline__ void sweep5(int kb, int ke, Owners ow, const Stream& 
         if (!FIRSTW) bv[j] = in.p[(in.base + min(tb + U5 + j, ncol - 1)) & in.mask];
       }
     } else {
+      // the boundaries that reach lane 0 in this block (bit j: step tb + j);
+      // every earlier one was taken by an earlier block
+      uint32_t ent = 0;
+      while (bnext < tb + U5) {
+        ent |= 1u << (bnext - tb);
+        ++nb;
+        bnext = boundary(bnd, nb, nseq, S.ncol);
+      }
       // Steps past T in the last block are harmless, so they are not tested
       // for: the final boundary reaches lane 63 at step T, whose owners
       // report then (in the loop) or after it; later steps only shift that
@@ -397,8 +402,8 @@ __device__ __forceinline__ void sweep5(int kb, int ke, Owners ow, const Stream& 
       // (the stores check the column); no lane's values are read afterwards.
 #pragma unroll
       for (int j = 0; j < U5; ++j) {
-        step5c<K, ZFILL, STORE, WIDE, STACK>(L, M, nb, bnext, ow, bnd, ord, nseq, tb + j, w[j], above(j), S,
-                                             G2, out, a, z, oa, ob);
+        step5c<K, ZFILL, STORE, WIDE, STACK>(L, M, (ent >> j) & 1u, ow, bnd, ord, nseq, tb + j, w[j], above(j),
+                                             S, G2, out, a, z, oa, ob);
         if (!FIRSTW) bv[j] = in.p[(in.base + min(tb + U5 + j, ncol - 1)) & in.mask];
       }
     }
