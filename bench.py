@@ -18,6 +18,8 @@ roofline   the dominant launch's integer VALU rate: 7 algorithmic int ops per
 cpu_baseline  the oracle recurrence (oracle/liboracle.so, OpenMP over pairs)
            on a bounded random sample of the same pairs
 
+config1    configuration 1 (1,000 proteins of ~200-600): plain and batched
+           all-pairs GCUPS, NJ and progressive alignment times
 long_pairs configuration 3 (256 sequences of 8,000-20,000): all-pairs GCUPS,
            NJ and progressive alignment times
 traceback  the alignment form (flag kernels + traceback) on 20,000 random pairs
@@ -170,6 +172,38 @@ def traceback_bench(M, ctx, st, L, npairs=20000, seed=7):
                       "jobs": p, "cells": c,
                       "ms": round(m, 3)} for k, p, c, m in stats],
         "wall_s_with_host_strings": round(wall, 3),
+    }
+
+
+def config1_bench(M, ctx):
+    """Configuration 1 (1,000 proteins of ~200-600 residues, the plain GPU
+    drop-in's workload): all pairs in the plain form (one int32 launch per
+    bucket) and in the batched form (device time of a resident pass), the NJ
+    tree and the progressive alignment along it (wall time)."""
+    seqs = M.synth(1, 0)
+    L = np.array([len(s) for s in seqs], dtype=np.int64)
+    cells = int((L.sum() ** 2 - (L * L).sum()) // 2)
+    st = ctx.upload(seqs)
+    st.all_pairs_resident(M.FORM_PLAIN)  # plan
+    ms_p = min(st.all_pairs_resident(M.FORM_PLAIN) for _ in range(3))
+    st.all_pairs_resident(M.FORM_BATCHED)  # plan and graph
+    ms_b = min(st.all_pairs_resident(M.FORM_BATCHED) for _ in range(3))
+    t0 = time.perf_counter()
+    joins = st.nj()
+    t1 = time.perf_counter()
+    rows = st.msa(joins)
+    t2 = time.perf_counter()
+    st.free()
+    return {
+        "workload": "configuration 1: %d sequences, lengths %d..%d, %.3g cells (i<j)"
+                    % (len(seqs), L.min(), L.max(), cells),
+        "plain_ms": round(ms_p, 2),
+        "plain_gcups": round(cells / (ms_p * 1e-3) / 1e9, 1),
+        "batched_ms": round(ms_b, 2),
+        "batched_gcups": round(cells / (ms_b * 1e-3) / 1e9, 1),
+        "nj_ms": round((t1 - t0) * 1e3, 1),
+        "msa_s": round(t2 - t1, 3),
+        "msa_columns": len(rows[0]) if rows else 0,
     }
 
 
@@ -470,6 +504,7 @@ def main():
     # their own) are dropped before the other workloads
     st.free()
     tunables = dict(ctx.tuning, **tunes)
+    config1 = None if (args.no_long or args.batched_only) else config1_bench(M, ctx)
     long_pairs = None if (args.no_long or args.batched_only) else long_pairs_bench(M, ctx)
     served = None
     if not (args.no_serve or args.batched_only):
@@ -508,6 +543,7 @@ def main():
         "tunables": tunables,
         "traceback": tb,
         "nj_ms": nj_ms,
+        "config1": config1,
         "long_pairs": long_pairs,
         "served": served,
         "cpu_baseline": None if args.no_cpu else cpu_baseline(seqs),
