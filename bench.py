@@ -247,101 +247,21 @@ def long_pairs_bench(M, ctx):
 
 
 def served_bench(M, ctx, njobs=2000, load=0.75, seed=0, sets=(), what=None, unloaded_every=4):
-    """Served form on the job workload: njobs jobs, sizes power-law (exponent
-    1.5) on 8..512 sequences, each a tree-evolved family of lengths 100..2000;
-    a job is the whole run (all-pairs scores, NJ tree, progressive alignment)
-    unless `what` says otherwise. Offline: every job submitted at once,
-    drained. Unloaded: one job at a time, every `unloaded_every`-th job (a
-    sample over all sizes, to bound the run time). Loaded: Poisson arrivals
-    at `load` x the offline job rate; latency runs from a job's scheduled
-    arrival to the end of its last stage."""
-    if what is None:
-        what = M.SERVE_MSA
+    """Served form on the job workload (mestrado_amd.served.measure): njobs
+    jobs, sizes power-law (exponent 1.5) on 8..512 sequences, each a
+    tree-evolved family of lengths 100..2000; a job is the whole run
+    (all-pairs scores, NJ tree, progressive alignment) unless `what` says
+    otherwise; offline, unloaded (every `unloaded_every`-th job alone) and
+    loaded (Poisson arrivals at `load` x the offline job rate)."""
+    from mestrado_amd import served
 
     def note(msg):
         print("served: " + msg, file=sys.stderr, flush=True)
 
-    sizes = M.job_sizes(seed, njobs)
-    jobs = [M.synth(4, seed * 1000003 + k, int(n)) for k, n in enumerate(sizes)]
+    jobs = served.job_workload(njobs, seed)
     note("%d jobs generated" % njobs)
-    jcells = []
-    for j in jobs:
-        L = np.array([len(x) for x in j], dtype=np.int64)
-        jcells.append(int((L.sum() ** 2 - (L * L).sum()) // 2))
-    total_cells = float(sum(jcells))
-    srv = M.Server(ctx, 16 << 30)
-    for k, v in sets:
-        srv.set(k, v)
-    # offline
-    t0 = time.perf_counter()
-    ids = [srv.submit(j, what) for j in jobs]
-    srv.drain()
-    t_off = time.perf_counter() - t0
-    for jid in ids:
-        srv.release(jid)
-    srv.completed(1 << 16)
-    note("offline %.2f s, %.3g cells/s estimate" % (t_off, srv.stats()["cells_per_s"]))
-    # unloaded
-    lat_u = []
-    for j in jobs[::unloaded_every]:
-        jid = srv.submit(j, what)
-        srv.drain()
-        info = srv.job(jid)
-        lat_u.append(info["t_done"] - info["t_submit"])
-        srv.release(jid)
-        if len(lat_u) % 100 == 0:
-            note("unloaded %d jobs" % len(lat_u))
-    srv.completed(1 << 16)
-    # loaded
-    rng = np.random.default_rng(seed + 1)
-    rate_off = njobs / t_off
-    arr = np.cumsum(rng.exponential(1.0 / (load * rate_off), njobs))
-    perm = rng.permutation(njobs)
-    k, done, lat, first_of, stage_s = 0, 0, [], {}, []
-    t0m = time.monotonic()
-    t_last = t0m
-    while done < njobs:
-        now = time.monotonic() - t0m
-        while k < njobs and arr[k] <= now:
-            first_of[srv.submit(jobs[perm[k]], what)] = k
-            k += 1
-        srv.step()
-        for jid in srv.completed():
-            info = srv.job(jid)
-            lat.append(info["t_done"] - (t0m + arr[first_of[jid]]))
-            tt = srv.times(jid)
-            stage_s.append((tt[2] - tt[0], (tt[3] - tt[2]) if tt[3] else 0.0,
-                            (tt[4] - (tt[3] or tt[2]))))
-            t_last = max(t_last, info["t_done"])
-            srv.release(jid)
-            done += 1
-            if done % 500 == 0:
-                note("loaded %d jobs, %.2f s" % (done, time.monotonic() - t0m))
-    t_load = t_last - t0m
-    srv.close()
-    ms = lambda v, q: round(float(np.percentile(v, q)) * 1e3, 3)
-    stages = {M.SERVE_SCORES: "all-pairs scores", M.SERVE_TREE: "all-pairs scores + NJ tree",
-              M.SERVE_MSA: "all-pairs scores + NJ tree + progressive alignment"}[what]
-    return {
-        "workload": "%d jobs, sizes power-law 1.5 on 8..512, lengths 100..2000, %.3g cells (i<j)"
-                    % (njobs, total_cells),
-        "job": stages,
-        "unloaded_sample": "every %d-th job (%d jobs)" % (unloaded_every, len(lat_u)),
-        "offline_jobs_per_s": round(rate_off, 1),
-        "offline_gcups": round(total_cells / t_off / 1e9, 1),
-        "load": load,
-        "jobs_per_s": round(njobs / t_load, 1),
-        "gcups": round(total_cells / t_load / 1e9, 1),
-        "frac_of_offline": round((njobs / t_load) / rate_off, 3),
-        "p50_ms": ms(lat, 50), "p99_ms": ms(lat, 99),
-        "unloaded_p50_ms": ms(lat_u, 50), "unloaded_p99_ms": ms(lat_u, 99),
-        "p99_over_unloaded_p99": round(float(np.percentile(lat, 99) / np.percentile(lat_u, 99)), 3),
-        "loaded_stage_p50_ms": {name: ms([x[i] for x in stage_s], 50)
-                                for i, name in enumerate(("scores", "tree", "alignment"))},
-        "loaded_stage_p99_ms": {name: ms([x[i] for x in stage_s], 99)
-                                for i, name in enumerate(("scores", "tree", "alignment"))},
-        "params": dict(sets),
-    }
+    return served.measure(ctx, jobs, what=what, load=load, seed=seed, sets=sets,
+                          unloaded_every=unloaded_every, note=note)
 
 
 def main():

@@ -363,12 +363,20 @@ class Server:
     jobs (all-pairs scores, and their NJ tree and alignment when asked) on one
     context."""
 
-    def __init__(self, ctx, budget_bytes=8 << 30):
+    def __init__(self, ctx, budget_bytes=8 << 30, tuned=True):
+        """tuned: apply the served settings of the device's tuning table
+        (mestrado_amd.autotune --serve), when it has them."""
         self.ctx = ctx
         self._L = ctx._L
         h = C.c_void_p()
         ctx._ck(self._L.mst_server_create(ctx.h, int(budget_bytes), C.byref(h)), "mst_server_create")
         self.h = h
+        self.settings = {}
+        if tuned:
+            from . import autotune
+            for k, v in autotune.served_settings(ctx.arch()):
+                self.set(k, v)
+                self.settings[k] = v
 
     def close(self):
         if self.h:

@@ -16,6 +16,11 @@ A bucket is named by the median sequence length of the set it was tuned on
 the bucket `default` names; apply_for(ctx, lengths) picks the bucket of a set.
 
     python -m mestrado_amd.autotune --config 2        # tune on the 4,000-protein set
+    python -m mestrado_amd.autotune --serve 500       # the served form's settings on 500 jobs
+
+The served search covers the server settings of SERVE_KEYS (item sizes of
+the score rounds, the multi-sweep kernel's form) and keeps the best under
+"served"; a Server created on a tuned context applies it.
 """
 import argparse
 import itertools
@@ -112,6 +117,54 @@ def tune(ctx, seqs, reps=2, log=None):
     return best, results
 
 
+# Settings of the served form (mst_server_set) the served search covers; the
+# table keeps the best under "served".
+SERVE_KEYS = ("chunk", "chunk6", "s6p2")
+
+
+def serve_candidates():
+    for chunk, chunk6, s6p2 in itertools.product((8, 12), (4, 8), (0, 1)):
+        yield {"chunk": chunk, "chunk6": chunk6, "s6p2": s6p2}
+
+
+def tune_served(ctx, jobs, log=None):
+    """Runs the served measurement (mestrado_amd.served.measure: offline,
+    alone, and Poisson arrivals at 0.75 of the offline rate) for every
+    candidate setting; the best is the lowest p99 over the unloaded p99, the
+    served form's target, ties going to the higher offline rate."""
+    from . import served
+    results = []
+    for params in serve_candidates():
+        r = served.measure(ctx, jobs, sets=list(params.items()))
+        results.append(dict(params, ratio=r["p99_over_unloaded_p99"], p99_ms=r["p99_ms"],
+                            offline_jobs_per_s=r["offline_jobs_per_s"]))
+        if log:
+            log("serve %s: ratio %.3f, p99 %.1f ms, offline %.1f jobs/s"
+                % (params, r["p99_over_unloaded_p99"], r["p99_ms"], r["offline_jobs_per_s"]))
+    best = min(results, key=lambda r: (r["ratio"], -r["offline_jobs_per_s"]))
+    return best, results
+
+
+def save_served(arch, best, workload):
+    os.makedirs(TUNING_DIR, exist_ok=True)
+    t = load(arch) or {"arch": arch_key(arch), "buckets": {}}
+    entry = {k: best[k] for k in SERVE_KEYS}
+    entry.update(ratio=best["ratio"], p99_ms=best["p99_ms"], offline_jobs_per_s=best["offline_jobs_per_s"],
+                 workload=workload)
+    t["served"] = entry
+    with open(table_path(arch), "w") as f:
+        json.dump(t, f, indent=1, sort_keys=True)
+        f.write("\n")
+    return table_path(arch)
+
+
+def served_settings(arch):
+    """The table's served settings, [(key, value)], or []."""
+    t = load(arch)
+    entry = (t or {}).get("served") or {}
+    return [(k, entry[k]) for k in SERVE_KEYS if k in entry]
+
+
 def save(arch, bucket, best, workload, default=False):
     os.makedirs(TUNING_DIR, exist_ok=True)
     t = load(arch) or {"arch": arch_key(arch), "buckets": {}}
@@ -136,12 +189,25 @@ def main():
     ap.add_argument("--no-save", action="store_true")
     ap.add_argument("--default", action="store_true", help="make this bucket the default")
     ap.add_argument("--dir", default=None, help="directory of the tables (default: the package's)")
+    ap.add_argument("--serve", type=int, default=0, metavar="NJOBS",
+                    help="search the served form's settings on NJOBS jobs of the job workload instead")
     args = ap.parse_args()
     log = lambda m: print(m, file=sys.stderr, flush=True)
     global TUNING_DIR
     if args.dir:
         TUNING_DIR = args.dir
     ctx = M.Context(tuned=False)
+    if args.serve:
+        from . import served
+        jobs = served.job_workload(args.serve, args.seed)
+        best, results = tune_served(ctx, jobs, log)
+        out = {"arch": ctx.arch(), "served": best, "results": results}
+        if not args.no_save:
+            out["saved"] = save_served(ctx.arch(), best, "served job workload, seed %d, %d jobs"
+                                       % (args.seed, args.serve))
+        print(json.dumps(out))
+        ctx.close()
+        return 0
     seqs = M.synth(args.config, args.seed)
     lengths = [len(s) for s in seqs]
     bucket = bucket_of(lengths)

@@ -33,3 +33,42 @@ def test_committed_tables_are_well_formed():
         assert t["default"] in t["buckets"]
         for entry in t["buckets"].values():
             assert set(entry) - {"ms", "workload"} <= set(T.KEYS) and "chunk" in entry
+
+
+def test_served_settings_round_trip(tmp_path, monkeypatch):
+    monkeypatch.setattr(T, "TUNING_DIR", str(tmp_path))
+    assert T.served_settings("gfx999") == []
+    best = {"chunk": 12, "chunk6": 8, "s6p2": 1, "ratio": 2.5, "p99_ms": 200.0, "offline_jobs_per_s": 300.0}
+    T.save_served("gfx999", best, "w")
+    assert dict(T.served_settings("gfx999")) == {"chunk": 12, "chunk6": 8, "s6p2": 1}
+    # a batched bucket saved afterwards keeps the served entry
+    T.save("gfx999", "<=1024", {"chunk": 16, "ms": 1.0}, "w2")
+    t = T.load("gfx999")
+    assert t["served"]["chunk6"] == 8 and t["buckets"]["<=1024"]["chunk"] == 16
+    assert list(T.serve_candidates())  # the search has candidates over SERVE_KEYS only
+    assert all(set(c) == set(T.SERVE_KEYS) for c in T.serve_candidates())
+
+
+class _FakeCtx:
+    """Records the tunables the table applies (no device)."""
+
+    def __init__(self, arch):
+        self._arch, self.set_calls = arch, []
+
+    def arch(self):
+        return self._arch
+
+    def _tune_set(self, key, value):
+        self.set_calls.append((key, value))
+
+
+def test_bucket_of_set_is_applied(tmp_path, monkeypatch):
+    monkeypatch.setattr(T, "TUNING_DIR", str(tmp_path))
+    T.save("gfx999", "<=1024", {"chunk": 64, "stack": 4, "ms": 1.0}, "w", default=True)
+    T.save("gfx999", ">1024", {"chunk": 48, "stack": 3, "ms": 2.0}, "w")
+    ctx = _FakeCtx("gfx999")
+    applied = T.apply_for(ctx, [5000, 9000, 20000])
+    assert applied["bucket"] == ">1024" and dict(ctx.set_calls) == {"chunk": 48, "stack": 3}
+    ctx.set_calls.clear()
+    assert T.apply_for(ctx, [100, 150]) == {}  # no <=256 bucket: nothing applied
+    assert T.apply(ctx)["bucket"] == "<=1024" and dict(ctx.set_calls)["chunk"] == 64
