@@ -33,8 +33,10 @@ def measure(ctx, jobs, what=None, load=0.75, seed=0, sets=(), unloaded_every=4, 
         jcells.append(int((L.sum() ** 2 - (L * L).sum()) // 2))
     total_cells = float(sum(jcells))
     srv = Server(ctx, budget_bytes)
+    settings = dict(srv.settings)  # the tuning table's served settings
     for k, v in sets:
         srv.set(k, v)
+        settings[k] = v
     # offline
     t0 = time.perf_counter()
     ids = [srv.submit(j, what) for j in jobs]
@@ -103,5 +105,5 @@ def measure(ctx, jobs, what=None, load=0.75, seed=0, sets=(), unloaded_every=4, 
                                 for i, name in enumerate(("scores", "tree", "alignment"))},
         "loaded_stage_p99_ms": {name: ms([x[i] for x in stage_s], 99)
                                 for i, name in enumerate(("scores", "tree", "alignment"))},
-        "params": dict(sets),
+        "params": settings,
     }
