@@ -63,6 +63,7 @@ def measure(ctx, jobs, what=None, load=0.75, seed=0, sets=(), unloaded_every=4, 
     arr = np.cumsum(rng.exponential(1.0 / (load * rate_off), njobs))
     perm = rng.permutation(njobs)
     k, done, lat, first_of, stage_s = 0, 0, [], {}, []
+    steps, step_s = 0, 0.0
     t0m = time.monotonic()
     t_last = t0m
     while done < njobs:
@@ -70,7 +71,10 @@ def measure(ctx, jobs, what=None, load=0.75, seed=0, sets=(), unloaded_every=4, 
         while k < njobs and arr[k] <= now:
             first_of[srv.submit(jobs[perm[k]], what)] = k
             k += 1
+        ts = time.perf_counter()
         srv.step()
+        step_s += time.perf_counter() - ts
+        steps += 1
         for jid in srv.completed():
             info = srv.job(jid)
             lat.append(info["t_done"] - (t0m + arr[first_of[jid]]))
@@ -106,4 +110,6 @@ def measure(ctx, jobs, what=None, load=0.75, seed=0, sets=(), unloaded_every=4, 
         "loaded_stage_p99_ms": {name: ms([x[i] for x in stage_s], 99)
                                 for i, name in enumerate(("scores", "tree", "alignment"))},
         "params": settings,
+        "loaded_host": {"steps": steps, "ms_per_step": round(step_s / max(steps, 1) * 1e3, 4),
+                        "loop_ms_per_step": round((t_last - t0m) / max(steps, 1) * 1e3, 4)},
     }
