@@ -157,7 +157,33 @@ __device__ __forceinline__ void rows5(Lane2<K>& L, uint32_t fin, uint32_t hup, s
     const s2 e = pmax(L.E[i], L.Hm[i]);
     f = pmax(f, hu);
     const s2 dg = hd + prow_w<WIDE>(w, i);
-    const s2 h = pmax(pmax(e, f), dg) - G2;
+    const s2 h = pmax(pmax(e, dg), f) - G2;
+    hd = L.Hm[i];
+    L.Hm[i] = h;
+    L.E[i] = e;
+    hu = h;
+  }
+  L.Fout = as_u(f);
+  L.Hout = as_u(hu);
+}
+
+// rows5 for a step at which some lanes restart from the column -1 boundary,
+// when that boundary is 0 in both halves: keep = 0 on a restarting lane, 1 in
+// each half elsewhere (km: 0 or all ones). The lane's E and the previous
+// column's H (hd of every row) then read as 0: E by a mask, hd folded into the
+// diagonal's add as hd * keep + profile (one v_pk_mad_u16 in place of the add),
+// K + 2 instructions more than rows5 where writing the boundary into the
+// lane's 2K + 1 registers costs more.
+template <int K, bool WIDE>
+__device__ __forceinline__ void rows5r(Lane2<K>& L, uint32_t fin, uint32_t hup, s2 hd, const uint32_t* w,
+                                       s2 G2, uint32_t km, s2 keep) {
+  s2 f = as_s2(fin), hu = as_s2(hup);
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    const s2 e = as_s2(as_u(pmax(L.E[i], L.Hm[i])) & km);
+    f = pmax(f, hu);
+    const s2 dg = hd * keep + prow_w<WIDE>(w, i);
+    const s2 h = pmax(pmax(e, dg), f) - G2;
     hd = L.Hm[i];
     L.Hm[i] = h;
     L.E[i] = e;
@@ -206,8 +232,8 @@ __device__ __forceinline__ void step5(Lane2<K>& L, int t, const uint32_t* w, int
 // every lane's last sequence. enter: a boundary reaches lane 0 at this step
 // (the block's entries are found before its first step, so the steps carry
 // no boundary search of their own).
-// z: the boundary value in the sweep's frame (0 unless re-based).
-template <int K, bool FIRSTW, bool STORE, bool WIDE, bool STACK>
+// z: the boundary value in the sweep's frame (0 unless re-based, REB).
+template <int K, bool FIRSTW, bool STORE, bool WIDE, bool STACK, bool REB = false>
 __device__ __forceinline__ void step5c(Lane2<K>& L, uint64_t& M, uint32_t enter, Owners& ow, int bnd,
                                        int ord, int nseq, int t, const uint32_t* w, int2 bv,
                                        const Stream& S, s2 G2, const Link& out, const Score5Args& a,
@@ -223,16 +249,22 @@ __device__ __forceinline__ void step5c(Lane2<K>& L, uint64_t& M, uint32_t enter,
   row_above<FIRSTW, STACK>(L.Fout, L.Hout, bv, fin, hup);
   // bit l of the uniform M is lane l's condition: used as the lane mask as is
   const bool rst = __builtin_amdgcn_inverse_ballot_w64(M);
-  s2 hd = as_s2(L.hdiag);
-  if (rst) {
-    hd = z;
+  if (REB) {
+    s2 hd = as_s2(L.hdiag);
+    if (rst) {
+      hd = z;
 #pragma unroll
-    for (int i = 0; i < K; ++i) {
-      L.Hm[i] = z;
-      L.E[i] = z;
+      for (int i = 0; i < K; ++i) {
+        L.Hm[i] = z;
+        L.E[i] = z;
+      }
     }
+    rows5<K, WIDE>(L, fin, hup, hd, w, G2);
+  } else {
+    const uint32_t km = rst ? 0u : 0xFFFFFFFFu;
+    const s2 keep = as_s2(rst ? 0u : 0x00010001u);
+    rows5r<K, WIDE>(L, fin, hup, as_s2(L.hdiag), w, G2, km, keep);
   }
-  rows5<K, WIDE>(L, fin, hup, hd, w, G2);
   const int c = t - (MST_WAVE - 1);
   if (STORE && lane_id() == MST_WAVE - 1 && c >= 0 && c < S.ncol)
     out.p[(out.base + c) & out.mask] = make_int2((int)L.Fout, (int)L.Hout);
@@ -402,8 +434,8 @@ __device__ __forceinline__ void sweep5(int kb, int ke, Owners ow, const Stream& 
       // (the stores check the column); no lane's values are read afterwards.
 #pragma unroll
       for (int j = 0; j < U5; ++j) {
-        step5c<K, ZFILL, STORE, WIDE, STACK>(L, M, (ent >> j) & 1u, ow, bnd, ord, nseq, tb + j, w[j], above(j),
-                                             S, G2, out, a, z, oa, ob);
+        step5c<K, ZFILL, STORE, WIDE, STACK, REBASE>(L, M, (ent >> j) & 1u, ow, bnd, ord, nseq, tb + j, w[j],
+                                                     above(j), S, G2, out, a, z, oa, ob);
         if (!FIRSTW) bv[j] = in.p[(in.base + min(tb + U5 + j, ncol - 1)) & in.mask];
       }
     }
