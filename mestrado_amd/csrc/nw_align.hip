@@ -677,13 +677,16 @@ __global__ void __launch_bounds__(MST_WAVE) align_kernel(AlignArgs a) {
 // ---- traceback: one wave per job ----
 //
 // The walk (NeedlemanWunsch.cu:409-468) is one dependent chain of cell reads,
-// so its cost is the latency of each read. A wave keeps a tile of the flags
-// in LDS: 64 columns by up to 64 rows (8 lanes' dwords of one strip) whose
-// top-right cell is the walk's position when the tile is loaded; each lane
-// loads one column (8 dwords, independent loads). The walk then reads LDS
-// (every lane reads the same word, a broadcast) until it leaves the tile,
-// typically after 64 steps, and the next tile is loaded. Every lane runs the
-// walk (uniform control flow); lane 0 writes the path.
+// so its cost is the latency of each step. A wave holds a tile of the flags
+// in registers: 64 columns by up to 64 rows (8 groups of a lane's 8 rows of
+// one strip), lane l holding column x0 + l, one VGPR per group, whose top-
+// right cell is the walk's position when the tile is loaded. Each lane loads
+// its column's 8 words (independent loads, one memory latency per tile) and,
+// for the packed kernel's bit planes, rewrites them as 8 row nibbles in
+// parallel. A step then reads its word from the current group's VGPR with one
+// v_readlane (no LDS round trip) and runs the state machine on scalars; the
+// path's ops collect one per lane of a VGPR (a select on the lane) and are
+// written 64 at a time. The walk leaves a tile typically after 64 steps.
 
 constexpr int TB_GROUPS = 8;  // 8-row groups of a tile
 
@@ -694,81 +697,115 @@ __device__ __forceinline__ int boundary_nib(int x, int y) {
   return 6;             // UP + contUp
 }
 
+// bit r of b (8 bits) at bit 4r
+__device__ __forceinline__ uint32_t spread4(uint32_t b) {
+  b = (b | (b << 12)) & 0x000F000Fu;
+  b = (b | (b << 6)) & 0x03030303u;
+  b = (b | (b << 3)) & 0x11111111u;
+  return b;
+}
+
+// A flag word of the packed kernel (bit planes: bytes s1 = diagonal not a
+// maximum, s2 = E > F, s4 = F opened, s3 = E opened; bit r = row r) as the 8
+// row nibbles (LEFT, UP, contUp, contLeft from bit 3 down): LEFT = !s1 | s2,
+// UP = !s1 | !s2, contUp = !s4, contLeft = !s3.
+__device__ __forceinline__ uint32_t planes_to_nibbles(uint32_t w) {
+  const uint32_t s1 = w & 0xFFu, s2 = (w >> 8) & 0xFFu, s4 = (w >> 16) & 0xFFu, s3 = w >> 24;
+  const uint32_t left = (~s1 | s2) & 0xFFu, up = ~(s1 & s2) & 0xFFu;
+  return (spread4(left) << 3) | (spread4(up) << 2) | (spread4(~s4 & 0xFFu) << 1) | spread4(~s3 & 0xFFu);
+}
+
 __global__ void __launch_bounds__(MST_WAVE) traceback_kernel(TraceArgs a) {
-  __shared__ uint32_t tile[TB_GROUPS][MST_WAVE];
   const int lane = lane_id();
   for (int64_t j = (int64_t)blockIdx.x + a.first; j < a.last; j += gridDim.x) {
     const AlignJob job = a.jobs[j];
     const int n = job.n, m = job.m;
     const int Tn = n + MST_WAVE - 1;
     const int kr = job.k;  // rows per lane of the job's flag layout
+    const bool planes = job.fmt != FMT_NIBBLE;
     const uint32_t* fl = a.flags + job.flag_off;
     uint8_t* out = a.ops + job.ops_off;
     // The walk's row y is kept as (g, r): group g = y / kr (a flag kernel
     // lane's rows; 64 groups a strip) and row r = y % kr in it, stepped by
-    // hand so that no step divides. Tile: columns [cx0, cx0 + 63], groups
-    // [cg0, cg1].
-    int cx0 = 1, cg0 = 1, cg1 = 0;
-    auto nib = [&](int x, int g, int r) -> int {
-      if (x < 0 || g < 0) return boundary_nib(x, g);
-      if (x < cx0 || x > cx0 + MST_WAVE - 1 || g < cg0 || g > cg1) {
-        const int gs = g & ~(MST_WAVE - 1);  // first group of the strip
-        const int s = g / MST_WAVE;
-        const int g0 = max(gs, g - (TB_GROUPS - 1));
-        const int c = x - (MST_WAVE - 1) + lane;
-        __syncthreads();  // the previous tile has been read
-#pragma unroll
-        for (int gi = 0; gi < TB_GROUPS; ++gi) {
-          const int G = g0 + gi;
-          uint32_t v = 0;
-          if (G <= g && c >= 0) {
-            const int lg = G - gs;  // lane of the flag kernel holding group G
-            v = fl[((int64_t)s * Tn + c + lg) * MST_WAVE + lg];
-          }
-          tile[gi][lane] = v;
-        }
-        __syncthreads();
-        cx0 = x - (MST_WAVE - 1);
-        cg0 = g0;
-        cg1 = g;
-      }
-      // every lane reads the same word: keep the walk in scalar registers
-      const uint32_t w = __builtin_amdgcn_readfirstlane(tile[g - cg0][x - cx0]);
-      if (job.fmt == FMT_NIBBLE) return (int)((w >> (4 * r)) & 15u);
-      // bit planes: s1 (diagonal not a maximum), s2 (E > F), s4 (F opened),
-      // s3 (E opened) of row r
-      const int s1 = (w >> r) & 1, s2 = (w >> (8 + r)) & 1;
-      const int s4 = (w >> (16 + r)) & 1, s3 = (w >> (24 + r)) & 1;
-      const int eqd = s1 ^ 1;
-      const int left = eqd | s2, up = eqd | (s2 ^ 1);
-      return (left << 3) | (up << 2) | ((s4 ^ 1) << 1) | (s3 ^ 1);
-    };
+    // hand so that no step divides.
+    //
+    // The state machine: a move is t2 = 1 (up), 2 (left) or 3 (diagonal), the
+    // cell's own direction nibble >> 2, unless the previous move forces one:
+    // an up move whose cell had contUp continues up, a left move whose cell
+    // had contLeft continues left. prev = previous t2 << 2 | its cell's two
+    // continuation bits; kForced holds the forced move (0: none) of each prev
+    // in 2 bits. The walk ends at a cell with no direction (the corner).
+    static_assert(OP_DIAG == 0 && OP_LEFT == 1 && OP_UP == 2, "op = 3 - t2");
+    constexpr uint32_t kForced = (1u << (2 * 6)) | (1u << (2 * 7)) | (2u << (2 * 9)) | (2u << (2 * 11));
     int x = n - 1, g = (m - 1) / kr, r = (m - 1) % kr;
-    int prev = 12;
-    int cur = nib(x, g, r);
+    int prev = 12;  // as after a diagonal move: nothing forced
     int k = 0;
-    while (cur & 12) {
-      // the state machine as selects (scalar, no branches): continue an
-      // up or left run when the previous move's continuation bit says so,
-      // else take the cell's direction (11 diagonal, 01 up, 10 left)
-      const int pd = prev & 12, cd = cur & 12;
-      const bool cont_up = pd == 4 && (prev & 2);
-      const bool cont_left = pd == 8 && (prev & 1);
-      const int own = cd == 4 ? 4 : (cd == 8 ? 8 : 12);
-      const int todo = cont_up ? 4 : (cont_left ? 8 : own);
-      if (lane == 0) out[k] = (uint8_t)(todo == 12 ? OP_DIAG : (todo == 8 ? OP_LEFT : OP_UP));
+    int obuf = 0;  // op of step k in lane k % 64
+    bool done = false;
+    // one step from cell nibble cur; false at a cell with no direction
+    auto step = [&](int cur) -> bool {
+      if (!(cur & 12)) return false;
+      const int forced = (int)((kForced >> (2 * prev)) & 3u);
+      const int t2 = forced ? forced : cur >> 2;
+      obuf = lane == (k & (MST_WAVE - 1)) ? 3 - t2 : obuf;
+      if ((k & (MST_WAVE - 1)) == MST_WAVE - 1) out[k - (MST_WAVE - 1) + lane] = (uint8_t)obuf;
       ++k;
-      x -= (todo >> 3) & 1;  // 8 and 12 move left
-      if (todo & 4) {        // 4 and 12 move up
-        const bool wrap = r == 0;
-        r = wrap ? kr - 1 : r - 1;
-        g -= wrap ? 1 : 0;
+      x -= t2 >> 1;  // left and diagonal
+      r -= t2 & 1;   // up and diagonal
+      const int wrap = r < 0;
+      r += wrap ? kr : 0;
+      g -= wrap;
+      prev = (t2 << 2) | (cur & 3);
+      return true;
+    };
+    // Inside the matrix, a tile at a time: columns [cx0, cx0 + 63] by groups
+    // [cg0, g at the load]; the walk only moves left and up, so it leaves a
+    // tile to the left or the top.
+    while (!done && (x | g) >= 0) {
+      const int gs = g & ~(MST_WAVE - 1);  // first group of the strip
+      const int s = g / MST_WAVE;
+      const int cg0 = max(gs, g - (TB_GROUPS - 1));
+      const int cx0 = x - (MST_WAVE - 1);
+      const int c = cx0 + lane;
+      // a word outside the tile (a column left of 0, a group past g) reads
+      // the job's first flag word instead and is never read by the walk
+      uint32_t vt[TB_GROUPS];
+#pragma unroll
+      for (int gi = 0; gi < TB_GROUPS; ++gi) {
+        const int G = cg0 + gi;
+        const int lg = G - gs;  // lane of the flag kernel holding group G
+        const bool in = G <= g && c >= 0;
+        vt[gi] = fl[in ? ((int64_t)s * Tn + c + lg) * MST_WAVE + lg : 0];
       }
-      prev = todo | (cur & 3);
-      cur = nib(x, g, r);
+      if (planes) {
+#pragma unroll
+        for (int gi = 0; gi < TB_GROUPS; ++gi) vt[gi] = planes_to_nibbles(vt[gi]);
+      }
+      int gcur = -1;
+      uint32_t vcur = 0;
+      do {
+        if (g != gcur) {
+          // the group's VGPR by selects (an indexed register array would go
+          // to scratch memory)
+          const int gi = g - cg0;
+          vcur = vt[0];
+#pragma unroll
+          for (int q = 1; q < TB_GROUPS; ++q) vcur = gi == q ? vt[q] : vcur;
+          gcur = g;
+        }
+        const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)vcur, x - cx0);
+        if (!step((int)((w >> (4 * r)) & 15u))) {
+          done = true;
+          break;
+        }
+      } while ((x | g) >= 0 && x >= cx0 && g >= cg0);
     }
+    // On row -1 or column -1 the directions are fixed (left along row -1, up
+    // along column -1) up to the corner.
+    while (!done && step(boundary_nib(x, g))) {
+    }
+    if (lane < (k & (MST_WAVE - 1))) out[(k & ~(MST_WAVE - 1)) + lane] = (uint8_t)obuf;
     if (lane == 0) a.lens[j] = k;
-    __syncthreads();  // the tile is rebuilt for the next job
   }
 }
 
