@@ -73,6 +73,8 @@ struct BatchedPlan {
   int grid8 = 0;
   std::vector<int64_t> carry8_off, frames8_off;  // per score8 class, in int2 elements
   int8_t* d_scodes = nullptr;   // codes in length order
+  uint32_t* d_soffs8 = nullptr;  // the same codes as profile offsets (code x kProfStride8)
+  int64_t ncodes = 0;
   int64_t* d_soff = nullptr;    // their offsets by sorted position
   unsigned int* d_counters = nullptr;
   int ncounters = 0;

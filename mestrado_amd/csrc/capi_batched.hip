@@ -49,6 +49,12 @@ __global__ void gather_codes(const int8_t* codes, const int64_t* off, const int3
   for (int64_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
 }
 
+// codes as profile offsets (code x kProfStride8)
+__global__ void codes_to_offsets8(const int8_t* codes, int64_t n, uint32_t* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (uint32_t)(uint8_t)codes[i] * kProfStride8;
+}
+
 }  // namespace
 
 // Largest value a packed 16-bit lane can reach for R padded rows and n columns.
@@ -532,6 +538,9 @@ int upload_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P) {
     CK(hipMalloc(&P.d_soff, sizeof(int64_t) * (N + 1)));
     CK(hipMemcpy(P.d_soff, soff.data(), sizeof(int64_t) * (N + 1), hipMemcpyHostToDevice));
     CK(hipMalloc(&P.d_scodes, std::max<int64_t>(soff[N], 1)));
+    CK(hipMalloc(&P.d_soffs8, sizeof(uint32_t) * (soff[N] + 2 * kOffs8Pad)));
+    CK(hipMemset(P.d_soffs8, 0, sizeof(uint32_t) * (soff[N] + 2 * kOffs8Pad)));
+    P.ncodes = soff[N];
   }
   CK(hipMalloc(&P.d_counters, sizeof(unsigned int) * P.ncounters));
   if (!P.xlist.empty())
@@ -539,6 +548,10 @@ int upload_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P) {
   if (streamed) {
     hipLaunchKernelGGL(gather_codes, dim3(s.n), dim3(256), 0, ctx->stream, s.d_codes, s.d_off,
                        P.d_xlist, P.d_soff, P.d_scodes);
+    CK(hipGetLastError());
+    if (P.ncodes > 0)
+      hipLaunchKernelGGL(codes_to_offsets8, dim3((unsigned)((P.ncodes + 255) / 256)), dim3(256), 0, ctx->stream,
+                         P.d_scodes, P.ncodes, P.d_soffs8 + kOffs8Pad);
     CK(hipGetLastError());
     CK(hipStreamSynchronize(ctx->stream));
   }
@@ -575,6 +588,7 @@ int enqueue_batched(mst_ctx* ctx, const SeqSet& s, int id, BatchedPlan& P, int32
   auto args5 = [&](const BClass& c, const Item3* items) {
     Score5Args a;
     a.scodes = P.d_scodes;
+    a.soffs8 = P.d_soffs8 ? P.d_soffs8 + kOffs8Pad : nullptr;
     a.soff = P.d_soff;
     a.order = P.d_xlist;
     a.items = items;
@@ -766,7 +780,7 @@ int run_batched(mst_ctx* ctx, int id, int32_t* d_out, bool timed) {
 void free_batched(BatchedPlan& P) {
   if (P.graph) hipGraphExecDestroy(P.graph);
   void* ps[] = {P.d_xlist, P.d_counters, P.d_items5, P.d_scodes, P.d_soff, P.d_items6,
-                P.d_carry, P.d_pair2, P.d_items8, P.d_carry8, P.d_frames8};
+                P.d_carry, P.d_pair2, P.d_items8, P.d_carry8, P.d_frames8, P.d_soffs8};
   for (void* p : ps)
     if (p) hipFree(p);
   P = BatchedPlan();

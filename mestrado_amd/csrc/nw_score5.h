@@ -42,7 +42,17 @@ struct Score5Args {
   // stacked launches: the second row pair (positions) of each item, on lanes
   // 32..63 of the strip (the first on lanes 0..31); null otherwise
   const int2* pair2 = nullptr;
+  // scodes as byte offsets into a profile of 8 dwords per lane (code x
+  // kProfStride8), same indexing, with kOffs8Pad zero entries before and
+  // after; the kernels of such profiles take a step's profile address with
+  // one add (null: from scodes)
+  const uint32_t* soffs8 = nullptr;
 };
+// residue stride of a profile of 8 dwords per lane: 64 lanes x 32 bytes
+constexpr uint32_t kProfStride8 = 64 * 8 * 4;
+// zero entries around soffs8: a block's columns lie within 64 + 8 of its
+// stream (steps 0 .. ncol + 70, lanes 0..63)
+constexpr int kOffs8Pad = 128;
 
 // score5: rows up to 1024 in W strips of 64*K rows, one wave per strip.
 // K in 3..5 for W = 1, K in 3..4 for W = 2..4.

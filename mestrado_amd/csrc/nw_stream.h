@@ -297,6 +297,25 @@ __device__ __forceinline__ int code_at(uint2 v, int j) {
   return (int)(((j < 4 ? v.x : v.y) >> (8 * (j & 3))) & 0xFFu);
 }
 
+// block_codes from the codes as profile offsets (Score5Args::soffs8): the
+// lane's 8 offsets, one 32-byte load, unclamped. A column outside the stream
+// reads a neighbouring sequence's offset or the array's zero padding
+// (kOffs8Pad entries at each end), a valid profile address either way, and
+// the lane's values at such a column are never read.
+struct Offs8 {
+  uint4 lo, hi;
+};
+__device__ __forceinline__ Offs8 block_offs8(const uint32_t* __restrict__ X8, int t0) {
+  Offs8 v;
+  __builtin_memcpy(&v, X8 + (t0 - lane_id()), sizeof v);
+  return v;
+}
+
+__device__ __forceinline__ uint32_t off_at(const Offs8& v, int j) {
+  const uint4 q = j < 4 ? v.lo : v.hi;
+  return (j & 3) == 0 ? q.x : ((j & 3) == 1 ? q.y : ((j & 3) == 2 ? q.z : q.w));
+}
+
 // One wave's sweep of a stream (column sequences kb .. ke-1, at most
 // kStreamSeqs) for one strip of the row pair; ow: the strip's owners of last
 // rows. in/out: the row above / this strip's last row. Progress counters count
@@ -322,13 +341,18 @@ __device__ __forceinline__ int code_at(uint2 v, int j) {
 // at the start of every block, the row -1 boundary and the restart values are
 // taken in the current frame, `in` words are converted from the writer's
 // frame as they are used, and the frame of each block goes to rb.fr_out.
-template <int K, bool FIRSTW, bool STORE, bool WIDE = false, bool STACK = false, bool REBASE = false>
-__device__ __forceinline__ void sweep5(int kb, int ke, Owners ow, const Stream& S, const uint8_t* prof_lane,
-                                       s2 G2, const Link& in, int gin, const int* prog_in,
-                                       int* cons_out, const Link& out, int gout, int* prog_out,
-                                       const int* cons_in, int out_span, int pub_every,
-                                       const Score5Args& a, Rebase rb = Rebase{nullptr, nullptr}) {
+//
+// OFF8 (profiles of 8 dwords per lane, Score5Args::soffs8 set): the step's
+// profile address is one add from the codes as offsets; sweep5 below picks
+// the form once per sweep, so each loop carries only its own registers.
+template <int K, bool FIRSTW, bool STORE, bool WIDE, bool STACK, bool REBASE, bool OFF8>
+__device__ __forceinline__ void sweep5_body(int kb, int ke, Owners ow, const Stream& S, const uint8_t* prof_lane,
+                                            s2 G2, const Link& in, int gin, const int* prog_in,
+                                            int* cons_out, const Link& out, int gout, int* prog_out,
+                                            const int* cons_in, int out_span, int pub_every,
+                                            const Score5Args& a, Rebase rb) {
   constexpr int D = KDW<K, WIDE>::v;
+  static_assert(!OFF8 || D == 8, "offsets of a profile of 8 dwords per lane");
   // the step's row -1 input comes from bv (the converted `in` word, or the
   // boundary in the current frame) unless the sweep is a plain first strip
   constexpr bool ZFILL = FIRSTW && !REBASE;
@@ -358,7 +382,13 @@ __device__ __forceinline__ void sweep5(int kb, int ke, Owners ow, const Stream& 
 #pragma unroll
     for (int j = 0; j < U5; ++j) bv[j] = make_int2(0, 0);
   }
-  uint2 codes = block_codes(X, 0, ncol);
+  const uint32_t* X8 = OFF8 ? a.soffs8 + S.base : nullptr;
+  uint2 codes = make_uint2(0u, 0u);
+  Offs8 offs = {make_uint4(0u, 0u, 0u, 0u), make_uint4(0u, 0u, 0u, 0u)};
+  if (OFF8)
+    offs = block_offs8(X8, 0);
+  else
+    codes = block_codes(X, 0, ncol);
   int blk = 0;
   for (int tb = 0; tb < T; tb += U5, ++blk) {
     s2 z = (s2){0, 0};      // the boundary value in the current frame
@@ -401,9 +431,15 @@ __device__ __forceinline__ void sweep5(int kb, int ke, Owners ow, const Stream& 
     };
     // profile dwords of this block's 8 steps
     uint32_t w[U5][D];
+    if (OFF8) {
 #pragma unroll
-    for (int j = 0; j < U5; ++j) read_dwords<D>(prof_lane + code_at(codes, j) * (MST_WAVE * D * 4), w[j]);
-    codes = block_codes(X, tb + U5, ncol);
+      for (int j = 0; j < U5; ++j) read_dwords<D>(prof_lane + off_at(offs, j), w[j]);
+      offs = block_offs8(X8, tb + U5);
+    } else {
+#pragma unroll
+      for (int j = 0; j < U5; ++j) read_dwords<D>(prof_lane + code_at(codes, j) * (MST_WAVE * D * 4), w[j]);
+      codes = block_codes(X, tb + U5, ncol);
+    }
     if (!FIRSTW) wait_at_least(prog_in, gin + min(tb + 2 * U5, ncol));
     if (STORE && out_span > 0) {
       // this block writes columns tb-63 .. tb-56, whose slots held the columns
@@ -454,6 +490,25 @@ __device__ __forceinline__ void sweep5(int kb, int ke, Owners ow, const Stream& 
 #pragma unroll
   for (int q = 0; q < kMaxOwners; ++q)
     if (ow.o[q].lane >= 0 && ow.o[q].k < nseq) emit_owner<K>(L, ow.o[q], bnd, ord, nseq, S, a, oa, ob);
+}
+
+template <int K, bool FIRSTW, bool STORE, bool WIDE = false, bool STACK = false, bool REBASE = false>
+__device__ __forceinline__ void sweep5(int kb, int ke, Owners ow, const Stream& S, const uint8_t* prof_lane,
+                                       s2 G2, const Link& in, int gin, const int* prog_in,
+                                       int* cons_out, const Link& out, int gout, int* prog_out,
+                                       const int* cons_in, int out_span, int pub_every,
+                                       const Score5Args& a, Rebase rb = Rebase{nullptr, nullptr}) {
+  if constexpr (KDW<K, WIDE>::v == 8) {
+    if (a.soffs8 != nullptr) {
+      sweep5_body<K, FIRSTW, STORE, WIDE, STACK, REBASE, true>(kb, ke, ow, S, prof_lane, G2, in, gin, prog_in,
+                                                                cons_out, out, gout, prog_out, cons_in,
+                                                                out_span, pub_every, a, rb);
+      return;
+    }
+  }
+  sweep5_body<K, FIRSTW, STORE, WIDE, STACK, REBASE, false>(kb, ke, ow, S, prof_lane, G2, in, gin, prog_in,
+                                                             cons_out, out, gout, prog_out, cons_in, out_span,
+                                                             pub_every, a, rb);
 }
 
 // Profile bytes S[Y_r][x] + go + ge of rows r0 + lane*K + i for the residues
