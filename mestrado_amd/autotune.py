@@ -88,13 +88,15 @@ def apply_for(ctx, lengths):
 
 def candidates():
     # k5min = 3 lost by 5-8 ms on the 4,000-protein set in a wider search, k1max 14-16 and items of 72 or
-    # more sequences by 3-70 ms, k1max 12 and chunk6 2 by 1-3 ms; the search keeps the knobs that stayed
-    # close: the item sizes, where the strips-in-turn kernel (score8) takes over (k8 0: rows above
-    # 1,024 on the multi-sweep kernel, 2: rows above 832 on score8) and how far row pairs stack
-    for chunk, chunk_k9, k8, chunk8, stack in itertools.product((48, 64), (0, 32), (0, 2), (4, 8), (3, 4)):
+    # more sequences by 3-70 ms, chunk6 2 by 1-3 ms; the search keeps the knobs that stayed close: the item
+    # sizes, where the strips-in-turn kernel (score8) takes over (k8 0: rows above 1,024 on the multi-sweep
+    # kernel, 2: rows above 64 * k1max on score8; k1max 12 or 13 rows per lane of one wave) and how far row
+    # pairs stack
+    for chunk, chunk_k9, k8, chunk8, stack, k1max in itertools.product((48, 64), (0, 32), (0, 2), (4, 8), (3, 4),
+                                                                       (12, 13)):
         if k8 == 0 and chunk8 != 4:
             continue  # chunk8 only sizes score8's items
-        yield {"chunk": chunk, "chunk6": 4, "wg5": 0, "k5min": 1, "k1max": 13, "chunk_k9": chunk_k9,
+        yield {"chunk": chunk, "chunk6": 4, "wg5": 0, "k5min": 1, "k1max": k1max, "chunk_k9": chunk_k9,
                "k8": k8, "chunk8": chunk8, "stack": stack}
 
 
