@@ -85,6 +85,7 @@ bool pk16_usable(const mst_ctx* ctx) {
 // A work item's column stream of the multi-sweep kernel stays within
 // kScore6MaxCols (the per-workgroup row buffer).
 constexpr int kOneWaveShortK = 5;
+constexpr int kLoneChunk = 8;  // item size of a stacked-range row pair that did not stack
 constexpr int64_t kScore6MaxCols = 16384;
 
 // score8's strips for m rows: ns strips of at most 512 rows, K rows per lane
@@ -159,7 +160,9 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
   std::vector<int64_t> soff(N + 1, 0);  // code offsets by sorted position
   for (int k = 0; k < N; ++k) soff[k + 1] = soff[k] + len(order[k]);
   int64_t cap8 = 0, cap8r = 0;
+  bool lone = false;  // this row pair is a stacked-range pair that did not stack
   for (int k = 0; 2 * k < N; ++k) {
+    lone = false;
     const int ya = order[2 * k];
     const bool single = (2 * k + 1 >= N);
     const int yb = single ? -1 : order[2 * k + 1];
@@ -267,6 +270,10 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
         if (kind == 8) Cw = ctx->tune.chunk8;
         // the classes of many rows per lane have few row pairs: smaller items
         if (kind == 5 && W <= 2 && W * K >= 9 && ctx->tune.chunk_k9 > 0) Cw = ctx->tune.chunk_k9;
+        // a row pair of the stacked range left on its own (its successor too
+        // long to stack with) is a class of one row pair: items of a few
+        // sequences spread it over the device instead of a few workgroups
+        if (lone) Cw = std::min(Cw, kLoneChunk);
         size_t c0 = 0;
         while (c0 < elig.size()) {
           // a chunk is a run of consecutive positions; score6 keeps its column
@@ -370,6 +377,7 @@ int build_batched_plan(mst_ctx* ctx, const SeqSet& s, BatchedPlan& P) {
       ++k;  // the next pair is done
       continue;
     }
+    lone = ctx->tune.stack && kind == 5 && W == 1 && K <= ctx->tune.stack;
     if (single) {
       add_cols(2 * k, N - 1, 2 * k, -1);
     } else if (len(ya) == 0) {

@@ -395,8 +395,8 @@ def test_config2_subset_all_pairs_batched(ctx):
 
 def test_config2_full_matrix_forms_agree(ctx):
     # the whole 4,000-sequence matrix (1.8e12 cells, every packed class of the
-    # default routes: one-wave classes up to 832 rows, score8 above, and the
-    # int32 pairs): the batched form equals
+    # default routes: one-wave classes up to 64 * k1max rows, score8 above,
+    # and the re-based pairs): the batched form equals
     # the plain int32 form entry for entry, both are symmetric with the self
     # scores on the diagonal, and a random sample of entries equals the oracle
     seqs = M.synth(2, 0)
@@ -407,9 +407,11 @@ def test_config2_full_matrix_forms_agree(ctx):
     assert len(bad) == 0, [(len(seqs[i]), len(seqs[j]), gb[i, j], gp[i, j]) for i, j in bad[:8]]
     assert (gb == gb.T).all()
     kinds = {r[0] for r in st.launch_stats(M.FORM_BATCHED)}
-    # rows above 832 on score8 at K = 6..8, and the pairs past the 16-bit
-    # bound re-based (their longest item ends well within the pass)
-    assert {-4006, -4007, -4008, -1105, -1113} <= kinds and any(-5008 <= k <= -5004 for k in kinds)
+    # one wave up to 64 * k1max rows (12 or 13 rows per lane), rows above on
+    # score8 at K = 6..8, and the pairs past the 16-bit bound re-based (their
+    # longest item ends well within the pass)
+    k1max = ctx.tune_get("k1max")
+    assert {-4006, -4007, -4008, -1105, -(1100 + k1max)} <= kinds and any(-5008 <= k <= -5004 for k in kinds)
     assert not any(k > 0 for k in kinds)
     rng = np.random.default_rng(33)
     pi = rng.integers(0, len(seqs), 1500)
