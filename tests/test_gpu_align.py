@@ -60,6 +60,19 @@ def test_strip_boundaries(ctx):
     _check(ctx, seqs, pi.ravel(), pj.ravel())
 
 
+def test_packed_rows_per_lane(ctx):
+    # the packed flag kernel lays a job's rows out at k = 1..8 rows per lane
+    # (one strip of 64k rows up to 512, two strips of 64k each up to 1,024):
+    # lengths that give every k in one and in two strips, all pairs, so the
+    # traceback steps through each layout's groups
+    rng = np.random.default_rng(14)
+    lens = [60, 100, 150, 230, 300, 350, 420, 480, 530, 650, 720, 800, 860, 1000]
+    seqs = ["".join(rng.choice(list(O.STANDARD20), L)) for L in lens]
+    idx = np.arange(len(lens))
+    pi, pj = np.meshgrid(idx, idx, indexing="ij")
+    _check(ctx, seqs, pi.ravel(), pj.ravel())
+
+
 def test_long_pairs(ctx):
     # 2,000-6,000 rows: 4 to 12 strips of 512 rows
     rng = np.random.default_rng(12)
