@@ -187,6 +187,14 @@ int mst_server_job(mst_server* srv, int64_t job, int* state, int* n, double* t_s
  * reached): times[0] submission, [1] admission, [2] scores done, [3] tree
  * done, [4] done. */
 int mst_server_job_times(mst_server* srv, int64_t job, double* times);
+/* Stage accounting of a job (no counterpart in the reference, which serves
+ * nothing): stats[0] merge batches that held its merges, [1] seconds of its
+ * alignment stage with at least one of them in flight, [2] seconds with none
+ * in flight (its next merges waiting for their children, a free merge slot,
+ * memory or the host loop), [3] score rounds that held its items, [4]
+ * seconds from submission to the first of them, [5] from submission to
+ * admission. */
+int mst_server_job_stats(mst_server* srv, int64_t job, double* stats);
 /* Copies a done job's n x n scores out (may be NULL) and forgets the job:
  * take its tree and alignment first. */
 int mst_server_result(mst_server* srv, int64_t job, int32_t* scores_nxn);
@@ -220,7 +228,16 @@ int mst_server_stats(mst_server* srv, int* waiting, int* running, int* inflight,
  * (merges of at most this many cells batch apart from larger ones, default
  * 4e6), "rounds" (score rounds kept in flight, 1..3, default 3),
  * "persistent" (1: score launches of as many workgroups as fit, each taking
- * items until none are left; 0: a workgroup per item; default 1). */
+ * items until none are left; 0: a workgroup per item; default 1),
+ * "old_share" (0..1: the share of each score round the oldest job still
+ * scoring takes first, the rest going earliest deadline first; default 0),
+ * "share_largest" (1: that share goes to the job with the most cells left
+ * instead; default 0), "k8" (1: row pairs above 64 k1max rows on the
+ * strips-in-turn kernel, items of at most 16384 columns; 0: two-wave and
+ * multi-sweep kernels; default 1), "shared_backlog" (cells: when positive,
+ * "round_cells_shared" and "score_share" apply only while the score work
+ * not yet launched is below it; default 0: whenever jobs are in their NJ or
+ * alignment stage). */
 int mst_server_set(mst_server* srv, const char* key, double value);
 /* Measurement helper (no counterpart in the reference): integer VALU rate of
  * the context's device in T ops/s, int32 (packed = 0) or packed int16

@@ -55,6 +55,7 @@ def lib():
         L.mst_server_submit_job.argtypes = [P, P, P, C.c_int, C.c_int, C.POINTER(C.c_int64)]
         L.mst_server_tree.argtypes = [P, C.c_int64, P]
         L.mst_server_job_times.argtypes = [P, C.c_int64, P]
+        L.mst_server_job_stats.argtypes = [P, C.c_int64, P]
         L.mst_server_msa.argtypes = [P, C.c_int64, P, C.c_int64, C.POINTER(C.c_int)]
         L.mst_server_step.argtypes = [P, C.POINTER(C.c_int)]
         L.mst_server_drain.argtypes = [P]
@@ -438,6 +439,16 @@ class Server:
         t = np.zeros(5, dtype=np.float64)
         self.ctx._ck(self._L.mst_server_job_times(self.h, jid, _p(t)), "mst_server_job_times")
         return t
+
+    def job_stats(self, jid):
+        """Stage accounting of a job: merge batches that held its merges,
+        seconds of the alignment stage with at least one of them in flight
+        (busy) or none (idle), score rounds that held its items, and seconds
+        from submission to the first of them and to admission."""
+        t = np.zeros(6, dtype=np.float64)
+        self.ctx._ck(self._L.mst_server_job_stats(self.h, jid, _p(t)), "mst_server_job_stats")
+        return {"batches": int(t[0]), "busy_s": float(t[1]), "idle_s": float(t[2]), "rounds": int(t[3]),
+                "first_round_s": float(t[4]), "admit_s": float(t[5])}
 
     def result(self, jid):
         n = self.job(jid)["n"]

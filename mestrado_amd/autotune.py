@@ -19,8 +19,10 @@ the bucket `default` names; apply_for(ctx, lengths) picks the bucket of a set.
     python -m mestrado_amd.autotune --serve 500       # the served form's settings on 500 jobs
 
 The served search covers the server settings of SERVE_KEYS (item sizes of
-the score rounds, the multi-sweep kernel's form) and keeps the best under
-"served"; a Server created on a tuned context applies it.
+the score rounds, the kernel of the rows above 64 k1max and the multi-sweep
+kernel's form, and the score rounds' size and grid share while jobs are in
+their NJ or alignment stage) and keeps the best under "served"; a Server
+created on a tuned context applies it.
 """
 import argparse
 import itertools
@@ -121,12 +123,17 @@ def tune(ctx, seqs, reps=2, log=None):
 
 # Settings of the served form (mst_server_set) the served search covers; the
 # table keeps the best under "served".
-SERVE_KEYS = ("chunk", "chunk6", "s6p2")
+SERVE_KEYS = ("chunk", "chunk6", "s6p2", "k8", "score_share", "round_cells_shared")
 
 
 def serve_candidates():
-    for chunk, chunk6, s6p2 in itertools.product((8, 12), (4, 8), (0, 1)):
-        yield {"chunk": chunk, "chunk6": chunk6, "s6p2": s6p2}
+    # the long rows on the multi-sweep kernel (k8 0, its one-pipeline form,
+    # the earlier table's choice), or on the strips-in-turn kernel (k8 1)
+    # with the score rounds whole or, while jobs are in their NJ or alignment
+    # stage, halved and on part of the device; item sizes of both
+    yield {"chunk": 12, "chunk6": 8, "s6p2": 0, "k8": 0, "score_share": 1.0, "round_cells_shared": 2e10}
+    for chunk, chunk6, (share, rcs) in itertools.product((8, 12), (4, 8), ((1.0, 2e10), (0.75, 1e10), (0.5, 1e10))):
+        yield {"chunk": chunk, "chunk6": chunk6, "s6p2": 0, "k8": 1, "score_share": share, "round_cells_shared": rcs}
 
 
 def tune_served(ctx, jobs, log=None):
@@ -150,7 +157,7 @@ def tune_served(ctx, jobs, log=None):
 def save_served(arch, best, workload):
     os.makedirs(TUNING_DIR, exist_ok=True)
     t = load(arch) or {"arch": arch_key(arch), "buckets": {}}
-    entry = {k: best[k] for k in SERVE_KEYS}
+    entry = {k: best[k] for k in SERVE_KEYS if k in best}
     entry.update(ratio=best["ratio"], p99_ms=best["p99_ms"], offline_jobs_per_s=best["offline_jobs_per_s"],
                  workload=workload)
     t["served"] = entry

@@ -81,11 +81,17 @@ double now_s() {
 
 // Classes of the streamed kernels: 0 = score6<4>; then score5<K, 1> for K =
 // 1..16, score5<K, 2> for K = 3..8, score5<K, 3> and score5<K, 4> for K = 3..4
-// (the routes of the batched form's planner); last, score6<3>.
-constexpr int kClasses = 1 + 16 + 6 + 2 + 2 + 1;
-constexpr int kClassScore6K3 = kClasses - 1;
+// (the routes of the batched form's planner); score6<3>; last, score8<K> for
+// K = 4..8 (strips in turn on one wave, the rows above 64 k1max).
+constexpr int kClassScore6K3 = 1 + 16 + 6 + 2 + 2;
+constexpr int kClassScore8 = kClassScore6K3 + 1;  // + K - 4
+constexpr int kClasses = kClassScore8 + (kScore8MaxK - kScore8MinK + 1);
+// score8 items hold at most this many columns, so a wave's part always fits
+// its row buffer
+constexpr int64_t kServeScore8Cols = 16384;
 int class_of(int kind, int K, int W) {
   if (kind == 6) return K == 3 ? kClassScore6K3 : 0;
+  if (kind == 8) return kClassScore8 + (K - kScore8MinK);
   if (W == 1) return K;
   if (W == 2) return 17 + (K - 3);
   return 23 + (W - 3) * 2 + (K - 3);
@@ -147,6 +153,14 @@ struct Job {
   int merges_done = 0;
   bool leaves_done = false;
   Pool msa_pool;                 // over [o_msa, o_msa + msa_cap)
+  // alignment-stage accounting: merge batches holding the job's merges, and
+  // the stage's time with at least one of them in flight (busy) or none
+  int msa_inflight = 0;
+  int64_t msa_batches = 0;
+  double msa_busy = 0, msa_idle = 0, t_msa_mark = 0;
+  // score-stage accounting: rounds holding its items, the first's launch
+  int64_t score_rounds = 0;
+  double t_first_round = 0;
 };
 
 struct RoundBuf {
@@ -190,6 +204,9 @@ struct mst_server {
   int64_t* d_job_out = nullptr;
   int* d_job_ld = nullptr;
   int2* d_carry = nullptr;
+  int2* d_carry8 = nullptr;  // score8: a row buffer per wave and class
+  int grid8 = 1;
+  int k8 = 1;                // 1: the rows above 64 k1max on score8 (0: two-wave score5, score6)
   int grid6 = 1;
   int grid7 = 1;        // workgroups of the two-pipeline multi-sweep kernel
   int64_t carry_rows = 0;  // row buffers (of 16384 columns) per multi-sweep class
@@ -248,6 +265,9 @@ struct mst_server {
   };
   MsaSlot ms[kMsaSlots];
   double small_cells = 4e6;  // merges up to this many cells batch apart from larger ones
+  double old_share = 0;      // share of a round for the oldest job still scoring
+  int share_largest = 0;     // 1: for the job with the most cells left instead
+  double shared_backlog = 0; // > 0: rounds shrink for the later stages only below this backlog (cells)
 };
 
 namespace {
@@ -308,14 +328,15 @@ int64_t job_footprint(const Job& J) { return job_layout(J).total; }
 // Streamed plan of one job whose sequences sit at positions pos0 .. pos0+n-1
 // in length order (lens ascending): the row pairs (2k, 2k+1) against every
 // later position, cut into runs of `chunk` column sequences.
-void plan_job(const mst_ctx* ctx, Job& J, const std::vector<int>& lens, int chunk, int chunk6) {
+void plan_job(const mst_ctx* ctx, Job& J, const std::vector<int>& lens, int chunk, int chunk6, bool k8) {
   const int N = J.n;
   const int p0 = (int)J.pos0;
   auto add = [&](int kind, int K, int W, int64_t R, int pa, int pb, int first, int last) {
     std::vector<int> elig;
     for (int t = first; t <= last; ++t) {
       const int64_t nx = lens[t];
-      if (kind == 0 || nx == 0 || pk16_bound(ctx, R, nx) > 32767 || (kind == 6 && nx > 16384)) {
+      if (kind == 0 || nx == 0 || pk16_bound(ctx, R, nx) > 32767 || (kind == 6 && nx > 16384) ||
+          (kind == 8 && nx > kServeScore8Cols)) {
         J.fb.push_back(make_int2(p0 + t, p0 + pa));
         if (pb >= 0) J.fb.push_back(make_int2(p0 + t, p0 + pb));
       } else {
@@ -327,8 +348,9 @@ void plan_job(const mst_ctx* ctx, Job& J, const std::vector<int>& lens, int chun
     while (c0 < elig.size()) {
       size_t c1 = c0 + 1;
       int64_t cols = lens[elig[c0]];
-      while (c1 < elig.size() && (int)(c1 - c0) < (kind == 6 ? chunk6 : chunk) && elig[c1] == elig[c1 - 1] + 1 &&
-             (kind != 6 || cols + lens[elig[c1]] <= 16384)) {
+      while (c1 < elig.size() && (int)(c1 - c0) < (kind == 6 || kind == 8 ? chunk6 : chunk) &&
+             elig[c1] == elig[c1 - 1] + 1 && (kind != 6 || cols + lens[elig[c1]] <= 16384) &&
+             (kind != 8 || cols + lens[elig[c1]] <= kServeScore8Cols)) {
         cols += lens[elig[c1]];
         ++c1;
       }
@@ -354,6 +376,11 @@ void plan_job(const mst_ctx* ctx, Job& J, const std::vector<int>& lens, int chun
     } else if (m <= 64 * ctx->tune.k1max) {  // one wave (the batched form's routes)
       K = std::max(ctx->tune.k5min, (m + 63) / 64);
       kind = 5;
+    } else if (k8) {
+      // strips of at most 512 rows in turn on one wave (score8)
+      const int ns = (m + 511) / 512;
+      K = std::max(kScore8MinK, std::min(kScore8MaxK, (m + 64 * ns - 1) / (64 * ns)));
+      kind = 8;
     } else if ((m + 127) / 128 <= ctx->tune.k2max) {  // two waves
       W = 2;
       K = (m + 127) / 128;
@@ -372,7 +399,7 @@ void plan_job(const mst_ctx* ctx, Job& J, const std::vector<int>& lens, int chun
         K = cap(3) / 0.89 < cap(4) ? 3 : 4;
       }
     }
-    const int64_t R = kind == 6 ? (int64_t)(m + 64 * K - 1) / (64 * K) * 64 * K : (int64_t)W * 64 * K;
+    const int64_t R = kind == 6 || kind == 8 ? (int64_t)(m + 64 * K - 1) / (64 * K) * 64 * K : (int64_t)W * 64 * K;
     if (single) {
       add(kind, K, W, R, pa, -1, pa, N - 1);
     } else if (lens[pa] == 0) {
@@ -474,7 +501,7 @@ int admit(mst_server* S, Job& J) {
   launch_encode((const char*)(S->d_arena + o_ascii), (const int64_t*)(S->d_arena + o_roff), N,
                 (int8_t*)(S->d_arena + o_codes), ctx->stream);
   CK(hipGetLastError());
-  plan_job(ctx, J, lens, S->chunk, S->chunk6 > 0 ? S->chunk6 : S->chunk);
+  plan_job(ctx, J, lens, S->chunk, S->chunk6 > 0 ? S->chunk6 : S->chunk, S->k8 != 0);
   J.roff.assign(roff.begin(), roff.end());
   J.where.assign(N, 0);
   for (int i = 0; i < N; ++i) J.where[order[i]] = i;
@@ -557,6 +584,13 @@ int launch_round(mst_server* S, int* launched) {
   // behind a whole round of persistent score kernels
   bool later = false;
   for (int64_t id : S->running) later |= S->jobs[id].stage == kStageNj || S->jobs[id].stage == kStageMsa;
+  if (later && S->shared_backlog > 0) {
+    // with a large backlog of score work the device's throughput counts
+    // more than the later stages' latency: whole rounds
+    double backlog = 0;
+    for (int64_t id : order) backlog += (double)S->jobs[id].cells_left;
+    if (backlog > S->shared_backlog) later = false;
+  }
   const int64_t budget = (int64_t)(later ? std::min(S->round_cells, S->round_cells_shared) : S->round_cells);
   const double share = later ? S->score_share : 1.0;
   int64_t cells = 0;
@@ -565,22 +599,39 @@ int launch_round(mst_server* S, int* launched) {
   std::vector<int> fb_job;
   Round R;
   R.buf = bi;
-  for (int64_t id : order) {
-    if (cells >= budget) break;
+  // items of job id until the round holds `limit` cells
+  auto take = [&](int64_t id, int64_t limit) {
     Job& J = S->jobs[id];
-    while (J.next_item < J.items.size() && cells < budget) {
+    while (J.next_item < J.items.size() && cells < limit) {
       const size_t q = J.next_item++;
       by_cls[J.item_cls[q]].push_back(std::make_pair(J.items[q], J.slot));
       cells += J.item_cells[q];
       J.cells_left -= J.item_cells[q];
     }
-    while (J.next_item == J.items.size() && J.next_fb < J.fb.size() && cells < budget) {
+    while (J.next_item == J.items.size() && J.next_fb < J.fb.size() && cells < limit) {
       const int2 p = J.fb[J.next_fb++];
       fb.push_back(p);
       fb_job.push_back(J.slot);
       cells += 1;  // the int32 kernel's share is small; it is timed with the round
     }
+  };
+  // old_share: the oldest job still scoring takes up to that share of the
+  // round first, so a large job, whose deadline lies far out, is not left
+  // the device's idle share until then; the rest goes earliest deadline first
+  int64_t oldest = -1;
+  if (S->old_share > 0) {
+    for (int64_t id : order)
+      if (oldest < 0 || (S->share_largest ? S->jobs[id].cells_left > S->jobs[oldest].cells_left
+                                          : S->jobs[id].t_submit < S->jobs[oldest].t_submit))
+        oldest = id;
+    take(oldest, (int64_t)(S->old_share * (double)budget));
+  }
+  for (int64_t id : order) {
+    if (cells >= budget && id != oldest) continue;
+    Job& J = S->jobs[id];
+    take(id, budget);
     J.refs += 1;
+    if (J.score_rounds++ == 0) J.t_first_round = now_s();
     R.jobs.push_back(id);
     if (J.next_item == J.items.size() && J.next_fb == J.fb.size()) R.finishing.push_back(id);
   }
@@ -615,10 +666,15 @@ int launch_round(mst_server* S, int* launched) {
   int nside = 0;
   for (int c = 0; c < kClasses; ++c) {
     if (ranges[c].second == ranges[c].first) continue;
-    // the multi-sweep classes keep a side stream each (0 for K = 4, 1 for
-    // K = 3): their row buffers are per class, so launches of one class, of
+    // the multi-sweep and score8 classes keep a side stream each (score6:
+    // 0 for K = 4, 1 for K = 3; score8: 0 for K = 8, 1 for 7, 2 for 6, 3 for 5
+    // and 4): their row buffers are per class, so launches of one class, of
     // this round and the rounds still in flight, must not overlap
-    const int si = c == 0 ? 0 : (c == kClassScore6K3 ? 1 : nside++ % 4);
+    const int k8c = c >= kClassScore8 ? c - kClassScore8 + kScore8MinK : 0;
+    const int si = c == 0 ? 0
+                   : c == kClassScore6K3 ? 1
+                   : k8c ? (k8c >= 6 ? 8 - k8c : 3)
+                         : nside++ % 4;
     hipStream_t st = S->side[si];
     if (!used[si]) {
       CK(hipStreamWaitEvent(st, B.ev_go, 0));
@@ -643,7 +699,12 @@ int launch_round(mst_server* S, int* launched) {
     a.carry = S->d_carry + (c == kClassScore6K3 ? S->carry_rows * 16384 : 0);
     a.carry_cap = 16384;
     const int64_t nit = ranges[c].second - ranges[c].first;
-    if (c == 0 || c == kClassScore6K3) {
+    if (k8c) {
+      a.carry = S->d_carry8 + (int64_t)(k8c - kScore8MinK) * S->grid8 * kScore8Waves * kServeScore8Cols;
+      a.carry_cap = kServeScore8Cols;
+      launch_score8(k8c, false, a,
+                    (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)(S->grid8 * share), nit)), st);
+    } else if (c == 0 || c == kClassScore6K3) {
       if (S->s6p2)
         launch_score7(c == 0 ? 4 : 3, a, (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)(S->grid7 * share), nit)), st);
       else
@@ -733,6 +794,10 @@ void start_msa(Job& J) {
   J.merge_state.assign(N > 1 ? N - 1 : 0, 0);
   J.merges_done = 0;
   J.leaves_done = false;
+  J.msa_inflight = 0;
+  J.msa_batches = 0;
+  J.msa_busy = J.msa_idle = 0;
+  J.t_msa_mark = now_s();
 }
 
 // The job's score rounds are over: on to NJ, the alignment or done.
@@ -1019,6 +1084,15 @@ int launch_merges(mst_server* S) {
                         ctx->flag16 ? ctx->smax : -1));
     CK(hipEventRecord(slot.ev, slot.st));
     slot.busy = true;
+    const double t = now_s();
+    for (int64_t id : slot.jobs) {
+      Job& J = S->jobs[id];
+      if (J.msa_inflight++ == 0) {
+        J.msa_idle += t - J.t_msa_mark;
+        J.t_msa_mark = t;
+      }
+      J.msa_batches += 1;
+    }
   }
   return MST_OK;
 }
@@ -1069,6 +1143,10 @@ int retire_merges(mst_server* S, bool wait, int* ndone) {
     for (int64_t id : slot.jobs) {
       Job& J = S->jobs[id];
       J.refs -= 1;
+      if (--J.msa_inflight == 0) {
+        J.msa_busy += t - J.t_msa_mark;
+        J.t_msa_mark = t;
+      }
       if (!J.cancel && J.state == MST_JOB_RUNNING && J.merges_done == J.n - 1) {
         // done once every batch that wrote its groups has retired
         bool pending = false;
@@ -1107,6 +1185,10 @@ extern "C" int mst_server_create(mst_ctx* ctx, int64_t budget_bytes, mst_server*
   if ((e = hipMalloc(&S->d_order, sizeof(int32_t) * max_pos)) != hipSuccess) return fail("order", e);
   if ((e = hipMalloc(&S->d_job_out, sizeof(int64_t) * max_slots)) != hipSuccess) return fail("jobs", e);
   if ((e = hipMalloc(&S->d_job_ld, sizeof(int) * max_slots)) != hipSuccess) return fail("jobs", e);
+  S->grid8 = ctx->num_cus * score8_groups_per_cu();
+  if ((e = hipMalloc(&S->d_carry8, sizeof(int2) * (size_t)(kScore8MaxK - kScore8MinK + 1) * S->grid8 *
+                                       kScore8Waves * kServeScore8Cols)) != hipSuccess)
+    return fail("carry8", e);
   S->grid6 = ctx->num_cus * score6_groups_per_cu();
   S->grid7 = ctx->num_cus * score7_groups_per_cu();
   // a row buffer per workgroup (two per workgroup of the two-pipeline form)
@@ -1165,7 +1247,8 @@ extern "C" int mst_server_destroy(mst_server* S) {
     for (void* p : hs)
       if (p) hipHostFree(p);
   }
-  void* ps[] = {S->d_arena, S->d_soff, S->d_order, S->d_job_out, S->d_job_ld, S->d_carry, S->d_bufs};
+  void* ps[] = {S->d_arena, S->d_soff, S->d_order, S->d_job_out, S->d_job_ld, S->d_carry, S->d_bufs,
+                S->d_carry8};
   for (void* p : ps)
     if (p) hipFree(p);
   for (hipStream_t st : S->side)
@@ -1301,6 +1384,20 @@ extern "C" int mst_server_job_times(mst_server* S, int64_t job, double* times) {
   return MST_OK;
 }
 
+extern "C" int mst_server_job_stats(mst_server* S, int64_t job, double* stats) {
+  if (!S || !stats) return MST_E_ARG;
+  auto it = S->jobs.find(job);
+  if (it == S->jobs.end()) return S->ctx->fail(MST_E_ARG, "mst_server_job_stats", "unknown job");
+  const Job& J = it->second;
+  stats[0] = (double)J.msa_batches;
+  stats[1] = J.msa_busy;
+  stats[2] = J.msa_idle;
+  stats[3] = (double)J.score_rounds;
+  stats[4] = J.t_first_round > 0 ? J.t_first_round - J.t_submit : 0;
+  stats[5] = J.t_admit > 0 ? J.t_admit - J.t_submit : 0;
+  return MST_OK;
+}
+
 extern "C" int mst_server_result(mst_server* S, int64_t job, int32_t* scores_nxn) {
   if (!S) return MST_E_ARG;
   mst_ctx* ctx = S->ctx;
@@ -1403,6 +1500,10 @@ extern "C" int mst_server_set(mst_server* S, const char* key, double value) {
     S->round_cells = value;
     return MST_OK;
   }
+  if (!strcmp(key, "k8") && (value == 0 || value == 1)) {
+    S->k8 = (int)value;
+    return MST_OK;
+  }
   if (!strcmp(key, "s6p2") && (value == 0 || value == 1)) {
     S->s6p2 = (int)value;
     return MST_OK;
@@ -1437,6 +1538,18 @@ extern "C" int mst_server_set(mst_server* S, const char* key, double value) {
   }
   if (!strcmp(key, "rounds") && value >= 1 && value <= mst_server::kBufs) {
     S->max_rounds = (int)value;
+    return MST_OK;
+  }
+  if (!strcmp(key, "old_share") && value >= 0 && value <= 1) {
+    S->old_share = value;
+    return MST_OK;
+  }
+  if (!strcmp(key, "shared_backlog") && value >= 0) {
+    S->shared_backlog = value;
+    return MST_OK;
+  }
+  if (!strcmp(key, "share_largest") && (value == 0 || value == 1)) {
+    S->share_largest = (int)value;
     return MST_OK;
   }
   if (!strcmp(key, "small_cells") && value >= 0) {

@@ -47,12 +47,15 @@ def measure(ctx, jobs, what=None, load=0.75, seed=0, sets=(), unloaded_every=4, 
     srv.completed(1 << 16)
     note("offline %.2f s, %.3g cells/s estimate" % (t_off, srv.stats()["cells_per_s"]))
     # unloaded
-    lat_u = []
+    lat_u, info_u = [], []
     for j in jobs[::unloaded_every]:
         jid = srv.submit(j, what)
         srv.drain()
         info = srv.job(jid)
         lat_u.append(info["t_done"] - info["t_submit"])
+        tt = srv.times(jid)
+        info_u.append((lat_u[-1], info["n"], (tt[2] - tt[0], (tt[3] - tt[2]) if tt[3] else 0.0,
+                                               (tt[4] - (tt[3] or tt[2]))), srv.job_stats(jid)))
         srv.release(jid)
         if len(lat_u) % 100 == 0:
             note("unloaded %d jobs" % len(lat_u))
@@ -62,7 +65,7 @@ def measure(ctx, jobs, what=None, load=0.75, seed=0, sets=(), unloaded_every=4, 
     rate_off = njobs / t_off
     arr = np.cumsum(rng.exponential(1.0 / (load * rate_off), njobs))
     perm = rng.permutation(njobs)
-    k, done, lat, first_of, stage_s = 0, 0, [], {}, []
+    k, done, lat, first_of, stage_s, job_info = 0, 0, [], {}, [], []
     steps, step_s = 0, 0.0
     t0m = time.monotonic()
     t_last = t0m
@@ -81,6 +84,8 @@ def measure(ctx, jobs, what=None, load=0.75, seed=0, sets=(), unloaded_every=4, 
             tt = srv.times(jid)
             stage_s.append((tt[2] - tt[0], (tt[3] - tt[2]) if tt[3] else 0.0,
                             (tt[4] - (tt[3] or tt[2]))))
+            js = srv.job_stats(jid)
+            job_info.append((lat[-1], info["n"], stage_s[-1], js))
             t_last = max(t_last, info["t_done"])
             srv.release(jid)
             done += 1
@@ -88,6 +93,23 @@ def measure(ctx, jobs, what=None, load=0.75, seed=0, sets=(), unloaded_every=4, 
                 note("loaded %d jobs, %.2f s" % (done, time.monotonic() - t0m))
     t_load = t_last - t0m
     srv.close()
+    # the slowest 1% of the jobs (those the p99 is about): their size and
+    # where their time went, loaded and alone
+    mean = lambda v: round(float(np.mean(v)), 3)
+
+    def slowest(rows):
+        slow = sorted(rows, key=lambda r: -r[0])[:max(1, len(rows) // 100)]
+        return {
+            "jobs": len(slow), "latency_ms": mean([r[0] * 1e3 for r in slow]), "n": mean([r[1] for r in slow]),
+            "scores_ms": mean([r[2][0] * 1e3 for r in slow]), "tree_ms": mean([r[2][1] * 1e3 for r in slow]),
+            "alignment_ms": mean([r[2][2] * 1e3 for r in slow]),
+            "merge_batches": mean([r[3]["batches"] for r in slow]),
+            "alignment_busy_ms": mean([r[3]["busy_s"] * 1e3 for r in slow]),
+            "alignment_idle_ms": mean([r[3]["idle_s"] * 1e3 for r in slow]),
+            "score_rounds": mean([r[3]["rounds"] for r in slow]),
+            "first_round_ms": mean([r[3]["first_round_s"] * 1e3 for r in slow]),
+            "admit_ms": mean([r[3]["admit_s"] * 1e3 for r in slow]),
+        }
     ms = lambda v, q: round(float(np.percentile(v, q)) * 1e3, 3)
     stages = {SERVE_SCORES: "all-pairs scores", SERVE_TREE: "all-pairs scores + NJ tree",
               SERVE_MSA: "all-pairs scores + NJ tree + progressive alignment"}[what]
@@ -109,6 +131,8 @@ def measure(ctx, jobs, what=None, load=0.75, seed=0, sets=(), unloaded_every=4, 
                                 for i, name in enumerate(("scores", "tree", "alignment"))},
         "loaded_stage_p99_ms": {name: ms([x[i] for x in stage_s], 99)
                                 for i, name in enumerate(("scores", "tree", "alignment"))},
+        "loaded_slowest_1pct": slowest(job_info),
+        "unloaded_slowest_1pct": slowest(info_u),
         "params": settings,
         "loaded_host": {"steps": steps, "ms_per_step": round(step_s / max(steps, 1) * 1e3, 4),
                         "loop_ms_per_step": round((t_last - t0m) / max(steps, 1) * 1e3, 4)},

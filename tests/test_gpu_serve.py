@@ -48,15 +48,18 @@ def test_jobs_alone_and_in_mix_match_oracle(ctx):
         assert st["waiting"] == 0 and st["running"] == 0 and st["arena_used"] == 0
 
 
-@pytest.mark.parametrize("s6p2", [1, 0])
-def test_long_rows_multi_sweep_forms(ctx, s6p2):
-    # rows above 1,024 (the multi-sweep classes at 3 and 4 rows per lane) with
-    # the two-pipeline kernel (the default) or the one-pipeline one, rounds
-    # small enough that several are in flight
+@pytest.mark.parametrize("k8,s6p2", [(1, 0), (0, 1), (0, 0)])
+def test_long_rows_multi_sweep_forms(ctx, k8, s6p2):
+    # rows above 64 k1max: on the strips-in-turn kernel (score8, the default;
+    # each class with its own row buffers and stream), or on the two-wave and
+    # multi-sweep classes (3 and 4 rows per lane above 1,024) with the
+    # two-pipeline or the one-pipeline kernel; rounds small enough that
+    # several are in flight
     rng = np.random.default_rng(9)
     jobs = [_job(rng, n, lo, hi) for n, lo, hi in [(6, 1030, 1500), (5, 1540, 2300), (7, 900, 2000)]]
     want = [O.all_pairs(j, S)[0] for j in jobs]
     with M.Server(ctx, 1 << 30) as srv:
+        srv.set("k8", k8)
         srv.set("s6p2", s6p2)
         srv.set("round_cells", 5e6)
         ids = [srv.submit(j) for j in jobs]
