@@ -23,10 +23,14 @@ def _jobs(seed):
     return [_job(rng, n, lo, hi) for n, lo, hi in spec]
 
 
-def test_jobs_alone_and_in_mix_match_oracle(ctx):
+@pytest.mark.parametrize("k8", [0, 1])
+def test_jobs_alone_and_in_mix_match_oracle(ctx, k8):
+    # both routes of the rows above 64 k1max: two-wave and multi-sweep
+    # classes, or the strips-in-turn kernel
     jobs = _jobs(1)
     want = [O.all_pairs(j, S)[0] for j in jobs]
     with M.Server(ctx, 1 << 30) as srv:
+        srv.set("k8", k8)
         for j, w in zip(jobs, want):  # one at a time
             jid = srv.submit(j)
             srv.drain()
