@@ -228,11 +228,8 @@ int mst_server_stats(mst_server* srv, int* waiting, int* running, int* inflight,
  * (merges of at most this many cells batch apart from larger ones, default
  * 4e6), "rounds" (score rounds kept in flight, 1..3, default 3),
  * "persistent" (1: score launches of as many workgroups as fit, each taking
- * items until none are left; 0: a workgroup per item; default 1),
- * "old_share" (0..1: the share of each score round the oldest job still
- * scoring takes first, the rest going earliest deadline first; default 0),
- * "share_largest" (1: that share goes to the job with the most cells left
- * instead; default 0), "k8" (1: row pairs above 64 k1max rows on the
+ * items until none are left; 0: a workgroup per item; default 1), "k8" (1:
+ * row pairs above 64 k1max rows on the
  * strips-in-turn kernel, items of at most 16384 columns; 0: two-wave and
  * multi-sweep kernels; default 1), "shared_backlog" (cells: when positive,
  * "round_cells_shared" and "score_share" apply only while the score work

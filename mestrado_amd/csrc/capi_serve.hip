@@ -265,8 +265,6 @@ struct mst_server {
   };
   MsaSlot ms[kMsaSlots];
   double small_cells = 4e6;  // merges up to this many cells batch apart from larger ones
-  double old_share = 0;      // share of a round for the oldest job still scoring
-  int share_largest = 0;     // 1: for the job with the most cells left instead
   double shared_backlog = 0; // > 0: rounds shrink for the later stages only below this backlog (cells)
 };
 
@@ -615,19 +613,8 @@ int launch_round(mst_server* S, int* launched) {
       cells += 1;  // the int32 kernel's share is small; it is timed with the round
     }
   };
-  // old_share: the oldest job still scoring takes up to that share of the
-  // round first, so a large job, whose deadline lies far out, is not left
-  // the device's idle share until then; the rest goes earliest deadline first
-  int64_t oldest = -1;
-  if (S->old_share > 0) {
-    for (int64_t id : order)
-      if (oldest < 0 || (S->share_largest ? S->jobs[id].cells_left > S->jobs[oldest].cells_left
-                                          : S->jobs[id].t_submit < S->jobs[oldest].t_submit))
-        oldest = id;
-    take(oldest, (int64_t)(S->old_share * (double)budget));
-  }
   for (int64_t id : order) {
-    if (cells >= budget && id != oldest) continue;
+    if (cells >= budget) break;
     Job& J = S->jobs[id];
     take(id, budget);
     J.refs += 1;
@@ -1540,16 +1527,8 @@ extern "C" int mst_server_set(mst_server* S, const char* key, double value) {
     S->max_rounds = (int)value;
     return MST_OK;
   }
-  if (!strcmp(key, "old_share") && value >= 0 && value <= 1) {
-    S->old_share = value;
-    return MST_OK;
-  }
   if (!strcmp(key, "shared_backlog") && value >= 0) {
     S->shared_backlog = value;
-    return MST_OK;
-  }
-  if (!strcmp(key, "share_largest") && (value == 0 || value == 1)) {
-    S->share_largest = (int)value;
     return MST_OK;
   }
   if (!strcmp(key, "small_cells") && value >= 0) {
