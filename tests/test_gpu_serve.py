@@ -177,6 +177,19 @@ def test_msa_jobs_alone_and_in_mix(ctx):
                 srv.step()
         srv.drain()
         for jid, j, w, what in zip(ids, jobs + jobs[::-1], want + want[::-1], whats):
+            # stage accounting: a job scored in at least one round after its
+            # admission; an alignment job (one sequence too: a batch writes
+            # its leaf row) spends its stage with one of its merge batches in
+            # flight or none
+            tt, js = srv.times(jid), srv.job_stats(jid)
+            if any(len(x) for x in j):
+                assert js["rounds"] >= 1 and 0 <= js["admit_s"] <= js["first_round_s"] <= tt[2] - tt[0]
+            if what == M.SERVE_MSA:
+                assert js["batches"] >= 1
+                start = tt[3] or tt[2]  # no tree for one sequence: from the scores
+                assert abs(js["busy_s"] + js["idle_s"] - (tt[4] - start)) < 1e-3
+            else:
+                assert js["batches"] == 0
             _check_job(srv, jid, j, w, what)
         st = srv.stats()
         assert st["waiting"] == 0 and st["running"] == 0 and st["arena_used"] == 0
