@@ -687,6 +687,11 @@ int launch_round(mst_server* S, int* launched) {
     a.carry_cap = 16384;
     const int64_t nit = ranges[c].second - ranges[c].first;
     if (k8c) {
+      // the score8 row buffers, at the first round that needs them (a region
+      // per class; ~2 GiB on 256 compute units)
+      if (!S->d_carry8)
+        CK(hipMalloc(&S->d_carry8, sizeof(int2) * (size_t)(kScore8MaxK - kScore8MinK + 1) * S->grid8 *
+                                       kScore8Waves * kServeScore8Cols));
       a.carry = S->d_carry8 + (int64_t)(k8c - kScore8MinK) * S->grid8 * kScore8Waves * kServeScore8Cols;
       a.carry_cap = kServeScore8Cols;
       launch_score8(k8c, false, a,
@@ -1173,9 +1178,6 @@ extern "C" int mst_server_create(mst_ctx* ctx, int64_t budget_bytes, mst_server*
   if ((e = hipMalloc(&S->d_job_out, sizeof(int64_t) * max_slots)) != hipSuccess) return fail("jobs", e);
   if ((e = hipMalloc(&S->d_job_ld, sizeof(int) * max_slots)) != hipSuccess) return fail("jobs", e);
   S->grid8 = ctx->num_cus * score8_groups_per_cu();
-  if ((e = hipMalloc(&S->d_carry8, sizeof(int2) * (size_t)(kScore8MaxK - kScore8MinK + 1) * S->grid8 *
-                                       kScore8Waves * kServeScore8Cols)) != hipSuccess)
-    return fail("carry8", e);
   S->grid6 = ctx->num_cus * score6_groups_per_cu();
   S->grid7 = ctx->num_cus * score7_groups_per_cu();
   // a row buffer per workgroup (two per workgroup of the two-pipeline form)
