@@ -13,6 +13,12 @@
 
 namespace mst {
 
+// Raises the calling wave's instruction-issue priority (s_setprio 3): where
+// a latency-bound kernel shares SIMDs with a throughput kernel's waves (the
+// served loop's merge batches and NJ beside its score rounds), its waves
+// issue first; on an otherwise idle device it changes nothing.
+__device__ __forceinline__ void issue_first() { __builtin_amdgcn_s_setprio(3); }
+
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
 // v_mov_b32_dpp wave_shr:1 with bound_ctrl off: lane 0 keeps `old`.

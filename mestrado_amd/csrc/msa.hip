@@ -106,6 +106,7 @@ constexpr int kTabChunk = 64, kTabChunkLong = 256;
 
 __global__ void mb_counts_kernel(const MergeDesc* __restrict__ d, const Prefix* __restrict__ pre,
                                  int count, int64_t total, uint8_t* __restrict__ scratch) {
+  issue_first();
   const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= total) return;
   const int i = find_merge(count, x, [&](int k) { return pre[k].col0; });
@@ -125,6 +126,7 @@ __global__ void mb_counts_kernel(const MergeDesc* __restrict__ d, const Prefix* 
 __global__ void mb_p_kernel(const MergeDesc* __restrict__ d, const Prefix* __restrict__ pre,
                             int count, int64_t total, const int8_t* __restrict__ S,
                             uint8_t* __restrict__ scratch) {
+  issue_first();
   const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= total) return;
   const int i = find_merge(count, x, [&](int k) { return pre[k].yrow0; });
@@ -141,6 +143,7 @@ __global__ void __launch_bounds__(256) mb_table_kernel(const MergeDesc* __restri
                                                        const Prefix* __restrict__ pre, int count,
                                                        int64_t total, int go_plus_ge,
                                                        uint8_t* __restrict__ scratch) {
+  issue_first();
   const int lane = threadIdx.x & 63;
   const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (w >= total) return;
@@ -216,6 +219,7 @@ __global__ void __launch_bounds__(kPathThreads) mb_path_kernel(const MergeDesc* 
                                                                const int32_t* __restrict__ tlens,
                                                                uint8_t* __restrict__ scratch,
                                                                int32_t* __restrict__ lens) {
+  issue_first();
   __shared__ int wx[kPathThreads / MST_WAVE], wy[kPathThreads / MST_WAVE];
   const int i = blockIdx.x;
   const MergeDesc m = d[i];
@@ -260,6 +264,7 @@ __global__ void __launch_bounds__(kPathThreads) mb_path_kernel(const MergeDesc* 
 // One workgroup per output row: X's kX rows then Y's kY rows, L columns.
 __global__ void mb_rows_kernel(const MergeDesc* __restrict__ d, int count,
                                const uint8_t* __restrict__ scratch, const int32_t* __restrict__ lens) {
+  issue_first();
   const int64_t r = blockIdx.x;
   const int i = find_merge(count, r, [&](int k) { return d[k].row0; });
   const MergeDesc m = d[i];
@@ -277,6 +282,7 @@ __global__ void mb_rows_kernel(const MergeDesc* __restrict__ d, int count,
 
 // One workgroup per leaf: forward codes from the reversed code block.
 __global__ void mb_leaf_kernel(const int8_t* __restrict__ codes, const LeafDesc* __restrict__ l) {
+  issue_first();
   const LeafDesc q = l[blockIdx.x];
   for (int t = threadIdx.x; t < q.len; t += blockDim.x) q.dst[t] = (uint8_t)codes[q.src + q.len - 1 - t];
 }

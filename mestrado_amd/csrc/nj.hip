@@ -663,6 +663,7 @@ __global__ void __launch_bounds__(NJ_THREADS) nj_kernel(NjArgs a) { nj_run(a); }
 
 // Independent matrices (jobs of the served form), one workgroup each.
 __global__ void __launch_bounds__(NJ_THREADS) nj_batch_kernel(const NjArgs* __restrict__ args) {
+  issue_first();
   nj_run(args[blockIdx.x]);
 }
 
@@ -672,12 +673,14 @@ hipError_t launch_nj(const NjArgs& a, hipStream_t st) {
 }
 
 __global__ void scores_to_float_kernel(const int32_t* __restrict__ s, float* __restrict__ d, int64_t n) {
+  issue_first();
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) d[i] = (float)s[i];
 }
 
 // act and ids (2n each): 0..n-1 in the first half, zeros in the second
 __global__ void nj_init_kernel(int* __restrict__ act, int* __restrict__ ids, int n) {
+  issue_first();
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < 2 * n) {
     act[i] = i < n ? i : 0;

@@ -244,6 +244,7 @@ __device__ void arun_tile(ALane& L, const int8_t* X, int n, int c0, int c1, cons
 
 template <int SRC>
 __global__ void __launch_bounds__(MST_WAVE) align_tile_kernel(TileArgs a) {
+  issue_first();
   __shared__ int8_t S_lds[MST_LETTERS * MST_LETTERS];
   __shared__ __attribute__((aligned(16))) uint8_t prof[SRC == SRC_SEQ ? MST_LETTERS * MST_WAVE * AK : 16];
   const int lane = lane_id();
@@ -483,6 +484,7 @@ __device__ void arun_strip_pipe(ALane& L, const int8_t* X, int n, const uint8_t*
 // the same sweep can reach wave W-1).
 template <int SRC, int W>
 __global__ void __launch_bounds__(MST_WAVE* W) align_pipe_kernel(AlignArgs a) {
+  issue_first();
   constexpr int SLOT = (SRC == SRC_SEQ) ? MST_LETTERS * MST_WAVE * AK : 16;
   __shared__ int8_t S_lds[MST_LETTERS * MST_LETTERS];
   __shared__ __attribute__((aligned(16))) uint8_t prof[W * SLOT];
@@ -605,6 +607,7 @@ __global__ void __launch_bounds__(MST_WAVE* W) align_pipe_kernel(AlignArgs a) {
 
 template <int SRC>
 __global__ void __launch_bounds__(MST_WAVE) align_kernel(AlignArgs a) {
+  issue_first();
   __shared__ int8_t S_lds[MST_LETTERS * MST_LETTERS];
   // the profile only for scores from sequences; a score table needs none
   __shared__ __attribute__((aligned(16))) uint8_t prof[SRC == SRC_SEQ ? MST_LETTERS * MST_WAVE * AK : 16];
@@ -716,6 +719,7 @@ __device__ __forceinline__ uint32_t planes_to_nibbles(uint32_t w) {
 }
 
 __global__ void __launch_bounds__(MST_WAVE) traceback_kernel(TraceArgs a) {
+  issue_first();
   const int lane = lane_id();
   for (int64_t j = (int64_t)blockIdx.x + a.first; j < a.last; j += gridDim.x) {
     const AlignJob job = a.jobs[j];

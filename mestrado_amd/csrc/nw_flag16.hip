@@ -426,6 +426,7 @@ __device__ __forceinline__ void f16_item(const F16Args& a, const F16Item item, i
 // strips' profiles in LDS (12 KB: twelve waves per compute unit).
 template <bool PROF>
 __global__ void __launch_bounds__(MST_WAVE) flag16_kernel(F16Args a) {
+  issue_first();
   __shared__ uint32_t prof[PROF ? 2 * MST_LETTERS * MST_WAVE : 1];
   __shared__ int8_t Sn[PROF ? MST_LETTERS * MST_LETTERS : 1];
   if (PROF)
@@ -500,6 +501,7 @@ __global__ void __launch_bounds__(256) seq_table_kernel(const AlignJob* __restri
                                                         const int8_t* __restrict__ seqs,
                                                         const int8_t* __restrict__ subst, int bonus,
                                                         uint8_t* __restrict__ tab) {
+  issue_first();
   __shared__ int8_t S_lds[MST_LETTERS * MST_LETTERS];
   __shared__ uint2 prof[MST_LETTERS][MST_WAVE];
   for (int i = threadIdx.x; i < MST_LETTERS * MST_LETTERS; i += blockDim.x) S_lds[i] = subst[i];
