@@ -217,7 +217,10 @@ int mst_server_stats(mst_server* srv, int* waiting, int* running, int* inflight,
  * pipeline), "slack" (a
  * job's deadline
  * is its submission plus slack times its estimated run time, default 2;
- * jobs go earliest deadline first), "rate" (initial throughput estimate in
+ * jobs go earliest deadline first), "est_floor" (seconds: the least
+ * estimated run time in that deadline, so jobs below it share one deadline
+ * horizon and go in arrival order; default 0), "deadline_add" (seconds added
+ * to every deadline, default 0), "rate" (initial throughput estimate in
  * cells/s, for those estimates), "round_cells_shared" and "score_share"
  * (while jobs are in their NJ or alignment stage: cells per round, default
  * 2e10, and the share of the score kernels' grid, default 1; lower values let

@@ -21,7 +21,8 @@ the bucket `default` names; apply_for(ctx, lengths) picks the bucket of a set.
 The served search covers the server settings of SERVE_KEYS (item sizes of
 the score rounds, the kernel of the rows above 64 k1max and the multi-sweep
 kernel's form, and the score rounds' size and grid share while jobs are in
-their NJ or alignment stage) and keeps the best under "served"; a Server
+their NJ or alignment stage, and the jobs' deadlines: slack x the estimated
+run time, with a floor on the estimate) and keeps the best under "served"; a Server
 created on a tuned context applies it.
 """
 import argparse
@@ -123,17 +124,23 @@ def tune(ctx, seqs, reps=2, log=None):
 
 # Settings of the served form (mst_server_set) the served search covers; the
 # table keeps the best under "served".
-SERVE_KEYS = ("chunk", "chunk6", "s6p2", "k8", "score_share", "round_cells_shared")
+SERVE_KEYS = ("chunk", "chunk6", "s6p2", "k8", "score_share", "round_cells_shared", "slack", "est_floor")
 
 
 def serve_candidates():
-    # the long rows on the multi-sweep kernel (k8 0, its one-pipeline form,
-    # the earlier table's choice), or on the strips-in-turn kernel (k8 1)
-    # with the score rounds whole or, while jobs are in their NJ or alignment
-    # stage, halved and on part of the device; item sizes of both
-    yield {"chunk": 12, "chunk6": 8, "s6p2": 0, "k8": 0, "score_share": 1.0, "round_cells_shared": 2e10}
-    for chunk, chunk6, (share, rcs) in itertools.product((8, 12), (4, 8), ((1.0, 2e10), (0.75, 1e10), (0.5, 1e10))):
-        yield {"chunk": chunk, "chunk6": chunk6, "s6p2": 0, "k8": 1, "score_share": share, "round_cells_shared": rcs}
+    # the long rows on the multi-sweep kernel (k8 0, its one-pipeline form)
+    # under deadlines of slack x the estimated run time, the estimate with
+    # or without a floor of 0.1 s; or on the strips-in-turn kernel (k8 1),
+    # its four best item sizes and round shares (the score rounds whole or,
+    # while jobs are in their NJ or alignment stage, on part of the device)
+    # under the slower deadlines
+    base = {"chunk": 12, "chunk6": 8, "s6p2": 0, "k8": 0, "score_share": 1.0, "round_cells_shared": 2e10}
+    for slack, floor in ((2.0, 0.0), (3.0, 0.0), (2.0, 0.1), (3.0, 0.1)):
+        yield dict(base, slack=slack, est_floor=floor)
+    for chunk, chunk6, share, rcs in ((8, 4, 0.75, 1e10), (8, 8, 1.0, 2e10), (12, 4, 0.5, 1e10),
+                                      (8, 4, 0.5, 1e10)):
+        yield {"chunk": chunk, "chunk6": chunk6, "s6p2": 0, "k8": 1, "score_share": share,
+               "round_cells_shared": rcs, "slack": 3.0, "est_floor": 0.1}
 
 
 def tune_served(ctx, jobs, log=None):

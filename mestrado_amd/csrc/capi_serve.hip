@@ -230,6 +230,7 @@ struct mst_server {
   int persistent = 1;                // persistent score launches (else one workgroup per item)
   double slack = 2.0;          // deadline = submission + slack x estimated run time
   double deadline_add = 0.0;   //   + this many seconds
+  double est_floor = 0.0;      // least estimated run time in that deadline (seconds)
   double t_retire = 0;         // when the last round was seen finished
   int chunk = 12;
   int chunk6 = 8;  // items of the multi-sweep kernel (0: chunk)
@@ -513,6 +514,9 @@ int admit(mst_server* S, Job& J) {
   const double nn = (double)N / 512.0;
   if ((J.what & MST_SERVE_TREE) == MST_SERVE_TREE && N >= 2) est += 0.034 * nn * nn;
   if ((J.what & MST_SERVE_MSA) == MST_SERVE_MSA) est += 60e-6 * N;
+  // a floor on the estimate: jobs below it share one deadline horizon, so a
+  // large job is not passed by every smaller one that arrives after it
+  est = std::max(est, S->est_floor);
   J.deadline = J.t_submit + S->slack * est + S->deadline_add;
   J.ascii.clear();
   J.ascii.shrink_to_fit();
@@ -1527,6 +1531,10 @@ extern "C" int mst_server_set(mst_server* S, const char* key, double value) {
   }
   if (!strcmp(key, "rounds") && value >= 1 && value <= mst_server::kBufs) {
     S->max_rounds = (int)value;
+    return MST_OK;
+  }
+  if (!strcmp(key, "est_floor") && value >= 0 && value <= 100) {
+    S->est_floor = value;
     return MST_OK;
   }
   if (!strcmp(key, "shared_backlog") && value >= 0) {
