@@ -119,6 +119,9 @@ def test_cancel_and_misuse(ctx):
             srv.job(a)  # forgotten after its result was taken
         with pytest.raises(M.MestradoError):
             srv.set("no-such-key", 1)
+        with pytest.raises(M.MestradoError):
+            srv.set("est_floor", -1)  # a negative floor on the estimate
+        srv.set("est_floor", 0.05)
 
 
 JOIN_FIELDS = ("node", "left", "right", "d", "Ti", "Tj", "n", "q", "len_left", "len_right")
@@ -155,13 +158,16 @@ def _check_job(srv, jid, seqs, want, what):
     assert (srv.result(jid) == D).all()
 
 
-def test_msa_jobs_alone_and_in_mix(ctx):
+@pytest.mark.parametrize("deadlines", [(), (("slack", 3.0), ("est_floor", 0.1))])
+def test_msa_jobs_alone_and_in_mix(ctx, deadlines):
     """Scores, NJ tree and alignment of every job equal the one-job path's,
     alone and with jobs of every stage sharing rounds, NJ launches and merge
-    batches."""
+    batches; the deadline settings change only the order jobs are served in."""
     jobs = _msa_jobs(5)
     want = [_offline(ctx, j) for j in jobs]
     with M.Server(ctx, 1 << 30) as srv:
+        for k, v in deadlines:
+            srv.set(k, v)
         for j, w in zip(jobs, want):
             jid = srv.submit(j, M.SERVE_MSA)
             srv.drain()
