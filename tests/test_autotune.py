@@ -33,6 +33,20 @@ def test_committed_tables_are_well_formed():
         assert t["default"] in t["buckets"]
         for entry in t["buckets"].values():
             assert set(entry) - {"ms", "workload"} <= set(T.KEYS) and "chunk" in entry
+        if "served" in t:
+            extra = {"ratio", "p99_ms", "offline_jobs_per_s", "workload"}
+            assert set(t["served"]) - extra <= set(T.SERVE_KEYS)
+
+
+def test_serve_keys_are_server_settings():
+    # every setting the served search stores is one mst_server_set documents
+    # (the call itself needs a device)
+    here = os.path.dirname(os.path.abspath(__file__))
+    head = open(os.path.join(here, "..", "include", "mestrado.h")).read()
+    doc = head[:head.index("int mst_server_set(")]
+    doc = doc[doc.rindex("/*"):]
+    for k in T.SERVE_KEYS:
+        assert '"%s"' % k in doc, k
 
 
 def test_served_settings_round_trip(tmp_path, monkeypatch):
